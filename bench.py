@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Benchmark: Mpaths/s of plain wavefront PT (pt2 semantics) on one MI355X.
+
+Workload (the at-scale configuration): seeded procedural scene of 1,008,288
+triangles (displaced 512x512 heightfield + 200 geodesic icospheres), GGX
+metal / glass / mirror / Lambert materials, procedural 1024x512 HDR sky with a
+sun disc, pinhole 1920x1080, max depth 16, 4 wavelengths per path, FP32
+tracing/shading, FP64 framebuffer.  One step = one sample per pixel over the
+full frame (2,073,600 camera paths).  Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpaths/s plain PT, and relMSE at equal time (plain vs learned guiding), 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+TRACE_BYTES_PER_RAY = 48     # ray origin+direction read (32 B) + hit record written (16 B)
+SHADE_BYTES_PER_PATH = 224   # hit 16 + path state 48 + material 32 read; state 48 + ray 32 + shadow 48 written
+SHADOW_BYTES_PER_RAY = 36 + 32  # shadow ray 32 + contribution 16 + flags, radiance read/write
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=16)
+    ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on the host (0 = skip)")
+    ap.add_argument("--cpu-sample-rows", type=int, default=0, help="rows of the frame the CPU sample renders (0 = auto)")
+    ap.add_argument("--wavefront", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(scene, width, height, depth, rows):
+    """Oracle (FP64 C++ restatement of pt2, all host threads) on a bounded sample: `rows` rows x full width x 1 spp."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from naivetrace_amd import abi
+    oracle_lib.build()
+    threads = os.cpu_count() or 1
+    # A width x rows image with the same horizontal field of view is exactly the central band of
+    # rows of the full frame (pixel footprint 2 tan(fov/2)/width in both directions).
+    p = abi.default_params(width, rows, max_depth=depth, seed=5)
+    crop = scene
+    t0 = time.time()
+    oracle_lib.render(crop, p, 1, threads=threads, with_sq=False)
+    dt = time.time() - t0
+    paths = width * rows
+    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} rows x {width} px x 1 spp of the same scene ({paths} paths, {dt:.1f} s, oracle BVH build excluded)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is designed, not built)"}))
+        sys.exit(2)
+    from naivetrace_amd import abi, configs
+    from naivetrace_amd.renderer import GpuRenderer
+
+    t0 = time.time()
+    scene = configs.terrain_instances(args.width, args.height)
+    gen_s = time.time() - t0
+    p = abi.default_params(args.width, args.height, max_depth=args.depth, seed=1)
+    p.wavefront_paths = args.wavefront
+    r = GpuRenderer(scene, p)
+    st0 = r.stats()
+    for _ in range(args.warmup):
+        r.render_spp(1)
+    r.reset()
+    r.enable_kernel_timing(False)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render_spp(1)
+    t2 = time.perf_counter()
+    st = r.stats()
+    dt = t2 - t1
+    paths = args.width * args.height * args.steps
+    value = paths / dt / 1e6
+
+    # Kernel timing (HIP events on the library's own stream) over a separate timed pass.
+    r.enable_kernel_timing(True)
+    c0 = r.stats()
+    nt = 2
+    for _ in range(nt):
+        r.render_spp(1)
+    c1 = r.stats()
+    times = {k: r.kernel_time(k) for k in ("raygen", "trace", "shade", "shadow")}
+    rays_closest = c1["rays_closest"] - c0["rays_closest"]
+    rays_shadow = c1["rays_shadow"] - c0["rays_shadow"]
+    paths_t = args.width * args.height * nt
+    dom = max(times, key=lambda k: times[k][0])
+    ms_tot, launches = times[dom]
+    avg_ms = ms_tot / max(launches, 1)
+    if dom == "trace":
+        alg_bytes = TRACE_BYTES_PER_RAY * rays_closest / max(launches, 1)
+    elif dom == "shadow":
+        alg_bytes = SHADOW_BYTES_PER_RAY * rays_shadow / max(launches, 1)
+    else:
+        alg_bytes = SHADE_BYTES_PER_PATH * rays_closest / max(launches, 1)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": f"k_{dom}",
+                "avg_launch_ms": round(avg_ms, 4),
+                "bytes_per_unit": {"trace": TRACE_BYTES_PER_RAY, "shade": SHADE_BYTES_PER_PATH, "shadow": SHADOW_BYTES_PER_RAY}[dom]}
+    ktimes = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in times.items()}
+    r.close()
+
+    cpu = None
+    if args.cpu_baseline:
+        rows = args.cpu_sample_rows or 64
+        try:
+            cpu = cpu_baseline(scene, args.width, args.height, args.depth, rows)
+        except Exception as e:  # reported, not fatal for the GPU number
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "terrain_instances: 1,008,288 tris (512x512 heightfield + 200 icospheres f=11), GGX/glass/mirror/"
+                               "Lambert, 1024x512 HDR sky + sun disc, pinhole, 4 wavelengths/path",
+                   "width": args.width, "height": args.height, "spp_per_step": 1, "max_depth": args.depth,
+                   "triangles": scene.num_triangles, "wavefront_paths": st["wavefront_paths"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": {"rays_closest_per_path": round(rays_closest / paths_t, 3), "rays_shadow_per_path": round(rays_shadow / paths_t, 3),
+                   "mrays_per_s": round((rays_closest + rays_shadow) / (sum(v[0] for v in times.values()) * 1e-3) / 1e6, 1),
+                   "kernel_ms": ktimes, "bvh_build_ms": round(st0["bvh_build_ms"], 1), "scene_gen_s": round(gen_s, 2),
+                   "upload_ms": round(st0["upload_ms"], 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,315 @@
+/*
+ * ttgpu.h - C ABI of the MI355X wavefront path tracer.
+ *
+ * This is the drop-in boundary for ToyTrace's `pt2` (forward path tracing with
+ * next-event estimation and spectral null-path tracking) and `ptg` (guided)
+ * integrators. In the reference those sit behind `class RenderingAlgo`
+ * (src/renderingalgorithms_interface.hxx:13-30), created by
+ * `RenderAlgorithmFactory` (src/renderingalgorithms.cxx:27-42) from a `Scene`
+ * (src/scene.hxx:134-315) and `RenderingParameters` (src/scene.hxx:24-43).
+ *
+ * Across this ABI the scene is a flat, caller-owned description: float
+ * vertices, u32 triangle indices, a material id per primitive and small tables
+ * of shaders, media, emitters and lights with 36-bin spectra (370-730 nm,
+ * src/spectral.hxx:270-274).  Nothing but plain C types crosses the boundary.
+ * Every entry point returns a status code (TTGPU_OK == 0); the message of the
+ * last failure on a handle is available from ttgpu_last_error().  No C++
+ * exception escapes.
+ *
+ * Thread-safety: ttgpu_request_interrupt() and ttgpu_request_stop() may be
+ * called from any thread at any time, also before ttgpu_run() (mirrors
+ * RequestInterrupt/RequestFullStop, src/renderingalgorithms_pathtracing2.cxx:216-227).
+ * All other calls on one handle must come from one thread at a time.
+ */
+#ifndef TTGPU_H
+#define TTGPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TTGPU_NBINS 36          /* spectral bins, src/spectral.hxx:274 */
+#define TTGPU_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  TTGPU_OK = 0,
+  TTGPU_ERR_INVALID_ARGUMENT = 1,  /* std::invalid_argument in the reference */
+  TTGPU_ERR_RUNTIME = 2,           /* std::runtime_error in the reference */
+  TTGPU_ERR_DEVICE = 3,            /* HIP error */
+  TTGPU_ERR_INTERRUPTED = 4,       /* a pass was cut short by interrupt/stop */
+  TTGPU_ERR_OUT_OF_MEMORY = 5
+};
+
+/* ---- shaders (src/shader.cxx:251-1081) ------------------------------- */
+enum ttgpu_shader_kind {
+  TTGPU_SHADER_DIFFUSE = 0,             /* DiffuseShader, shader.cxx:251-318 */
+  TTGPU_SHADER_MIRROR = 1,              /* SpecularReflectiveShader, :322-366 */
+  TTGPU_SHADER_DIELECTRIC = 2,          /* SpecularTransmissiveDielectricShader, :378-630 */
+  TTGPU_SHADER_BECKMANN_METAL = 3,      /* MicrofacetShader (Beckmann, V-cavity VNDF), :634-707 */
+  TTGPU_SHADER_GGX_METAL = 4,           /* new: GGX NDF, Smith G2, visible-normal sampling */
+  TTGPU_SHADER_INVISIBLE = 5,           /* InvisibleShader, :948-972 */
+  TTGPU_SHADER_GLOSSY_DIELECTRIC = 6,   /* GlossyTransmissiveDielectricShader, :710-787 */
+  TTGPU_SHADER_DENSE_DIELECTRIC = 7     /* SpecularDenseDielectricShader, :977-1081 */
+};
+
+typedef struct ttgpu_shader {
+  int32_t kind;            /* ttgpu_shader_kind */
+  float alpha;             /* roughness (Beckmann/GGX/glossy dielectric) */
+  float alpha_min;         /* glossy dielectric lower roughness (== alpha without texture) */
+  float ior_ratio;         /* inside ior / outside ior */
+  float ior_lambda_coeff;  /* dispersion d(ior)/d(lambda nm); 0 = none (abbe_number in YAML) */
+  float specular;          /* dense dielectric specular reflectivity */
+  float reflectance[TTGPU_NBINS]; /* albedo / glossy reflectance spectrum */
+} ttgpu_shader;
+
+/* ---- phase functions (src/phasefunctions.cxx) -------------------------- */
+enum ttgpu_phase_kind {
+  TTGPU_PHASE_UNIFORM = 0,          /* phasefunctions.cxx:23-37 */
+  TTGPU_PHASE_HENYEY_GREENSTEIN = 1,/* :85-138 */
+  TTGPU_PHASE_RAYLEIGH = 2          /* :41-80 */
+};
+
+/* ---- media (src/shader.cxx:1162-1316, src/atmosphere.cxx) ------------ */
+enum ttgpu_medium_kind {
+  TTGPU_MEDIUM_VACUUM = 0,
+  TTGPU_MEDIUM_HOMOGENEOUS = 1,
+  TTGPU_MEDIUM_ATMOSPHERE = 2       /* exponential Rayleigh + Mie shells, FP64 geometry */
+};
+
+typedef struct ttgpu_medium {
+  int32_t kind;            /* ttgpu_medium_kind */
+  int32_t priority;        /* MediumTracker priority, src/rendering_util.hxx:63-90 */
+  int32_t phase_kind;      /* homogeneous: ttgpu_phase_kind */
+  float phase_g;           /* HG asymmetry */
+  float sigma_s[TTGPU_NBINS]; /* 1/length unit of the scene */
+  float sigma_a[TTGPU_NBINS];
+  /* Atmosphere (kind == TTGPU_MEDIUM_ATMOSPHERE), src/atmosphere.hxx:95-123,
+   * src/atmosphere.cxx:132-200.  Lengths in scene units; coefficients are
+   * sea-level values per scene unit.  Rayleigh scatters only; Mie scatters
+   * and absorbs and uses an HG phase with mie_g. */
+  double planet_center[3];
+  double planet_radius;
+  double atmosphere_height;     /* outer shell radius = planet_radius + height */
+  double rayleigh_scale_height;
+  double mie_scale_height;
+  float mie_g;
+  float rayleigh_sigma_s[TTGPU_NBINS];
+  float mie_sigma_s[TTGPU_NBINS];
+  float mie_sigma_a[TTGPU_NBINS];
+} ttgpu_medium;
+
+/* ---- materials (src/scene.hxx:46-72) ----------------------------------- */
+typedef struct ttgpu_material {
+  int32_t shader;          /* index into shaders, -1 = none (volume boundary only) */
+  int32_t medium;          /* medium inside (against the geometric normal), -1 = none */
+  int32_t outer_medium;    /* medium outside, -1 = none */
+  int32_t emitter;         /* index into area_emitters, -1 = none */
+} ttgpu_material;
+
+/* ---- area emitters (src/light.hxx:72-141) ------------------------------ */
+enum ttgpu_area_kind {
+  TTGPU_AREA_UNIFORM = 0,   /* UniformAreaLight: L = spectrum / (2 pi), one-sided */
+  TTGPU_AREA_PARALLEL = 1   /* ParallelAreaLight (delta in direction) */
+};
+
+typedef struct ttgpu_area_emitter {
+  int32_t kind;
+  float spectrum[TTGPU_NBINS];
+} ttgpu_area_emitter;
+
+/* ---- environment / infinite lights (src/light.hxx:148-288, light.cxx:88-189) */
+enum ttgpu_env_kind {
+  TTGPU_ENV_SUN = 0,         /* Sun: cone of opening_angle_deg, built-in solar spectrum x total_power */
+  TTGPU_ENV_DOME = 1,        /* DistantDomeLight: uniform upper hemisphere around dir */
+  TTGPU_ENV_DIRECTIONAL = 2, /* DistantDirectionalLight: delta direction */
+  TTGPU_ENV_MAP = 3          /* EnvMapLight: float RGB lat-long texture, row/col CDF sampling */
+};
+
+typedef struct ttgpu_env_light {
+  int32_t kind;
+  double dir[3];            /* SUN/DIRECTIONAL: direction the light travels (YAML "direction");
+                               DOME/MAP: up direction */
+  double opening_angle_deg; /* SUN */
+  double total_power;       /* SUN */
+  float spectrum[TTGPU_NBINS]; /* DOME / DIRECTIONAL */
+  int32_t tex_width, tex_height; /* MAP: row-major RGB float texture, row 0 = top (v=0) */
+  const float *texture_rgb;
+} ttgpu_env_light;
+
+typedef struct ttgpu_point_light {
+  double pos[3];
+  float spectrum[TTGPU_NBINS]; /* total power; radiant intensity = spectrum / (4 pi) */
+} ttgpu_point_light;
+
+/* ---- camera (src/camera.hxx) ------------------------------------------ */
+enum ttgpu_camera_kind {
+  TTGPU_CAMERA_PINHOLE = 0,   /* PerspectiveCamera, camera.hxx:62-156 */
+  TTGPU_CAMERA_FISHEYE = 1    /* FisheyeHemisphereCamera, camera.hxx:159-232 */
+};
+
+typedef struct ttgpu_camera {
+  int32_t kind;
+  double pos[3];
+  double dir[3];
+  double up[3];
+  double fov_deg;             /* pinhole: full horizontal angle */
+} ttgpu_camera;
+
+/* ---- scene ------------------------------------------------------------- */
+typedef struct ttgpu_scene {
+  int64_t num_vertices;
+  const float *vertices;        /* 3 x num_vertices */
+  const float *normals;         /* 3 x num_vertices or NULL (flat shading) */
+  int64_t num_triangles;
+  const uint32_t *indices;      /* 3 x num_triangles */
+  const int32_t *tri_material;  /* num_triangles */
+  int64_t num_spheres;
+  const float *spheres;         /* 4 x num_spheres: centre xyz, radius */
+  const int32_t *sphere_material;
+  int32_t num_shaders;
+  const ttgpu_shader *shaders;
+  int32_t num_media;
+  const ttgpu_medium *media;
+  int32_t empty_space_medium;   /* index of the medium filling space, -1 = vacuum */
+  int32_t num_materials;
+  const ttgpu_material *materials;
+  int32_t num_area_emitters;
+  const ttgpu_area_emitter *area_emitters;
+  int32_t num_env_lights;
+  const ttgpu_env_light *env_lights;
+  int32_t num_point_lights;
+  const ttgpu_point_light *point_lights;
+  ttgpu_camera camera;
+} ttgpu_scene;
+
+/* ---- parameters (src/scene.hxx:24-43) ---------------------------------- */
+enum ttgpu_algo { TTGPU_ALGO_PT2 = 0, TTGPU_ALGO_PTG = 1 };
+enum ttgpu_sample_mode {     /* --pt-sample-mode */
+  TTGPU_SAMPLE_BOTH = 0, TTGPU_SAMPLE_BSDF = 1, TTGPU_SAMPLE_LIGHTS = 2
+};
+enum ttgpu_mis_mode {
+  /* pt2's NEE MIS weight uses the light pdf without the light-selection
+   * probability (src/renderingalgorithms_pathtracing2.cxx:489-501) while the
+   * emission weight includes it (:641-644).  REFERENCE keeps that; CONSISTENT
+   * multiplies the selection probability into the NEE side as well. */
+  TTGPU_MIS_REFERENCE = 0, TTGPU_MIS_CONSISTENT = 1
+};
+enum ttgpu_guide_kind { TTGPU_GUIDE_NONE = 0, TTGPU_GUIDE_VMF_TREE = 1, TTGPU_GUIDE_LEARNED = 2 };
+
+typedef struct ttgpu_params {
+  int32_t algo;                 /* ttgpu_algo */
+  int32_t width, height;
+  int32_t max_ray_depth;        /* --rd, RayTermination max_node_count */
+  int32_t max_samples_per_pixel;/* --spp; <= 0: unbounded */
+  int32_t sample_mode;          /* ttgpu_sample_mode */
+  int32_t mis_mode;             /* ttgpu_mis_mode */
+  int32_t num_wavelengths;      /* wavelengths per path; 4 on the GPU */
+  uint64_t seed;                /* PCG32 initial state; stream = global sample index */
+  int64_t wavefront_paths;      /* paths in flight per wavefront; 0 = automatic */
+  int32_t linear_output;        /* --linear-out for GenerateImage */
+  int32_t num_threads;          /* CPU threads (oracle / host side) */
+  /* guiding (ptg) */
+  int32_t guide_kind;           /* ttgpu_guide_kind */
+  int32_t guide_max_spp;        /* training schedule, --guide-max-spp */
+  float guide_mix_bsdf;         /* one-sample MIS: probability to sample the BSDF (0.5) */
+  int32_t reserved[8];
+} ttgpu_params;
+
+typedef struct ttgpu_stats {
+  uint64_t paths_traced;        /* camera paths completed */
+  uint64_t rays_closest;        /* closest-hit queries */
+  uint64_t rays_shadow;         /* occlusion queries */
+  uint64_t total_spp;           /* samples per pixel rendered so far */
+  double render_ms;             /* device time of the last ttgpu_run/ttgpu_render_spp */
+  double bvh_build_ms;          /* host BVH build */
+  double upload_ms;
+  uint64_t bvh_nodes;
+  uint64_t scene_bytes;         /* device bytes of BVH + primitives */
+  uint64_t state_bytes;         /* device bytes of path state queues */
+  int32_t passes;
+  int32_t wavefront_paths;
+} ttgpu_stats;
+
+typedef struct ttgpu_handle ttgpu_handle;
+typedef void (*ttgpu_pass_callback)(void *user, int32_t is_complete_pass);
+
+/* Lifecycle.  Replaces PathTracingAlgo2's ctor + InitializeScene
+ * (src/renderingalgorithms_pathtracing2.cxx:242-263).  Copies everything it
+ * needs out of `scene` (the caller may free it afterwards), builds the BVH on
+ * the host and uploads scene and path-state queues to the current HIP device. */
+int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_handle **out);
+void ttgpu_destroy(ttgpu_handle *h);
+const char *ttgpu_last_error(const ttgpu_handle *h);
+/* Message of the last ttgpu_create failure (no handle exists then). */
+const char *ttgpu_create_error(void);
+
+/* PathTracingAlgo2::Run (src/renderingalgorithms_pathtracing2.cxx:269-295):
+ * passes with doubling spp (1,2,4,..,1024 per pass, capped so the total does
+ * not exceed max_samples_per_pixel; src/renderingalgorithms_simplebase.hxx:34-62)
+ * until the budget is used or a stop is requested.  max_passes <= 0: no cap.
+ * The callback (if set) fires after each pass with is_complete_pass = 1, or 0
+ * when the pass was interrupted. */
+int ttgpu_run(ttgpu_handle *h, int32_t max_passes);
+/* One pass of exactly `spp` samples per pixel (the benchmark step). */
+int ttgpu_render_spp(ttgpu_handle *h, int32_t spp);
+int ttgpu_request_interrupt(ttgpu_handle *h);
+int ttgpu_request_stop(ttgpu_handle *h);
+int ttgpu_set_pass_callback(ttgpu_handle *h, ttgpu_pass_callback cb, void *user);
+/* Clear framebuffer and sample counters (not the RNG stream position). */
+int ttgpu_reset_framebuffer(ttgpu_handle *h);
+
+/* Framebuffer: linear RGB sums (double, width*height*3, row-major, row 0 =
+ * first image row of the camera) and the samples per pixel accumulated.
+ * rgb_sq (optional, may be NULL): per-pixel sums of squared per-path RGB
+ * estimates, for variance estimates. */
+int ttgpu_read_framebuffer(ttgpu_handle *h, double *rgb, double *rgb_sq, uint64_t *spp);
+/* GenerateImage (src/renderingalgorithms_pathtracing2.cxx:298-309 with
+ * src/renderbuffer.hxx:99-125): 8-bit RGB, rows flipped, clamped, sRGB unless
+ * params.linear_output. */
+int ttgpu_generate_image(ttgpu_handle *h, uint8_t *rgb8);
+int ttgpu_stats_get(ttgpu_handle *h, ttgpu_stats *out);
+
+/* Timing of the most recent run measured with HIP events on the stream the
+ * kernels run on: per-kernel-class total milliseconds and launch counts.
+ * names: "trace", "shade", "shadow", "raygen", "media", "guide". */
+int ttgpu_kernel_time(ttgpu_handle *h, const char *kernel_class, double *total_ms, int64_t *launches);
+int ttgpu_enable_kernel_timing(ttgpu_handle *h, int32_t enable);
+
+/* ---- scene queries (replace EmbreeAccelerator, src/embreeaccelerator.cxx:87-304) */
+/* Closest hit for `count` rays; t = +inf and prim = -1 when nothing is hit.
+ * prim indexes triangles first, then spheres (num_triangles + sphere index). */
+int ttgpu_trace_closest(ttgpu_handle *h, int64_t count, const float *org, const float *dir,
+                        const float *tmax, float *t, int32_t *prim, float *u, float *v);
+int ttgpu_trace_occluded(ttgpu_handle *h, int64_t count, const float *org, const float *dir,
+                         const float *tmax, int32_t *occluded);
+
+/* ---- sampler entry points for the statistical tests ------------------------
+ * Run the GPU BSDF sampler of `shader` at a surface with geometric normal ng
+ * and shading normal ns (both already oriented towards wi), for `count`
+ * independent samples.  Mirrors Shader::SampleBSDF / EvaluateBSDF
+ * (src/shader.hxx:58-59).  bins: the 4 wavelength bins of the path.
+ * out_dir: 3*count, out_value: 4*count, out_pdf: count (negative = delta,
+ * src/sampler.hxx:552-590).  roughness: ShaderQuery::minimum_roughness. */
+int ttgpu_sample_shader(const ttgpu_shader *shader, const float wi[3], const float ng[3],
+                        const float ns[3], const int32_t bins[4], float roughness,
+                        uint64_t seed, int64_t count,
+                        float *out_dir, float *out_value, float *out_pdf);
+int ttgpu_eval_shader(const ttgpu_shader *shader, const float wi[3], const float ng[3],
+                      const float ns[3], const int32_t bins[4], float roughness,
+                      int64_t count, const float *dirs, float *out_value, float *out_pdf);
+/* Phase function sampling (src/phasefunctions.cxx): wi = reverse incident dir. */
+int ttgpu_sample_phase(int32_t phase_kind, float g, const float wi[3], uint64_t seed,
+                       int64_t count, float *out_dir, float *out_value, float *out_pdf);
+
+/* Device / build information string ("gfx950, 256 CUs, ..."). */
+const char *ttgpu_device_info(void);
+int ttgpu_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TTGPU_H */

@@ -1,0 +1,186 @@
+"""ctypes mirror of include/ttgpu.h and the loader of the HIP library.
+
+The structures here are field-for-field copies of the C declarations; the
+scene description they carry is the flattened form of ToyTrace's `Scene`
+(src/scene.hxx:134-315) and `RenderingParameters` (src/scene.hxx:24-43).
+The library is loaded from the package directory (built in-tree by
+`__graft_entry__.build()`); a missing library is an error, there is no
+fallback path.
+"""
+import ctypes as C
+import os
+
+NBINS = 36
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libttgpu.so")
+
+# enums (mirror include/ttgpu.h)
+SHADER_DIFFUSE, SHADER_MIRROR, SHADER_DIELECTRIC, SHADER_BECKMANN_METAL, SHADER_GGX_METAL, \
+    SHADER_INVISIBLE, SHADER_GLOSSY_DIELECTRIC, SHADER_DENSE_DIELECTRIC = range(8)
+PHASE_UNIFORM, PHASE_HENYEY_GREENSTEIN, PHASE_RAYLEIGH = range(3)
+MEDIUM_VACUUM, MEDIUM_HOMOGENEOUS, MEDIUM_ATMOSPHERE = range(3)
+AREA_UNIFORM, AREA_PARALLEL = range(2)
+ENV_SUN, ENV_DOME, ENV_DIRECTIONAL, ENV_MAP = range(4)
+CAMERA_PINHOLE, CAMERA_FISHEYE = range(2)
+ALGO_PT2, ALGO_PTG = range(2)
+SAMPLE_BOTH, SAMPLE_BSDF, SAMPLE_LIGHTS = range(3)
+MIS_REFERENCE, MIS_CONSISTENT = range(2)
+GUIDE_NONE, GUIDE_VMF_TREE, GUIDE_LEARNED = range(3)
+
+Spectrum = C.c_float * NBINS
+
+
+class Shader(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("alpha", C.c_float), ("alpha_min", C.c_float),
+                ("ior_ratio", C.c_float), ("ior_lambda_coeff", C.c_float), ("specular", C.c_float),
+                ("reflectance", Spectrum)]
+
+
+class Medium(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("priority", C.c_int32), ("phase_kind", C.c_int32),
+                ("phase_g", C.c_float), ("sigma_s", Spectrum), ("sigma_a", Spectrum),
+                ("planet_center", C.c_double * 3), ("planet_radius", C.c_double),
+                ("atmosphere_height", C.c_double), ("rayleigh_scale_height", C.c_double),
+                ("mie_scale_height", C.c_double), ("mie_g", C.c_float),
+                ("rayleigh_sigma_s", Spectrum), ("mie_sigma_s", Spectrum), ("mie_sigma_a", Spectrum)]
+
+
+class Material(C.Structure):
+    _fields_ = [("shader", C.c_int32), ("medium", C.c_int32), ("outer_medium", C.c_int32),
+                ("emitter", C.c_int32)]
+
+
+class AreaEmitter(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("spectrum", Spectrum)]
+
+
+class EnvLight(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dir", C.c_double * 3), ("opening_angle_deg", C.c_double),
+                ("total_power", C.c_double), ("spectrum", Spectrum), ("tex_width", C.c_int32),
+                ("tex_height", C.c_int32), ("texture_rgb", C.POINTER(C.c_float))]
+
+
+class PointLight(C.Structure):
+    _fields_ = [("pos", C.c_double * 3), ("spectrum", Spectrum)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pos", C.c_double * 3), ("dir", C.c_double * 3),
+                ("up", C.c_double * 3), ("fov_deg", C.c_double)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("num_vertices", C.c_int64), ("vertices", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("num_triangles", C.c_int64),
+                ("indices", C.POINTER(C.c_uint32)), ("tri_material", C.POINTER(C.c_int32)),
+                ("num_spheres", C.c_int64), ("spheres", C.POINTER(C.c_float)),
+                ("sphere_material", C.POINTER(C.c_int32)),
+                ("num_shaders", C.c_int32), ("shaders", C.POINTER(Shader)),
+                ("num_media", C.c_int32), ("media", C.POINTER(Medium)),
+                ("empty_space_medium", C.c_int32),
+                ("num_materials", C.c_int32), ("materials", C.POINTER(Material)),
+                ("num_area_emitters", C.c_int32), ("area_emitters", C.POINTER(AreaEmitter)),
+                ("num_env_lights", C.c_int32), ("env_lights", C.POINTER(EnvLight)),
+                ("num_point_lights", C.c_int32), ("point_lights", C.POINTER(PointLight)),
+                ("camera", Camera)]
+
+
+class Params(C.Structure):
+    _fields_ = [("algo", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("max_ray_depth", C.c_int32), ("max_samples_per_pixel", C.c_int32),
+                ("sample_mode", C.c_int32), ("mis_mode", C.c_int32), ("num_wavelengths", C.c_int32),
+                ("seed", C.c_uint64), ("wavefront_paths", C.c_int64), ("linear_output", C.c_int32),
+                ("num_threads", C.c_int32), ("guide_kind", C.c_int32), ("guide_max_spp", C.c_int32),
+                ("guide_mix_bsdf", C.c_float), ("reserved", C.c_int32 * 8)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("paths_traced", C.c_uint64), ("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("total_spp", C.c_uint64), ("render_ms", C.c_double), ("bvh_build_ms", C.c_double),
+                ("upload_ms", C.c_double), ("bvh_nodes", C.c_uint64), ("scene_bytes", C.c_uint64),
+                ("state_bytes", C.c_uint64), ("passes", C.c_int32), ("wavefront_paths", C.c_int32)]
+
+
+PASS_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32)
+
+# Every exported symbol of libttgpu.so, as declared in include/ttgpu.h.
+EXPORTS = [
+    "ttgpu_create", "ttgpu_destroy", "ttgpu_last_error", "ttgpu_create_error", "ttgpu_run",
+    "ttgpu_render_spp", "ttgpu_request_interrupt", "ttgpu_request_stop", "ttgpu_set_pass_callback",
+    "ttgpu_reset_framebuffer", "ttgpu_read_framebuffer", "ttgpu_generate_image", "ttgpu_stats_get",
+    "ttgpu_kernel_time", "ttgpu_enable_kernel_timing", "ttgpu_trace_closest", "ttgpu_trace_occluded",
+    "ttgpu_sample_shader", "ttgpu_eval_shader", "ttgpu_sample_phase", "ttgpu_device_info",
+    "ttgpu_abi_version",
+]
+
+_lib = None
+
+
+def _declare(lib):
+    P = C.POINTER
+    f32p, f64p, i32p, u8p, u64p, i64p = P(C.c_float), P(C.c_double), P(C.c_int32), P(C.c_uint8), P(C.c_uint64), P(C.c_int64)
+    h = C.c_void_p
+    sig = {
+        "ttgpu_create": (C.c_int, [P(SceneDesc), P(Params), P(C.c_void_p)]),
+        "ttgpu_destroy": (None, [h]),
+        "ttgpu_last_error": (C.c_char_p, [h]),
+        "ttgpu_create_error": (C.c_char_p, []),
+        "ttgpu_run": (C.c_int, [h, C.c_int32]),
+        "ttgpu_render_spp": (C.c_int, [h, C.c_int32]),
+        "ttgpu_request_interrupt": (C.c_int, [h]),
+        "ttgpu_request_stop": (C.c_int, [h]),
+        "ttgpu_set_pass_callback": (C.c_int, [h, PASS_CALLBACK, C.c_void_p]),
+        "ttgpu_reset_framebuffer": (C.c_int, [h]),
+        "ttgpu_read_framebuffer": (C.c_int, [h, f64p, f64p, u64p]),
+        "ttgpu_generate_image": (C.c_int, [h, u8p]),
+        "ttgpu_stats_get": (C.c_int, [h, P(Stats)]),
+        "ttgpu_kernel_time": (C.c_int, [h, C.c_char_p, f64p, i64p]),
+        "ttgpu_enable_kernel_timing": (C.c_int, [h, C.c_int32]),
+        "ttgpu_trace_closest": (C.c_int, [h, C.c_int64, f32p, f32p, f32p, f32p, i32p, f32p, f32p]),
+        "ttgpu_trace_occluded": (C.c_int, [h, C.c_int64, f32p, f32p, f32p, i32p]),
+        "ttgpu_sample_shader": (C.c_int, [P(Shader), f32p, f32p, f32p, i32p, C.c_float, C.c_uint64,
+                                          C.c_int64, f32p, f32p, f32p]),
+        "ttgpu_eval_shader": (C.c_int, [P(Shader), f32p, f32p, f32p, i32p, C.c_float, C.c_int64,
+                                        f32p, f32p, f32p]),
+        "ttgpu_sample_phase": (C.c_int, [C.c_int32, C.c_float, f32p, C.c_uint64, C.c_int64, f32p, f32p, f32p]),
+        "ttgpu_device_info": (C.c_char_p, []),
+        "ttgpu_abi_version": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load_library(path=None):
+    """Loads libttgpu.so (the HIP path).  Raises if it is missing: the product has no CPU fallback."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} not found: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = _declare(C.CDLL(p))
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def default_params(width, height, spp=0, max_depth=8, seed=1, algo=ALGO_PT2):
+    p = Params()
+    p.algo = algo
+    p.width, p.height = width, height
+    p.max_ray_depth = max_depth
+    p.max_samples_per_pixel = spp
+    p.sample_mode = SAMPLE_BOTH
+    p.mis_mode = MIS_REFERENCE
+    p.num_wavelengths = 4
+    p.seed = seed
+    p.wavefront_paths = 0
+    p.linear_output = 0
+    p.num_threads = 0
+    p.guide_kind = GUIDE_NONE
+    p.guide_max_spp = 512
+    p.guide_mix_bsdf = 0.5
+    return p

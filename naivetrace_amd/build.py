@@ -1,0 +1,57 @@
+"""Builds libttgpu.so in-tree with hipcc for gfx950 (CDNA4).
+
+Usage: python -m naivetrace_amd.build   (or __graft_entry__.build()).
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libttgpu.so")
+ARCH = os.environ.get("TTGPU_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["kernels.hip", "ttgpu_api.cpp"]
+CPP_SOURCES = ["bvh_build.cpp"]
+# Packed FP32 instructions are disabled for all device code.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-Wno-unused-result", "-ffp-contract=fast"]
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    cc = hipcc()
+    tmp = tempfile.mkdtemp(prefix="ttgpu_build_")
+    objs = []
+    try:
+        for src in HIP_SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            cmd = [cc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + HIP_FLAGS
+            if verbose:
+                print(" ".join(cmd))
+            subprocess.run(cmd, check=True)
+            objs.append(obj)
+        for src in CPP_SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj]
+            subprocess.run(cmd, check=True)
+            objs.append(obj)
+        out_tmp = OUT + ".tmp"
+        cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", out_tmp] + objs + ["-lpthread"]
+        subprocess.run(cmd, check=True)
+        os.replace(out_tmp, OUT)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,209 @@
+// Binned-SAH BVH build on the host CPU (multi-threaded for the top levels).
+#include "bvh_build.hpp"
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <thread>
+
+namespace ttgpu {
+namespace {
+
+constexpr int kBins = 32;
+constexpr float kInf = std::numeric_limits<float>::infinity();
+
+struct Box {
+  float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+  void grow(const float *l, const float *h) {
+    for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], l[k]); hi[k] = std::max(hi[k], h[k]); }
+  }
+  void grow(const Box &b) { grow(b.lo, b.hi); }
+  float area() const {
+    float d[3];
+    for (int k = 0; k < 3; ++k) d[k] = std::max(0.f, hi[k] - lo[k]);
+    return 2.f * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+  }
+};
+
+struct TmpNode {
+  Box box;
+  int left = -1, right = -1;  // tmp indices (inner)
+  int start = 0, count = 0;   // leaf range
+  int depth = 0;
+};
+
+struct Builder {
+  const std::vector<BuildPrim> &prims;
+  std::vector<float> cen;
+  std::vector<uint32_t> idx;
+  std::vector<TmpNode> tmp;
+  std::atomic<int> next{0};
+  int max_leaf;
+  std::atomic<int> max_depth{0};
+
+  Builder(const std::vector<BuildPrim> &p, int ml) : prims(p), max_leaf(ml) {
+    size_t n = p.size();
+    cen.resize(3 * n);
+    idx.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      idx[i] = (uint32_t)i;
+      for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (p[i].lo[k] + p[i].hi[k]);
+    }
+    tmp.resize(std::max<size_t>(1, 2 * n));
+  }
+
+  int Build(int begin, int end, int depth, int parallel_depth) {
+    int ni = next.fetch_add(1);
+    TmpNode &nd = tmp[ni];
+    nd.depth = depth;
+    int md = max_depth.load();
+    while (depth > md && !max_depth.compare_exchange_weak(md, depth)) {}
+    Box cb;
+    for (int i = begin; i < end; ++i) {
+      const BuildPrim &p = prims[idx[i]];
+      nd.box.grow(p.lo, p.hi);
+      const float *c = &cen[3 * idx[i]];
+      cb.grow(c, c);
+    }
+    const int n = end - begin;
+    if (n <= 1) {
+      nd.start = begin;
+      nd.count = n;
+      return ni;
+    }
+    // Binned SAH over all three axes.
+    float best_cost = kInf;
+    int best_axis = -1, best_split = -1;
+    for (int axis = 0; axis < 3; ++axis) {
+      float ext = cb.hi[axis] - cb.lo[axis];
+      if (!(ext > 0.f)) continue;
+      Box bins[kBins];
+      int counts[kBins] = {0};
+      float scale = kBins / ext;
+      for (int i = begin; i < end; ++i) {
+        int b = std::min(kBins - 1, (int)((cen[3 * idx[i] + axis] - cb.lo[axis]) * scale));
+        counts[b]++;
+        bins[b].grow(prims[idx[i]].lo, prims[idx[i]].hi);
+      }
+      float right_area[kBins];
+      int right_count[kBins];
+      Box acc;
+      int cnt = 0;
+      for (int b = kBins - 1; b > 0; --b) {
+        acc.grow(bins[b]);
+        cnt += counts[b];
+        right_area[b] = acc.area();
+        right_count[b] = cnt;
+      }
+      Box lacc;
+      int lcnt = 0;
+      for (int b = 0; b < kBins - 1; ++b) {
+        lacc.grow(bins[b]);
+        lcnt += counts[b];
+        if (lcnt == 0 || right_count[b + 1] == 0) continue;
+        float cost = lacc.area() * lcnt + right_area[b + 1] * right_count[b + 1];
+        if (cost < best_cost) { best_cost = cost; best_axis = axis; best_split = b; }
+      }
+    }
+    const float parent_area = std::max(nd.box.area(), 1e-30f);
+    // Traversal : intersection cost ratio 1 : 1 (costs relative to one primitive test).
+    float split_cost = 1.0f + best_cost / parent_area;
+    float leaf_cost = (float)n;
+    int mid;
+    if (best_axis < 0) {
+      if (n <= max_leaf) { nd.start = begin; nd.count = n; return ni; }
+      mid = begin + n / 2;  // all centroids coincide: split the range
+    } else {
+      if (n <= max_leaf && leaf_cost <= split_cost) { nd.start = begin; nd.count = n; return ni; }
+      float scale = kBins / (cb.hi[best_axis] - cb.lo[best_axis]);
+      auto it = std::partition(idx.begin() + begin, idx.begin() + end, [&](uint32_t p) {
+        int b = std::min(kBins - 1, (int)((cen[3 * p + best_axis] - cb.lo[best_axis]) * scale));
+        return b <= best_split;
+      });
+      mid = (int)(it - idx.begin());
+      if (mid == begin || mid == end) mid = begin + n / 2;
+    }
+    int l, r;
+    if (parallel_depth > 0 && n > 65536) {
+      std::thread t([&] { l = Build(begin, mid, depth + 1, parallel_depth - 1); });
+      r = Build(mid, end, depth + 1, parallel_depth - 1);
+      t.join();
+    } else {
+      l = Build(begin, mid, depth + 1, 0);
+      r = Build(mid, end, depth + 1, 0);
+    }
+    tmp[ni].left = l;
+    tmp[ni].right = r;
+    return ni;
+  }
+};
+
+void WriteChild(const TmpNode &c, int inner_index, float *xy, float *z, int *ch, int *cnt) {
+  if (c.count > 0 || c.left < 0) {
+    *ch = ~c.start;
+    *cnt = c.count;
+  } else {
+    *ch = inner_index;
+    *cnt = 0;
+  }
+  xy[0] = c.box.lo[0]; xy[1] = c.box.hi[0]; xy[2] = c.box.lo[1]; xy[3] = c.box.hi[1];
+  z[0] = c.box.lo[2]; z[1] = c.box.hi[2];
+}
+
+}  // namespace
+
+BuiltBvh BuildBvh(const std::vector<BuildPrim> &prims, int max_leaf, int num_threads) {
+  BuiltBvh out;
+  if (prims.empty()) return out;
+  Builder b(prims, max_leaf);
+  int pdepth = 0;
+  unsigned hc = num_threads > 0 ? (unsigned)num_threads : std::max(1u, std::thread::hardware_concurrency());
+  while ((1u << pdepth) < hc && pdepth < 6) ++pdepth;
+  int root = b.Build(0, (int)prims.size(), 0, pdepth);
+  int used = b.next.load();
+  // Number the inner nodes in depth-first order (root first) for the device layout.
+  std::vector<int> inner_id(used, -1);
+  std::vector<int> order_nodes;
+  std::vector<int> stack{root};
+  while (!stack.empty()) {
+    int t = stack.back();
+    stack.pop_back();
+    const TmpNode &nd = b.tmp[t];
+    if (nd.left < 0) continue;
+    inner_id[t] = (int)order_nodes.size();
+    order_nodes.push_back(t);
+    stack.push_back(nd.right);
+    stack.push_back(nd.left);
+  }
+  const TmpNode &rn = b.tmp[root];
+  for (int k = 0; k < 3; ++k) { out.root_lo[k] = rn.box.lo[k]; out.root_hi[k] = rn.box.hi[k]; }
+  out.max_depth = b.max_depth.load();
+  out.order = b.idx;
+  if (order_nodes.empty()) {
+    // Single leaf: a root whose right child is an empty leaf with an inverted box.
+    out.nodes.assign(16, 0.f);
+    float *n = out.nodes.data();
+    int ch[4];
+    WriteChild(rn, 0, n + 0, n + 8, &ch[0], &ch[2]);
+    n[4] = kInf; n[5] = -kInf; n[6] = kInf; n[7] = -kInf; n[10] = kInf; n[11] = -kInf;
+    ch[1] = ~0; ch[3] = 0;
+    std::memcpy(n + 12, ch, sizeof(ch));
+    out.num_nodes = 1;
+    return out;
+  }
+  out.num_nodes = (int)order_nodes.size();
+  out.nodes.assign(16 * (size_t)out.num_nodes, 0.f);
+  for (int i = 0; i < out.num_nodes; ++i) {
+    const TmpNode &nd = b.tmp[order_nodes[i]];
+    const TmpNode &l = b.tmp[nd.left], &r = b.tmp[nd.right];
+    float *n = &out.nodes[16 * (size_t)i];
+    int ch[4];
+    WriteChild(l, inner_id[nd.left], n + 0, n + 8, &ch[0], &ch[2]);
+    WriteChild(r, inner_id[nd.right], n + 4, n + 10, &ch[1], &ch[3]);
+    std::memcpy(n + 12, ch, sizeof(ch));
+  }
+  return out;
+}
+
+}  // namespace ttgpu

@@ -1,0 +1,26 @@
+// Host-side binned-SAH BVH builder (replaces Embree's BVH build,
+// src/embreeaccelerator.cxx:36-84, and Scene::BuildAccelStructure,
+// src/scene.cxx:116-141).  Produces the GPU layout of device_scene.hpp.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace ttgpu {
+
+struct BuildPrim {
+  float lo[3], hi[3];
+};
+
+struct BuiltBvh {
+  // Node array in the device layout (BvhNode, 16 floats each).
+  std::vector<float> nodes;      // 16 * num_nodes
+  std::vector<uint32_t> order;   // leaf-order position -> input primitive index
+  int num_nodes = 0;
+  int max_depth = 0;
+  float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
+};
+
+// SAH with 32 bins per axis, leaves of at most `max_leaf` primitives.
+BuiltBvh BuildBvh(const std::vector<BuildPrim> &prims, int max_leaf = 4, int num_threads = 0);
+
+}  // namespace ttgpu

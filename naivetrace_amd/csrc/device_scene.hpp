@@ -1,0 +1,157 @@
+// Device-side layout of the scene and of the path-state queues.
+//
+// Everything here lives in HBM as structure-of-arrays.  The scene is the
+// flattened ToyTrace `Scene` (src/scene.hxx:134-315) after the host BVH build:
+// primitives are stored in BVH leaf order so the traversal touches them
+// contiguously; every per-primitive array below is indexed by that order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/ttgpu.h"
+
+namespace ttgpu {
+
+constexpr int kNBins = 36;
+constexpr int kNW = 4;              // wavelengths per path
+constexpr int kStrata = kNBins / kNW;
+constexpr int kMaxEnvLights = 8;
+constexpr int kMaxMedia = 32;
+constexpr int kVacuum = 0xFF;       // medium id of vacuum in the packed tracker
+constexpr int kEmptySlot = 0xFE;
+
+// BVH2 node, 64 bytes: both child boxes are stored in the parent so one fetch
+// decides both children (Aila & Laine 2009 layout).
+struct alignas(16) BvhNode {
+  float4 lxy;   // left  lo.x hi.x lo.y hi.y
+  float4 rxy;   // right lo.x hi.x lo.y hi.y
+  float4 lrz;   // left lo.z hi.z, right lo.z hi.z
+  int4 ch;      // left, right: >= 0 inner node, < 0 leaf ~first_prim; z/w: prim counts of leaves
+};
+
+// Primitive record in leaf order, 48 bytes.
+//   triangle: a = (v0, orig_id bits), b = (v1, kind=0), c = (v2, 0)
+//   sphere  : a = (centre, orig_id bits), b = (radius, 0, 0, kind=1), c = 0
+struct alignas(16) PrimRecord {
+  float4 a, b, c;
+};
+
+struct DevShader {
+  int kind;
+  float alpha, alpha_min, ior_ratio, ior_lambda_coeff, specular;
+  float reflectance[kNBins];
+};
+
+struct DevMedium {
+  int kind, priority, phase_kind;
+  float phase_g;
+  float sigma_s[kNBins];
+  float sigma_t[kNBins];
+  double planet_center[3];
+  double planet_radius, rayleigh_inv_h, mie_inv_h, lower_cutoff;
+  float mie_g;
+  float rayleigh_sigma_s[kNBins], mie_sigma_s[kNBins], mie_sigma_a[kNBins];
+};
+
+struct DevMaterial {
+  int shader, medium, outer_medium, emitter;
+};
+
+struct DevEmitter {
+  int kind;
+  float radiance[kNBins];   // already divided by 2 pi for uniform emitters
+};
+
+struct DevEnvLight {
+  int kind;
+  float3 frame_x, frame_y, frame_z;   // sun: cone frame (z = centre of dir_out); map: up frame
+  float cos_open, pdf;               // sun
+  float spectrum[kNBins];            // sun (power * pdf * solar), dome, directional
+  float3 dir;                        // dome: up; directional: dir_out
+  int tex_w, tex_h;
+  const float *tex;                  // RGB float
+  const float *cdf_rows;             // h
+  const float *cdf_cols;             // h * w (row-major, normalised per row)
+};
+
+struct DevPointLight {
+  float3 pos;
+  float intensity[kNBins];           // spectrum / (4 pi)
+};
+
+struct DevCamera {
+  int kind;
+  float3 pos, right, up, dir;
+  float xmin, ymin, xper, yper, fish_delta;
+  int width, height;
+};
+
+struct DeviceScene {
+  const BvhNode *nodes;
+  const PrimRecord *prims;         // leaf order
+  const int *prim_material;        // leaf order
+  const float4 *prim_normals;      // leaf order, 3 per triangle (vertex normals), or null
+  const float *prim_area;          // leaf order
+  int num_prims;
+  int num_triangles;
+  // volume boundaries (separate BVH)
+  const BvhNode *vol_nodes;
+  const PrimRecord *vol_prims;
+  const int *vol_material;
+  int num_vol_prims;
+  // tables
+  const DevShader *shaders;
+  const DevMedium *media;
+  const DevMaterial *materials;
+  const DevEmitter *emitters;
+  const DevEnvLight *envs;
+  const DevPointLight *points;
+  const int *area_lights;          // leaf-order prim index of each area light
+  int num_area_lights, num_envs, num_points, num_media;
+  float type_probs[4];             // env, area, point, volume
+  float area_pick_prob;            // probability to pick one given area light (incl. type)
+  float bb_diag;
+  int empty_space_medium;          // kVacuum or medium id
+  int has_media;
+  uint64_t cam_tracker;            // packed medium tracker at the camera
+  DevCamera cam;
+  int sample_mode, mis_mode, max_depth;
+};
+
+// Path state, structure of arrays over the wavefront capacity.
+struct PathQueues {
+  float4 *org;          // xyz
+  float4 *dir;          // xyz
+  float4 *hit;          // t, u, v, prim (int bits); prim < 0: miss
+  float4 *thru;         // throughput per wavelength
+  float4 *rad;          // accumulated rgb, w: unused
+  uint32_t *bins;       // 4 x 8-bit bin indices
+  uint32_t *pixel;
+  uint2 *rng_state;
+  uint2 *rng_inc;
+  uint32_t *flags;      // bits 0-7 node count, 8 has_last_pdf, 9 last_pdf_delta, 10 finalize_after_shadow
+  float *last_pdf;
+  float *roughness;
+  uint2 *tracker;       // packed medium tracker: current | 4 slots
+  // shadow rays
+  float4 *sh_org;       // xyz, tmax
+  float4 *sh_dir;       // xyz
+  float4 *sh_contrib;   // rgb if unoccluded
+  // queues
+  uint32_t *queue_a, *queue_b, *shadow_queue;
+  uint32_t *counters;   // [0] queue_a count, [1] queue_b count, [2] shadow count, [3..] stats
+  // media (only when the scene has media): N x N matrices per path, row-major
+  float *w_track;       // 16 per path
+  float *w_track_null;  // 16 per path
+  int capacity;
+};
+
+enum FlagBits : uint32_t {
+  kFlagNodeMask = 0xFFu,
+  kFlagHasLastPdf = 1u << 8,
+  kFlagLastPdfDelta = 1u << 9,
+  kFlagFinalizeAfterShadow = 1u << 10,
+};
+
+enum Counter { kCountA = 0, kCountB = 1, kCountShadow = 2, kStatClosest = 4, kStatShadow = 5, kStatPaths = 6, kNumCounters = 8 };
+
+}  // namespace ttgpu

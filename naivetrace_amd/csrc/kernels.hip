@@ -1,0 +1,633 @@
+// Wavefront kernels of the MI355X path tracer (pt2 semantics).
+//
+// One bounce of every path in flight is:  trace (closest hit)  ->  shade
+// (emission with MIS, next-event estimation, BSDF sampling)  ->  shadow (any
+// hit of the compacted NEE rays, accumulate).  Path state is structure-of-
+// arrays in HBM; queues are compacted with wave-level ballot + one atomic per
+// wave.  The reference's per-path loop is
+// src/renderingalgorithms_pathtracing2.cxx:338-443.
+#include "device_math.hpp"
+#include "shading.hpp"
+#include "trace.hpp"
+#include "kernels.hpp"
+
+namespace ttgpu {
+
+// ---- wave-aggregated queue append ----------------------------------------
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+  const uint64_t mask = __ballot(pred);
+  const uint64_t active = __ballot(1);
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)active) - 1;
+  uint32_t base = 0;
+  if (lane == leader && mask) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ uint32_t grid_stride() { return gridDim.x * blockDim.x; }
+__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+__global__ void k_set_counters(uint32_t *counters, uint32_t a, uint32_t b, uint32_t shadow) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    counters[kCountA] = a;
+    counters[kCountB] = b;
+    counters[kCountShadow] = shadow;
+  }
+}
+__global__ void k_reset_counter(uint32_t *counters, int which) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counters[which] = 0;
+}
+__global__ void k_fill_f64(double *p, size_t n, double v) {
+  for (size_t i = gtid(); i < n; i += grid_stride()) p[i] = v;
+}
+
+// ---- camera (src/camera.hxx:105-124, 176-210) ----------------------------
+__device__ __forceinline__ bool camera_ray(const DevCamera &c, int px, int py, float r0, float r1, float3 &dir) {
+  if (c.kind == TTGPU_CAMERA_PINHOLE) {
+    float3 v = normalize(mk(c.xmin + (r0 + px) * c.xper, c.ymin + (r1 + py) * c.yper, 1.f));
+    dir = c.right * v.x + c.up * v.y + c.dir * v.z;
+    return true;
+  }
+  float u = (r0 + px) * c.fish_delta - 1.f, v = (r1 + py) * c.fish_delta - 1.f;
+  float rho = u * u + v * v;
+  float3 w = mk(2.f * u, 2.f * v, rho - 1.f) * (1.f / (1.f + rho));
+  w.z = -w.z;
+  if (rho <= 1.f) {
+    dir = c.right * w.x + c.up * w.y + c.dir * w.z;
+    return true;
+  }
+  dir = c.dir;
+  return false;
+}
+
+// Camera paths for global indices [base, base+count): pixel = g % npix, sample = spp_base + g / npix.
+// Lambda selection: LambdaSelectionStrategy::WithWeights (src/shader_util.hxx:83-91) with 4 strata.
+__global__ void __launch_bounds__(256) k_raygen(DeviceScene sc, PathQueues q, uint64_t base, uint32_t count, uint32_t npix,
+                                                uint32_t spp_base, uint64_t seed) {
+  for (uint32_t i = gtid(); i < count; i += grid_stride()) {
+    const uint64_t g = base + i;
+    const uint32_t pixel = (uint32_t)(g % npix);
+    const uint64_t sample = spp_base + g / npix;
+    Pcg rng;
+    rng.seed(seed, sample * npix + pixel);
+    int main_idx = (int)rng.bounded(kStrata);
+    int idx[4] = {main_idx, main_idx + kStrata, main_idx + 2 * kStrata, main_idx + 3 * kStrata};
+    int rot = (int)rng.bounded(4);
+    int t = idx[0]; idx[0] = idx[rot]; idx[rot] = t;
+    const float lambda0 = 370.f + 10.f * idx[0] + 10.f * rng.uniform();
+    rng.uniform(); rng.uniform(); rng.uniform();  // the other three jittered wavelengths (unused)
+    uint32_t bins = (uint32_t)idx[0] | ((uint32_t)idx[1] << 8) | ((uint32_t)idx[2] << 16) | ((uint32_t)idx[3] << 24);
+    const int px = pixel % sc.cam.width, py = pixel / sc.cam.width;
+    float r0 = rng.uniform(), r1 = rng.uniform();
+    float3 dir;
+    bool ok = camera_ray(sc.cam, px, py, r0, r1, dir);
+    q.org[i] = make_float4(sc.cam.pos.x, sc.cam.pos.y, sc.cam.pos.z, lambda0);
+    q.dir[i] = make_float4(dir.x, dir.y, dir.z, 0.f);
+    q.thru[i] = ok ? f4((float)kStrata) : f4(0.f);
+    q.rad[i] = f4(0.f);
+    q.bins[i] = bins;
+    q.pixel[i] = pixel;
+    q.rng_state[i] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+    q.rng_inc[i] = make_uint2((uint32_t)rng.inc, (uint32_t)(rng.inc >> 32));
+    q.flags[i] = 2u;  // node count 2: camera position and direction nodes (pt2.cxx:388-406)
+    q.last_pdf[i] = 0.f;
+    q.roughness[i] = 0.f;
+    q.tracker[i] = make_uint2((uint32_t)sc.cam_tracker, (uint32_t)(sc.cam_tracker >> 32));
+    q.queue_a[i] = i;
+  }
+}
+
+// ---- closest hit --------------------------------------------------------------
+__global__ void __launch_bounds__(kTraceBlock) k_trace(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue,
+                                                       const uint32_t *__restrict__ count_ptr, uint32_t *stats) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  const uint32_t count = *count_ptr;
+  for (uint32_t k = gtid(); k < count; k += grid_stride()) {
+    const uint32_t i = queue[k];
+    Ray r;
+    r.org = xyz(q.org[i]);
+    r.dir = xyz(q.dir[i]);
+    r.tmax = 3.0e38f;
+    HitRec h;
+    bool any = traverse<false>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
+    q.hit[i] = any ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim)) : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+  }
+  if (gtid() == 0) atomicAdd(stats + kStatClosest, count);
+}
+
+// ---- environment lights (src/light.hxx:148-288, src/light.cxx:29-189) -------------
+__device__ __forceinline__ uint32_t upper_bound(const float *cdf, uint32_t n, float r) {
+  uint32_t lo = 0, hi = n;  // first index with r < cdf[i]
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (r < cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  return lo >= n ? n - 1 : lo;
+}
+__device__ __forceinline__ float cmf_prob(const float *cdf, uint32_t i) { return cdf[i] - (i > 0 ? cdf[i - 1] : 0.f); }
+
+__device__ __forceinline__ void env_map_pixel(const DevEnvLight &e, float3 dir_out, int &x, int &y) {
+  float3 l = mk(dot(e.frame_x, dir_out), dot(e.frame_y, dir_out), dot(e.frame_z, dir_out));
+  // KartesianToSpherical (src/vec3f.hxx:298-323): component 2 is the polar axis.
+  const float X = l.x, Z = l.y, Y = l.z;
+  const float ax = fabsf(X), az = fabsf(Z);
+  const float r = sqrtf(X * X + Y * Y + Z * Z);
+  float theta = acosf(fminf(1.f, fmaxf(-1.f, Y / r)));
+  float phi;
+  if (ax > az) {
+    phi = atan2f(Z, ax);
+    phi = X > 0.f ? phi : kPi - phi;
+  } else {
+    phi = atan2f(X, az);
+    phi = Z > 0.f ? 0.5f * kPi - phi : 1.5f * kPi + phi;
+  }
+  float u = phi * (0.5f * kInvPi), v = theta * kInvPi;
+  u = u - floorf(u);
+  v = v - floorf(v);
+  if (u > 1.f - kFltEps) u -= kFltEps;
+  if (v > 1.f - kFltEps) v -= kFltEps;
+  if (!(u >= 0.f)) u = 0.f;  // also catches NaN directions
+  if (!(v >= 0.f)) v = 0.f;
+  x = max(0, min((int)(u * e.tex_w), e.tex_w - 1));
+  y = e.tex_h - 1 - max(0, min((int)(v * e.tex_h), e.tex_h - 1));
+}
+__device__ __forceinline__ void env_pixel_bounds(const DevEnvLight &e, int x, int y, float &phi0, float &z0, float &phi1, float &z1) {
+  float ul = (float)x / e.tex_w, vl = 1.f - (float)y / e.tex_h;
+  float uu = ul + 1.f / e.tex_w, vu = 1.f - (float)(y + 1) / e.tex_h;
+  phi0 = ul * 2.f * kPi; phi1 = uu * 2.f * kPi;
+  z0 = cosf(vl * kPi); z1 = cosf(vu * kPi);
+}
+__device__ float4 env_eval_one(const DevEnvLight &e, float3 dir_out, uint32_t bins, float *pdf) {
+  switch (e.kind) {
+    case TTGPU_ENV_SUN: {
+      bool in = dot(dir_out, e.frame_z) > e.cos_open;
+      *pdf = in ? e.pdf : 0.f;
+      return in ? take(e.spectrum, bins) : f4(0.f);
+    }
+    case TTGPU_ENV_DOME: {
+      bool in = dot(dir_out, -e.dir) > 0.f;
+      *pdf = in ? 1.f / (2.f * kPi) : 0.f;
+      return in ? take(e.spectrum, bins) : f4(0.f);
+    }
+    case TTGPU_ENV_DIRECTIONAL: *pdf = 0.f; return f4(0.f);
+    case TTGPU_ENV_MAP: {
+      int x, y;
+      env_map_pixel(e, dir_out, x, y);
+      const float *px = e.tex + 3 * ((size_t)y * e.tex_w + x);
+      float p0, z0, p1, z1;
+      env_pixel_bounds(e, x, y, p0, z0, p1, z1);
+      *pdf = cmf_prob(e.cdf_rows, y) * cmf_prob(e.cdf_cols + (size_t)y * e.tex_w, x) / ((z1 - z0) * (p1 - p0));
+      return rgb_to_spectral(mk(px[0], px[1], px[2]), bins);
+    }
+  }
+  *pdf = 0.f;
+  return f4(0.f);
+}
+// Sum of all env lights and their average pdf (TotalEnvironmentalRadianceField).
+__device__ float4 env_eval(const DeviceScene &sc, float3 dir_out, uint32_t bins, float *pdf) {
+  float4 L = f4(0.f);
+  float p = 0.f;
+  for (int i = 0; i < sc.num_envs; ++i) {
+    float pi;
+    L = L + env_eval_one(sc.envs[i], dir_out, bins, &pi);
+    p += pi;
+  }
+  *pdf = sc.num_envs ? p / sc.num_envs : 0.f;
+  return L;
+}
+__device__ void env_sample(const DeviceScene &sc, uint32_t bins, Pcg &rng, float3 &dir_out, float4 &val, float &pdf, bool &delta) {
+  int i = sc.num_envs > 1 ? (int)rng.bounded(sc.num_envs) : 0;
+  const DevEnvLight &e = sc.envs[i];
+  delta = false;
+  switch (e.kind) {
+    case TTGPU_ENV_SUN: {
+      float r0 = rng.uniform(), r1 = rng.uniform();
+      float3 l = to_sphere_section(e.cos_open, r0, r1);
+      dir_out = e.frame_x * l.x + e.frame_y * l.y + e.frame_z * l.z;
+      val = take(e.spectrum, bins);
+      pdf = e.pdf;
+      break;
+    }
+    case TTGPU_ENV_DOME: {
+      float r0 = rng.uniform(), r1 = rng.uniform();
+      float3 l = to_uniform_hemisphere(r0, r1);
+      dir_out = e.frame_x * l.x + e.frame_y * l.y + e.frame_z * l.z;
+      val = take(e.spectrum, bins);
+      pdf = 1.f / (2.f * kPi);
+      break;
+    }
+    case TTGPU_ENV_DIRECTIONAL: {
+      dir_out = e.dir;
+      val = take(e.spectrum, bins);
+      pdf = 1.f;
+      delta = true;
+      break;
+    }
+    default: {
+      float r0 = rng.uniform(), r1 = rng.uniform();
+      uint32_t y = upper_bound(e.cdf_rows, e.tex_h, r1);
+      const float *row = e.cdf_cols + (size_t)y * e.tex_w;
+      uint32_t x = upper_bound(row, e.tex_w, r0);
+      float p0, z0, p1, z1;
+      env_pixel_bounds(e, x, y, p0, z0, p1, z1);
+      float s0 = rng.uniform(), s1 = rng.uniform();
+      float3 l = to_sphere_section(s0, s1, p0, z0, p1, z1);
+      dir_out = e.frame_x * l.x + e.frame_y * l.y + e.frame_z * l.z;
+      const float *px = e.tex + 3 * ((size_t)y * e.tex_w + x);
+      val = rgb_to_spectral(mk(px[0], px[1], px[2]), bins);
+      pdf = cmf_prob(e.cdf_rows, y) * cmf_prob(row, x) / ((z1 - z0) * (p1 - p0));
+      break;
+    }
+  }
+  pdf /= (float)sc.num_envs;
+}
+
+// ---- surface interaction at a hit ----------------------------------------------
+__device__ __forceinline__ SurfHit make_hit(const DeviceScene &sc, int prim, float u, float v, float t, float3 org, float3 dir) {
+  SurfHit h;
+  const PrimRecord pr = sc.prims[prim];
+  h.prim = prim;
+  h.material = sc.prim_material[prim];
+  if (__float_as_int(pr.b.w) == 0) {
+    const float3 v0 = xyz(pr.a), v1 = xyz(pr.b), v2 = xyz(pr.c);
+    const float b0 = 1.f - u - v;
+    h.pos = v0 * b0 + v1 * u + v2 * v;
+    h.ng = normalize(cross(v1 - v0, v2 - v0));
+    if (sc.prim_normals) {
+      const float4 *nn = sc.prim_normals + 3 * (size_t)prim;
+      h.ns = normalize(xyz(nn[0]) * b0 + xyz(nn[1]) * u + xyz(nn[2]) * v);
+    } else {
+      h.ns = h.ng;
+    }
+    // PBRT position error bound gamma(7) * sum |b_i p_i| (src/primitive.cxx:291-299), projected on |n|.
+    const float g7 = 7.f * kFltEps * 0.5f / (1.f - 7.f * kFltEps * 0.5f);
+    float3 e = mk(fabsf(v0.x * b0) + fabsf(v1.x * u) + fabsf(v2.x * v), fabsf(v0.y * b0) + fabsf(v1.y * u) + fabsf(v2.y * v),
+                  fabsf(v0.z * b0) + fabsf(v1.z * u) + fabsf(v2.z * v)) * g7;
+    h.offset = e.x * fabsf(h.ng.x) + e.y * fabsf(h.ng.y) + e.z * fabsf(h.ng.z);
+  } else {
+    const float3 c = xyz(pr.a);
+    const float3 p = org + dir * t;
+    h.ng = normalize(p - c);
+    h.pos = c + h.ng * pr.b.x;
+    h.ns = h.ng;
+    const float g5 = 5.f * kFltEps * 0.5f / (1.f - 5.f * kFltEps * 0.5f);
+    h.offset = (fabsf(h.pos.x) * fabsf(h.ng.x) + fabsf(h.pos.y) * fabsf(h.ng.y) + fabsf(h.pos.z) * fabsf(h.ng.z)) * g5;
+  }
+  // Orient towards the incoming ray (SurfaceInteraction::SetOrientedNormals, src/scene.cxx:239-244).
+  const float sign = -copysignf(1.f, dot(dir, h.ng));
+  h.n = h.ng * sign;
+  h.ns = h.ns * sign;
+  return h;
+}
+
+__device__ __forceinline__ void finalize_path(const PathQueues &q, double *fb, double *fb_sq, uint32_t i, float4 rad) {
+  const uint32_t p = q.pixel[i];
+  atomicAdd(fb + 3 * (size_t)p + 0, (double)rad.x);
+  atomicAdd(fb + 3 * (size_t)p + 1, (double)rad.y);
+  atomicAdd(fb + 3 * (size_t)p + 2, (double)rad.z);
+  if (fb_sq) {
+    atomicAdd(fb_sq + 3 * (size_t)p + 0, (double)rad.x * rad.x);
+    atomicAdd(fb_sq + 3 * (size_t)p + 1, (double)rad.y * rad.y);
+    atomicAdd(fb_sq + 3 * (size_t)p + 2, (double)rad.z * rad.z);
+  }
+}
+
+__device__ __forceinline__ float4 add_rgb(float4 rad, float3 c) { return make_float4(rad.x + c.x, rad.y + c.y, rad.z + c.z, rad.w); }
+
+// ---- shade: emission, NEE, scattering (pt2.cxx:410-443, 479-556, 605-684) --------
+__global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
+                                               const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
+                                               uint32_t *count_out, double *fb, double *fb_sq) {
+  const uint32_t count = *count_in;
+  const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
+  const bool use_emission = sc.sample_mode != TTGPU_SAMPLE_LIGHTS;
+  const bool mis = use_nee && use_emission;
+  const uint32_t n_iter = (count + grid_stride() - 1) / grid_stride();
+  for (uint32_t it = 0; it < n_iter; ++it) {
+    const uint32_t k = it * grid_stride() + gtid();
+    bool active = k < count;
+    bool push = false, push_shadow = false;
+    uint32_t i = 0;
+    if (active) {
+      i = queue_in[k];
+      const float4 hitv = q.hit[i];
+      const float3 dir = xyz(q.dir[i]);
+      const float4 org4 = q.org[i];
+      const float lambda0 = org4.w;
+      float4 thru = q.thru[i];
+      float4 rad = q.rad[i];
+      const uint32_t bins = q.bins[i];
+      uint32_t flags = q.flags[i];
+      const float last_pdf = q.last_pdf[i];
+      const bool has_last = (flags & kFlagHasLastPdf) != 0;
+      const bool last_delta = (flags & kFlagLastPdfDelta) != 0;
+      Pcg rng = load_rng(q, i);
+      const int prim = __float_as_int(hitv.w);
+      if (prim < 0) {
+        // AddEnvEmission (pt2.cxx:661-684)
+        if (use_emission && sc.num_envs > 0) {
+          float penv;
+          float4 L = env_eval(sc, -dir, bins, &penv);
+          float w = 1.f;
+          if (has_last && !last_delta && mis) w = last_pdf / (last_pdf + penv * sc.type_probs[0]);
+          rad = add_rgb(rad, spectral_to_rgb(w * (L * thru), bins));
+        }
+        finalize_path(q, fb, fb_sq, i, rad);
+      } else {
+        const SurfHit h = make_hit(sc, prim, hitv.y, hitv.z, hitv.x, xyz(org4), dir);
+        const DevMaterial mat = sc.materials[h.material];
+        // MaybeAddEmission (pt2.cxx:627-658)
+        if (use_emission && mat.emitter >= 0) {
+          const DevEmitter &em = sc.emitters[mat.emitter];
+          if (em.kind == TTGPU_AREA_UNIFORM && dot(h.ng, -dir) > 0.f) {
+            float4 L = take(em.radiance, bins);
+            float w = 1.f;
+            if (has_last && !last_delta && mis) {
+              float p_light = sc.area_pick_prob / sc.prim_area[prim] * area_to_solid_angle(hitv.x, dir, h.n);
+              w = last_pdf / (last_pdf + p_light);
+            }
+            rad = add_rgb(rad, spectral_to_rgb(w * (L * thru), bins));
+          }
+        }
+        const DevShader &shd = sc.shaders[mat.shader];
+        float rough = q.roughness[i];
+        const float3 wi = -dir;
+        // AddDirectLighting (pt2.cxx:479-556), vacuum transmission.
+        if (use_nee && (sc.num_area_lights + sc.num_envs + sc.num_points) > 0) {
+          const float u_type = rng.uniform();
+          const float u_idx = rng.uniform();
+          int kind = 3;
+          {
+            float r = u_type;
+            while (kind > 0 && r >= sc.type_probs[kind]) { r -= sc.type_probs[kind]; --kind; }
+          }
+          float3 ldir = mk(0.f, 0.f, 1.f);
+          float len = 0.f;
+          float4 lw = f4(0.f);
+          float lpdf = 0.f;
+          bool ldelta = false;
+          // A type whose light count is zero can only be reached through float rounding of the
+          // selection probabilities; it contributes nothing.
+          bool valid = (kind == 0 && sc.num_envs > 0) || (kind == 1 && sc.num_area_lights > 0) || (kind == 2 && sc.num_points > 0);
+          if (!valid) {
+          } else if (kind == 1) {
+            const uint32_t li = min((uint32_t)(u_idx * sc.num_area_lights), (uint32_t)sc.num_area_lights - 1u);
+            const int lp = sc.area_lights[li];
+            const PrimRecord pr = sc.prims[lp];
+            const float r0 = rng.uniform(), r1 = rng.uniform();
+            float3 lpos, lng, lns;
+            if (__float_as_int(pr.b.w) == 0) {
+              const float3 v0 = xyz(pr.a), v1 = xyz(pr.b), v2 = xyz(pr.c);
+              const float s = sqrtf(r0);
+              const float b0 = 1.f - s, b1 = (1.f - r1) * s, b2 = r1 * s;
+              lpos = v0 * b0 + v1 * b1 + v2 * b2;
+              lng = normalize(cross(v1 - v0, v2 - v0));
+              if (sc.prim_normals) {
+                const float4 *nn = sc.prim_normals + 3 * (size_t)lp;
+                lns = normalize(xyz(nn[0]) * b0 + xyz(nn[1]) * b1 + xyz(nn[2]) * b2);
+              } else {
+                lns = lng;
+              }
+            } else {
+              lng = to_uniform_sphere(r0, r1);
+              lns = lng;
+              lpos = xyz(pr.a) + lng * pr.b.x;
+            }
+            float3 d = lpos - h.pos;
+            len = length(d);
+            ldir = len > 0.f ? d * (1.f / len) : h.n;
+            if (!(len > 0.f)) len = 2.220446049250313e-16f;
+            const DevEmitter &em = sc.emitters[sc.materials[sc.prim_material[lp]].emitter];
+            float4 Le = (em.kind == TTGPU_AREA_UNIFORM && dot(lng, -ldir) > 0.f) ? take(em.radiance, bins) : f4(0.f);
+            const float area_pdf = 1.f / sc.prim_area[lp];
+            const float dfac = fabsf(dot(lns, ldir));
+            lw = (dfac / (area_pdf * sc.area_pick_prob * len * len)) * Le;
+            lpdf = area_to_solid_angle(len, ldir, lng) * area_pdf;
+            if (sc.mis_mode == TTGPU_MIS_CONSISTENT) lpdf *= sc.area_pick_prob;
+          } else if (kind == 0) {
+            float3 dout;
+            float4 val;
+            float p;
+            env_sample(sc, bins, rng, dout, val, p, ldelta);
+            ldir = -dout;
+            len = 10.f * sc.bb_diag;
+            lw = (1.f / (p * sc.type_probs[0])) * val;
+            lpdf = p * (sc.mis_mode == TTGPU_MIS_CONSISTENT ? sc.type_probs[0] : 1.f);
+          } else if (kind == 2) {
+            const uint32_t li = min((uint32_t)(u_idx * sc.num_points), (uint32_t)sc.num_points - 1u);
+            const DevPointLight &pl = sc.points[li];
+            float3 d = pl.pos - h.pos;
+            len = length(d);
+            ldir = len > 0.f ? d * (1.f / len) : h.n;
+            if (!(len > 0.f)) len = 2.220446049250313e-16f;
+            const float prob = sc.type_probs[2] / sc.num_points;
+            lw = (1.f / (prob * len * len)) * take(pl.intensity, bins);
+            lpdf = 1.f;
+            ldelta = true;
+          } else {
+            valid = false;
+          }
+          if (valid) {
+            float bsdf_pdf;
+            float4 f = eval_bsdf(shd, h, wi, bins, rough, lambda0, ldir, &bsdf_pdf);
+            const float dfac = fabsf(dot(h.ns, ldir));
+            float w = 1.f;
+            if (!ldelta && mis) w = lpdf / (lpdf + bsdf_pdf);
+            float4 c = (w * dfac) * (thru * f * lw);
+            float3 crgb = spectral_to_rgb(c, bins);
+            if (crgb.x != 0.f || crgb.y != 0.f || crgb.z != 0.f) {
+              const float3 o = offset_origin(h, ldir);
+              q.sh_org[i] = make_float4(o.x, o.y, o.z, len * 0.9999f);
+              q.sh_dir[i] = make_float4(ldir.x, ldir.y, ldir.z, 0.f);
+              q.sh_contrib[i] = make_float4(crgb.x, crgb.y, crgb.z, 0.f);
+              push_shadow = true;
+            }
+          }
+        }
+        // MaybeScatter (pt2.cxx:605-622) and PrepareStateForAfterScattering (:583-591)
+        rough = fmaxf(my_roughness(shd), rough);
+        Scatter s = sample_bsdf(shd, h, wi, bins, rough, lambda0, rng);
+        uint32_t node = (flags & kFlagNodeMask) + 1u;
+        bool alive = node <= (uint32_t)sc.max_depth;
+        if (alive) {
+          thru = (fabsf(dot(h.ns, s.dir)) / s.pdf) * (thru * s.value);
+          alive = !all_zero(thru) && isfinite(thru.x + thru.y + thru.z + thru.w);
+        }
+        if (alive) {
+          const float3 o = offset_origin(h, s.dir);
+          q.org[i] = make_float4(o.x, o.y, o.z, lambda0);
+          q.dir[i] = make_float4(s.dir.x, s.dir.y, s.dir.z, 0.f);
+          q.thru[i] = thru;
+          q.last_pdf[i] = s.pdf;
+          q.roughness[i] = rough;
+          flags = (flags & ~(kFlagNodeMask | kFlagLastPdfDelta)) | node | kFlagHasLastPdf | (s.delta ? kFlagLastPdfDelta : 0u);
+          push = true;
+        } else if (push_shadow) {
+          flags |= kFlagFinalizeAfterShadow;
+        } else {
+          finalize_path(q, fb, fb_sq, i, rad);
+        }
+        q.flags[i] = flags;
+        q.rad[i] = rad;
+        store_rng(q, i, rng);
+      }
+    }
+    const uint32_t slot = wave_append(count_out, push);
+    if (push) queue_out[slot] = i;
+    const uint32_t sslot = wave_append(q.counters + kCountShadow, push_shadow);
+    if (push_shadow) q.shadow_queue[sslot] = i;
+  }
+}
+
+// ---- shadow rays: any hit + accumulate + finalize -----------------------------------
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(DeviceScene sc, PathQueues q, double *fb, double *fb_sq, uint32_t *stats) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  const uint32_t count = q.counters[kCountShadow];
+  for (uint32_t k = gtid(); k < count; k += grid_stride()) {
+    const uint32_t i = q.shadow_queue[k];
+    const float4 o = q.sh_org[i];
+    Ray r;
+    r.org = xyz(o);
+    r.dir = xyz(q.sh_dir[i]);
+    r.tmax = o.w;
+    HitRec h;
+    bool occ = traverse<true>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
+    float4 rad = q.rad[i];
+    if (!occ) rad = add_rgb(rad, xyz(q.sh_contrib[i]));
+    if (q.flags[i] & kFlagFinalizeAfterShadow) finalize_path(q, fb, fb_sq, i, rad);
+    else q.rad[i] = rad;
+  }
+  if (gtid() == 0) atomicAdd(stats + kStatShadow, count);
+}
+
+// ---- debug / test kernels -------------------------------------------------------------
+__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DeviceScene sc, int64_t n, const float *org, const float *dir, const float *tmax,
+                                                            float *t, int32_t *prim, float *u, float *v, int32_t *occl) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  for (int64_t i = gtid(); i < n; i += grid_stride()) {
+    Ray r;
+    r.org = mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+    r.dir = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+    r.tmax = tmax[i];
+    HitRec h;
+    if (occl) {
+      occl[i] = traverse<true>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h) ? 1 : 0;
+    } else {
+      bool any = traverse<false>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
+      t[i] = any ? h.t : __int_as_float(0x7f800000);
+      prim[i] = any ? __float_as_int(sc.prims[h.prim].a.w) : -1;
+      u[i] = any ? h.u : 0.f;
+      v[i] = any ? h.v : 0.f;
+    }
+  }
+}
+
+__global__ void k_sample_shader(DevShader s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, uint64_t seed, int64_t n,
+                                float *out_dir, float *out_val, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) {
+    Pcg rng;
+    rng.seed(seed, (uint64_t)i);
+    SurfHit h;
+    h.pos = mk(0.f, 0.f, 0.f);
+    h.ng = h.n = ng;
+    h.ns = ns;
+    h.offset = 0.f;
+    h.material = 0;
+    h.prim = 0;
+    Scatter sc = sample_bsdf(s, h, wi, bins, rough, 370.f + 10.f * (bins & 0xFF) + 5.f, rng);
+    out_dir[3 * i] = sc.dir.x; out_dir[3 * i + 1] = sc.dir.y; out_dir[3 * i + 2] = sc.dir.z;
+    out_val[4 * i] = sc.value.x; out_val[4 * i + 1] = sc.value.y; out_val[4 * i + 2] = sc.value.z; out_val[4 * i + 3] = sc.value.w;
+    out_pdf[i] = sc.delta ? -sc.pdf : sc.pdf;
+  }
+}
+
+__global__ void k_eval_shader(DevShader s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, int64_t n, const float *dirs,
+                              float *out_val, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) {
+    SurfHit h;
+    h.pos = mk(0.f, 0.f, 0.f);
+    h.ng = h.n = ng;
+    h.ns = ns;
+    h.offset = 0.f;
+    h.material = 0;
+    h.prim = 0;
+    float pdf;
+    float4 v = eval_bsdf(s, h, wi, bins, rough, 370.f + 10.f * (bins & 0xFF) + 5.f, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), &pdf);
+    out_val[4 * i] = v.x; out_val[4 * i + 1] = v.y; out_val[4 * i + 2] = v.z; out_val[4 * i + 3] = v.w;
+    out_pdf[i] = pdf;
+  }
+}
+
+__global__ void k_sample_phase(int kind, float g, float3 wi, uint64_t seed, int64_t n, float *out_dir, float *out_val, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) {
+    Pcg rng;
+    rng.seed(seed, (uint64_t)i);
+    float3 d;
+    float v;
+    phase_sample(kind, g, wi, rng, d, v);
+    out_dir[3 * i] = d.x; out_dir[3 * i + 1] = d.y; out_dir[3 * i + 2] = d.z;
+    out_val[i] = v;
+    out_pdf[i] = v;
+  }
+}
+
+// ---- host-side launchers ------------------------------------------------------------------
+static int g_trace_blocks = 0;
+
+hipError_t UploadSpectralTables(const float *to_rgb, const float *smits) {
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_spec_to_rgb), to_rgb, sizeof(float) * 3 * kNBins);
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_smits), smits, sizeof(float) * 7 * kNBins);
+}
+
+static int Blocks(uint64_t n, int block, int cap) {
+  uint64_t b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  return (int)(b > (uint64_t)cap ? cap : b);
+}
+
+void LaunchSetCounters(uint32_t *counters, uint32_t a, uint32_t b, uint32_t s, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, counters, a, b, s);
+}
+void LaunchResetCounter(uint32_t *counters, int which, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counters, which);
+}
+void LaunchFill(double *p, size_t n, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_f64, dim3(Blocks(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+}
+void LaunchRaygen(const DeviceScene &sc, const PathQueues &q, uint64_t base, uint32_t count, uint32_t npix, uint32_t spp_base,
+                  uint64_t seed, hipStream_t st) {
+  hipLaunchKernelGGL(k_raygen, dim3(Blocks(count, 256, 65535)), dim3(256), 0, st, sc, q, base, count, npix, spp_base, seed);
+}
+void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue, const uint32_t *count, uint32_t max_count,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_trace, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, queue, count,
+                     q.counters);
+}
+void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
+                 uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
+  hipLaunchKernelGGL(k_shade, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out, count_out,
+                     fb, fb_sq);
+}
+void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
+  hipLaunchKernelGGL(k_shadow, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq, q.counters);
+}
+void LaunchTraceRays(const DeviceScene &sc, int64_t n, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
+                     float *u, float *v, int32_t *occl, hipStream_t st) {
+  hipLaunchKernelGGL(k_trace_rays, dim3(Blocks(n, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, n, org, dir, tmax, t, prim, u, v,
+                     occl);
+}
+void LaunchSampleShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, uint64_t seed, int64_t n,
+                        float *d, float *v, float *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_sample_shader, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, s, wi, ng, ns, bins, rough, seed, n, d, v, p);
+}
+void LaunchEvalShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, int64_t n, const float *dirs,
+                      float *v, float *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_eval_shader, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, s, wi, ng, ns, bins, rough, n, dirs, v, p);
+}
+void LaunchSamplePhase(int kind, float g, float3 wi, uint64_t seed, int64_t n, float *d, float *v, float *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_sample_phase, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, kind, g, wi, seed, n, d, v, p);
+}
+
+}  // namespace ttgpu

@@ -1,0 +1,27 @@
+// Host-callable launchers of the wavefront kernels (defined in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "device_scene.hpp"
+
+namespace ttgpu {
+
+hipError_t UploadSpectralTables(const float *to_rgb, const float *smits);
+void LaunchSetCounters(uint32_t *counters, uint32_t a, uint32_t b, uint32_t s, hipStream_t st);
+void LaunchResetCounter(uint32_t *counters, int which, hipStream_t st);
+void LaunchFill(double *p, size_t n, double v, hipStream_t st);
+void LaunchRaygen(const DeviceScene &sc, const PathQueues &q, uint64_t base, uint32_t count, uint32_t npix, uint32_t spp_base,
+                  uint64_t seed, hipStream_t st);
+void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue, const uint32_t *count, uint32_t max_count,
+                 hipStream_t st);
+void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
+                 uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st);
+void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st);
+void LaunchTraceRays(const DeviceScene &sc, int64_t n, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
+                     float *u, float *v, int32_t *occl, hipStream_t st);
+void LaunchSampleShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, uint64_t seed, int64_t n,
+                        float *d, float *v, float *p, hipStream_t st);
+void LaunchEvalShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, int64_t n, const float *dirs,
+                      float *v, float *p, hipStream_t st);
+void LaunchSamplePhase(int kind, float g, float3 wi, uint64_t seed, int64_t n, float *d, float *v, float *p, hipStream_t st);
+
+}  // namespace ttgpu

@@ -1,0 +1,876 @@
+// Host side of the C ABI (include/ttgpu.h): scene validation and flattening,
+// BVH build, device upload, the pass schedule of PathTracingAlgo2::Run
+// (src/renderingalgorithms_pathtracing2.cxx:269-295) and the wavefront loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ttgpu.h"
+#include "bvh_build.hpp"
+#include "device_scene.hpp"
+#include "kernels.hpp"
+
+#include "spectral_tables.inc"
+
+using namespace ttgpu;
+
+namespace {
+
+constexpr double kPiD = 3.14159265358979323846;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void Check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct V3d {
+  double x, y, z;
+};
+V3d operator-(V3d a, V3d b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3d operator+(V3d a, V3d b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3d operator*(V3d a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double Dot(V3d a, V3d b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3d Cross(V3d a, V3d b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+V3d Normalized(V3d a) { return a * (1.0 / std::sqrt(Dot(a, a))); }
+float3 F3(V3d a) { return make_float3((float)a.x, (float)a.y, (float)a.z); }
+
+struct Frame3 {
+  V3d x, y, z;
+};
+Frame3 FrameZ(V3d n) {  // Duff et al. 2017
+  double sign = std::copysign(1.0, n.z);
+  const double a = -1.0 / (sign + n.z);
+  const double b = n.x * n.y * a;
+  return {{1.0 + sign * n.x * n.x * a, sign * b, -sign * n.x}, {b, sign + n.y * n.y * a, -n.y}, n};
+}
+
+}  // namespace
+
+struct ttgpu_handle {
+  std::string err;
+  ttgpu_params params{};
+  hipStream_t stream = nullptr;
+  DeviceScene dsc{};
+  PathQueues q{};
+  std::vector<void *> allocs;
+  double *fb = nullptr, *fb_sq = nullptr;
+  std::vector<uint64_t> pixel_spp;
+  uint64_t total_spp = 0;
+  int sched_spp = 1;
+  std::atomic<bool> stop{false}, interrupt{false};
+  ttgpu_pass_callback cb = nullptr;
+  void *cb_user = nullptr;
+  ttgpu_stats stats{};
+  bool timing = false;
+  struct Timed {
+    double ms = 0;
+    int64_t launches = 0;
+  };
+  std::map<std::string, Timed> timers;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_events;
+  std::vector<hipEvent_t> event_pool;
+  size_t event_next = 0;
+  int npix = 0;
+  uint32_t capacity = 0;
+  // host copies needed later
+  std::vector<float> host_area;
+  int num_triangles = 0;
+
+  template <class T> T *Alloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    Check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    allocs.push_back(p);
+    return (T *)p;
+  }
+  template <class T> T *Upload(const T *src, size_t n) {
+    T *d = Alloc<T>(n);
+    if (n) Check(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return d;
+  }
+  hipEvent_t NextEvent() {
+    if (event_next == event_pool.size()) {
+      hipEvent_t e;
+      Check(hipEventCreate(&e), "hipEventCreate");
+      event_pool.push_back(e);
+    }
+    return event_pool[event_next++];
+  }
+  template <class F> void Timed(const char *name, F &&launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    hipEvent_t a = NextEvent(), b = NextEvent();
+    Check(hipEventRecord(a, stream), "hipEventRecord");
+    launch();
+    Check(hipEventRecord(b, stream), "hipEventRecord");
+    pending_events.push_back({name, {a, b}});
+  }
+  void CollectTimers() {
+    if (pending_events.empty()) return;
+    Check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    for (auto &pe : pending_events) {
+      float ms = 0.f;
+      Check(hipEventElapsedTime(&ms, pe.second.first, pe.second.second), "hipEventElapsedTime");
+      auto &t = timers[pe.first];
+      t.ms += ms;
+      t.launches += 1;
+    }
+    pending_events.clear();
+    event_next = 0;
+  }
+  ~ttgpu_handle() {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto e : event_pool) hipEventDestroy(e);
+    for (void *p : allocs) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+std::once_flag g_tables_once;
+hipError_t g_tables_err = hipSuccess;
+
+void EnsureTables() {
+  std::call_once(g_tables_once, [] {
+    float to_rgb[3 * 36], smits[7 * 36];
+    for (int i = 0; i < 3; ++i)
+      for (int b = 0; b < 36; ++b) to_rgb[i * 36 + b] = (float)ttgpu_tables::kSpecToRGB[i][b];
+    for (int i = 0; i < 7; ++i)
+      for (int b = 0; b < 36; ++b) smits[i * 36 + b] = (float)ttgpu_tables::kSmits[i][b];
+    g_tables_err = UploadSpectralTables(to_rgb, smits);
+  });
+  Check(g_tables_err, "upload spectral tables");
+}
+
+void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
+  auto bad = [](const std::string &m) { throw std::invalid_argument(m); };
+  if (p.width <= 0 || p.height <= 0) bad("width and height must be positive");
+  if ((int64_t)p.width * p.height > (int64_t)1 << 31) bad("image too large");
+  if (p.max_ray_depth < 2 || p.max_ray_depth > 255) bad("max_ray_depth must be in [2, 255]");
+  if (p.num_wavelengths != 0 && p.num_wavelengths != 4) bad("the GPU path traces 4 wavelengths per path");
+  if (p.sample_mode < 0 || p.sample_mode > 2) bad("bad sample_mode");
+  if (s.num_triangles < 0 || s.num_spheres < 0 || s.num_vertices < 0) bad("negative counts");
+  if (s.num_triangles > 0 && (!s.vertices || !s.indices || !s.tri_material)) bad("triangles need vertices, indices, materials");
+  if (s.num_spheres > 0 && (!s.spheres || !s.sphere_material)) bad("spheres need data and materials");
+  if (s.num_triangles + s.num_spheres >= ((int64_t)1 << 31) - 1) bad("too many primitives");
+  for (int64_t i = 0; i < 3 * s.num_triangles; ++i)
+    if (s.indices[i] >= (uint64_t)s.num_vertices) bad("triangle vertex index out of range");
+  auto check_mat = [&](int m) {
+    if (m < 0 || m >= s.num_materials) bad("primitive material out of range");
+  };
+  for (int64_t i = 0; i < s.num_triangles; ++i) check_mat(s.tri_material[i]);
+  for (int64_t i = 0; i < s.num_spheres; ++i) check_mat(s.sphere_material[i]);
+  for (int i = 0; i < s.num_materials; ++i) {
+    const auto &m = s.materials[i];
+    if (m.shader >= s.num_shaders || m.medium >= s.num_media || m.outer_medium >= s.num_media || m.emitter >= s.num_area_emitters)
+      bad("material references out of range");
+    if (m.shader < 0 && m.medium < 0 && m.outer_medium < 0) bad("material without shader and medium");
+    if (m.shader < 0 && m.emitter >= 0) bad("emissive volume boundaries are not supported");
+  }
+  for (int i = 0; i < s.num_shaders; ++i)
+    if (s.shaders[i].kind < 0 || s.shaders[i].kind > TTGPU_SHADER_DENSE_DIELECTRIC) bad("unknown shader kind");
+  if (s.num_env_lights > kMaxEnvLights) bad("too many environment lights");
+  for (int i = 0; i < s.num_env_lights; ++i) {
+    const auto &e = s.env_lights[i];
+    if (e.kind < 0 || e.kind > TTGPU_ENV_MAP) bad("unknown env light kind");
+    if (e.kind == TTGPU_ENV_MAP && (e.tex_width <= 0 || e.tex_height <= 0 || !e.texture_rgb)) bad("env map needs a texture");
+  }
+  if (s.num_media > kMaxMedia) bad("too many media");
+  if (s.num_media > 0 || s.empty_space_medium >= 0) {
+    bool any_medium = s.empty_space_medium >= 0;
+    for (int i = 0; i < s.num_materials; ++i) any_medium |= s.materials[i].medium >= 0 || s.materials[i].outer_medium >= 0;
+    if (any_medium) bad("participating media are handled by the media kernels, which this build does not enable yet");
+  }
+  if (s.camera.kind != TTGPU_CAMERA_PINHOLE && s.camera.kind != TTGPU_CAMERA_FISHEYE) bad("unknown camera kind");
+}
+
+float PrimAreaF(const ttgpu_scene &s, int64_t pid) {
+  if (pid < s.num_triangles) {
+    const float *v = s.vertices;
+    const uint32_t *ix = s.indices + 3 * pid;
+    float a[3], b[3], c[3];
+    for (int k = 0; k < 3; ++k) { a[k] = v[3 * ix[0] + k]; b[k] = v[3 * ix[1] + k]; c[k] = v[3 * ix[2] + k]; }
+    float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    return 0.5f * std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  }
+  float r = s.spheres[4 * (pid - s.num_triangles) + 3];
+  return (float)(r * r * 4. * kPiD);
+}
+
+void BuildGeometry(ttgpu_handle *h, const ttgpu_scene &s, const std::vector<int64_t> &ids, bool surfaces) {
+  std::vector<BuildPrim> bp(ids.size());
+  for (size_t k = 0; k < ids.size(); ++k) {
+    int64_t pid = ids[k];
+    BuildPrim &b = bp[k];
+    if (pid < s.num_triangles) {
+      for (int a = 0; a < 3; ++a) { b.lo[a] = 3e38f; b.hi[a] = -3e38f; }
+      for (int j = 0; j < 3; ++j) {
+        const float *v = s.vertices + 3 * s.indices[3 * pid + j];
+        for (int a = 0; a < 3; ++a) { b.lo[a] = std::min(b.lo[a], v[a]); b.hi[a] = std::max(b.hi[a], v[a]); }
+      }
+    } else {
+      const float *sp = s.spheres + 4 * (pid - s.num_triangles);
+      for (int a = 0; a < 3; ++a) {
+        b.lo[a] = std::nextafter(sp[a] - sp[3], -3e38f);
+        b.hi[a] = std::nextafter(sp[a] + sp[3], 3e38f);
+      }
+    }
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  BuiltBvh bvh = BuildBvh(bp, 4, h->params.num_threads);
+  auto t1 = std::chrono::steady_clock::now();
+  h->stats.bvh_build_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (bvh.max_depth >= 63) throw std::runtime_error("BVH deeper than the traversal stack (63 levels)");
+  const size_t n = ids.size();
+  std::vector<PrimRecord> recs(std::max<size_t>(n, 1));
+  std::vector<int> mats(std::max<size_t>(n, 1), 0);
+  std::vector<float> areas(std::max<size_t>(n, 1), 0.f);
+  std::vector<float4> normals;
+  const bool smooth = surfaces && s.normals != nullptr;
+  if (smooth) normals.resize(3 * std::max<size_t>(n, 1));
+  for (size_t k = 0; k < n; ++k) {
+    int64_t pid = ids[bvh.order[k]];
+    PrimRecord r{};
+    int orig = (int)pid;
+    if (pid < s.num_triangles) {
+      const uint32_t *ix = s.indices + 3 * pid;
+      const float *v0 = s.vertices + 3 * ix[0], *v1 = s.vertices + 3 * ix[1], *v2 = s.vertices + 3 * ix[2];
+      r.a = make_float4(v0[0], v0[1], v0[2], 0.f);
+      r.b = make_float4(v1[0], v1[1], v1[2], 0.f);
+      r.c = make_float4(v2[0], v2[1], v2[2], 0.f);
+      std::memcpy(&r.a.w, &orig, 4);
+      int kind = 0;
+      std::memcpy(&r.b.w, &kind, 4);
+      mats[k] = s.tri_material[pid];
+      if (smooth)
+        for (int j = 0; j < 3; ++j) {
+          const float *nn = s.normals + 3 * ix[j];
+          normals[3 * k + j] = make_float4(nn[0], nn[1], nn[2], 0.f);
+        }
+    } else {
+      const float *sp = s.spheres + 4 * (pid - s.num_triangles);
+      r.a = make_float4(sp[0], sp[1], sp[2], 0.f);
+      r.b = make_float4(sp[3], 0.f, 0.f, 0.f);
+      std::memcpy(&r.a.w, &orig, 4);
+      int kind = 1;
+      std::memcpy(&r.b.w, &kind, 4);
+      mats[k] = s.sphere_material[pid - s.num_triangles];
+      if (smooth)
+        for (int j = 0; j < 3; ++j) normals[3 * k + j] = make_float4(0.f, 0.f, 1.f, 0.f);
+    }
+    areas[k] = PrimAreaF(s, pid);
+  }
+  if (bvh.num_nodes == 0) {
+    // No primitives: one node whose two children are empty leaves, so traversal ends at once.
+    bvh.nodes.assign(16, 0.f);
+    int ch[4] = {~0, ~0, 0, 0};
+    std::memcpy(bvh.nodes.data() + 12, ch, sizeof(ch));
+    bvh.num_nodes = 1;
+  }
+  const BvhNode *nodes = h->Upload((const BvhNode *)bvh.nodes.data(), bvh.num_nodes);
+  if (surfaces) {
+    h->dsc.nodes = nodes;
+    h->dsc.prims = h->Upload(recs.data(), recs.size());
+    h->dsc.prim_material = h->Upload(mats.data(), mats.size());
+    h->dsc.prim_area = h->Upload(areas.data(), areas.size());
+    h->dsc.prim_normals = smooth ? h->Upload(normals.data(), normals.size()) : nullptr;
+    h->dsc.num_prims = (int)n;
+    h->stats.bvh_nodes = bvh.num_nodes;
+    h->stats.scene_bytes += bvh.nodes.size() * 4 + recs.size() * sizeof(PrimRecord) + mats.size() * 4 + areas.size() * 4 +
+                            normals.size() * sizeof(float4);
+    // Area lights: emissive surface primitives.
+    std::vector<int> lights;
+    for (size_t k = 0; k < n; ++k)
+      if (s.materials[mats[k]].emitter >= 0) lights.push_back((int)k);
+    h->dsc.area_lights = h->Upload(lights.data(), lights.size());
+    h->dsc.num_area_lights = (int)lights.size();
+  } else {
+    h->dsc.vol_nodes = nodes;
+    h->dsc.vol_prims = h->Upload(recs.data(), recs.size());
+    h->dsc.vol_material = h->Upload(mats.data(), mats.size());
+    h->dsc.num_vol_prims = (int)n;
+  }
+}
+
+void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
+  std::vector<DevShader> shaders(std::max(1, s.num_shaders));
+  for (int i = 0; i < s.num_shaders; ++i) {
+    const auto &a = s.shaders[i];
+    DevShader &d = shaders[i];
+    d.kind = a.kind; d.alpha = a.alpha; d.alpha_min = a.alpha_min; d.ior_ratio = a.ior_ratio;
+    d.ior_lambda_coeff = a.ior_lambda_coeff; d.specular = a.specular;
+    std::memcpy(d.reflectance, a.reflectance, sizeof(d.reflectance));
+  }
+  h->dsc.shaders = h->Upload(shaders.data(), shaders.size());
+  std::vector<DevMaterial> mats(std::max(1, s.num_materials));
+  for (int i = 0; i < s.num_materials; ++i)
+    mats[i] = {s.materials[i].shader, s.materials[i].medium, s.materials[i].outer_medium, s.materials[i].emitter};
+  h->dsc.materials = h->Upload(mats.data(), mats.size());
+  std::vector<DevEmitter> ems(std::max(1, s.num_area_emitters));
+  for (int i = 0; i < s.num_area_emitters; ++i) {
+    ems[i].kind = s.area_emitters[i].kind;
+    // UniformAreaLight: spectrum / (2 pi) is the exitant radiance (src/light.hxx:77-80).
+    for (int b = 0; b < kNBins; ++b) ems[i].radiance[b] = (float)(s.area_emitters[i].spectrum[b] / (2. * kPiD));
+  }
+  h->dsc.emitters = h->Upload(ems.data(), ems.size());
+  std::vector<DevPointLight> pts(std::max(1, s.num_point_lights));
+  for (int i = 0; i < s.num_point_lights; ++i) {
+    const auto &p = s.point_lights[i];
+    pts[i].pos = make_float3((float)p.pos[0], (float)p.pos[1], (float)p.pos[2]);
+    for (int b = 0; b < kNBins; ++b) pts[i].intensity[b] = (float)(p.spectrum[b] / (4. * kPiD));
+  }
+  h->dsc.points = h->Upload(pts.data(), pts.size());
+  h->dsc.num_points = s.num_point_lights;
+  // Environment lights.
+  std::vector<DevEnvLight> envs(std::max(1, s.num_env_lights));
+  for (int i = 0; i < s.num_env_lights; ++i) {
+    const auto &e = s.env_lights[i];
+    DevEnvLight &d = envs[i];
+    std::memset(&d, 0, sizeof(d));
+    d.kind = e.kind;
+    V3d dir = Normalized({e.dir[0], e.dir[1], e.dir[2]});
+    if (e.kind == TTGPU_ENV_SUN) {
+      // Sun(total_power, -direction, opening angle) (src/parse_yaml_scene.cxx:555-562, light.hxx:254-263)
+      double cos_open = std::cos(kPiD / 180. * e.opening_angle_deg);
+      double pdf = 1. / (2. * kPiD * (1. - cos_open));
+      Frame3 f = FrameZ(dir * -1.0);
+      d.frame_x = F3(f.x); d.frame_y = F3(f.y); d.frame_z = F3(f.z);
+      d.cos_open = (float)cos_open;
+      d.pdf = (float)pdf;
+      for (int b = 0; b < kNBins; ++b) d.spectrum[b] = (float)(ttgpu_tables::kSunSpectrum[b] * e.total_power * pdf);
+    } else if (e.kind == TTGPU_ENV_DOME) {
+      Frame3 f = FrameZ(dir * -1.0);
+      d.frame_x = F3(f.x); d.frame_y = F3(f.y); d.frame_z = F3(f.z);
+      d.dir = F3(dir);
+      std::memcpy(d.spectrum, e.spectrum, sizeof(d.spectrum));
+    } else if (e.kind == TTGPU_ENV_DIRECTIONAL) {
+      d.dir = F3(dir * -1.0);  // direction the light travels
+      std::memcpy(d.spectrum, e.spectrum, sizeof(d.spectrum));
+    } else {
+      Frame3 f = FrameZ(dir);
+      d.frame_x = F3(f.x); d.frame_y = F3(f.y); d.frame_z = F3(f.z);
+      const int w = e.tex_width, hh = e.tex_height;
+      d.tex_w = w; d.tex_h = hh;
+      // Row/column CMFs weighted by sin(theta) (src/light.cxx:88-116).
+      std::vector<double> rows(hh), cols((size_t)w * hh);
+      for (int y = 0; y < hh; ++y) {
+        double row = 0.;
+        double theta = kPiD * (((double)(hh - 1 - y) + 0.5) / hh);
+        for (int x = 0; x < w; ++x) {
+          const float *px = e.texture_rgb + 3 * ((size_t)y * w + x);
+          row += std::sin(theta) * ((double)px[0] + px[1] + px[2]) / 3.;
+          cols[(size_t)y * w + x] = row;
+        }
+        for (int x = 0; x < w; ++x) cols[(size_t)y * w + x] = row > 0 ? cols[(size_t)y * w + x] / row : (double)(x + 1) / w;
+        rows[y] = row;
+      }
+      for (int y = 1; y < hh; ++y) rows[y] += rows[y - 1];
+      for (int y = 0; y < hh; ++y) rows[y] /= rows[hh - 1];
+      std::vector<float> rf(rows.begin(), rows.end()), cf(cols.begin(), cols.end());
+      d.cdf_rows = h->Upload(rf.data(), rf.size());
+      d.cdf_cols = h->Upload(cf.data(), cf.size());
+      d.tex = h->Upload(e.texture_rgb, 3 * (size_t)w * hh);
+      h->stats.scene_bytes += (rf.size() + cf.size() + 3 * (size_t)w * hh) * 4;
+    }
+  }
+  h->dsc.envs = h->Upload(envs.data(), envs.size());
+  h->dsc.num_envs = s.num_env_lights;
+  // Light type selection probabilities proportional to counts (src/lightpicker_trivial.cxx:35-51).
+  double counts[4] = {s.num_env_lights ? 1. : 0., (double)h->dsc.num_area_lights, (double)s.num_point_lights, 0.};
+  double tot = counts[0] + counts[1] + counts[2] + counts[3];
+  for (int k = 0; k < 4; ++k) h->dsc.type_probs[k] = tot > 0 ? (float)(counts[k] / tot) : 0.f;
+  h->dsc.area_pick_prob = h->dsc.num_area_lights ? (float)(counts[1] / tot / h->dsc.num_area_lights) : 0.f;
+  h->dsc.bb_diag = (float)bb_diag;
+  h->dsc.empty_space_medium = kVacuum;
+  h->dsc.has_media = 0;
+  h->dsc.cam_tracker = ((uint64_t)kEmptySlot << 32) | ((uint64_t)kEmptySlot << 40) | ((uint64_t)kEmptySlot << 48) |
+                       ((uint64_t)kEmptySlot << 56) | (uint64_t)kVacuum;
+  // Camera (src/camera.hxx:43-93, 165-174).
+  const auto &c = s.camera;
+  V3d up = Normalized({c.up[0], c.up[1], c.up[2]}), dir = Normalized({c.dir[0], c.dir[1], c.dir[2]});
+  double r = Dot(dir, up);
+  if (std::abs(r) <= 1.0 - 2.220446049250313e-16) up = Normalized(up - dir * r);
+  V3d right = Cross(up, dir);
+  DevCamera &dc = h->dsc.cam;
+  dc.kind = c.kind;
+  dc.pos = make_float3((float)c.pos[0], (float)c.pos[1], (float)c.pos[2]);
+  dc.right = F3(right); dc.up = F3(up); dc.dir = F3(dir);
+  const int W = h->params.width, H = h->params.height;
+  double fov = c.fov_deg * kPiD / 180.;
+  double a = std::tan(fov * 0.5), b = a * (double)H / W;
+  dc.xmin = (float)-a; dc.ymin = (float)-b; dc.xper = (float)(2. * a / W); dc.yper = (float)(2. * b / H);
+  dc.fish_delta = (float)(2. / std::min(W, H));
+  dc.width = W; dc.height = H;
+  h->dsc.sample_mode = h->params.sample_mode;
+  h->dsc.mis_mode = h->params.mis_mode;
+  h->dsc.max_depth = h->params.max_ray_depth;
+}
+
+void AllocateQueues(ttgpu_handle *h) {
+  const uint32_t N = h->capacity;
+  PathQueues &q = h->q;
+  q.capacity = (int)N;
+  q.org = h->Alloc<float4>(N); q.dir = h->Alloc<float4>(N); q.hit = h->Alloc<float4>(N);
+  q.thru = h->Alloc<float4>(N); q.rad = h->Alloc<float4>(N);
+  q.bins = h->Alloc<uint32_t>(N); q.pixel = h->Alloc<uint32_t>(N);
+  q.rng_state = h->Alloc<uint2>(N); q.rng_inc = h->Alloc<uint2>(N);
+  q.flags = h->Alloc<uint32_t>(N); q.last_pdf = h->Alloc<float>(N); q.roughness = h->Alloc<float>(N);
+  q.tracker = h->Alloc<uint2>(N);
+  q.sh_org = h->Alloc<float4>(N); q.sh_dir = h->Alloc<float4>(N); q.sh_contrib = h->Alloc<float4>(N);
+  q.queue_a = h->Alloc<uint32_t>(N); q.queue_b = h->Alloc<uint32_t>(N); q.shadow_queue = h->Alloc<uint32_t>(N);
+  q.counters = h->Alloc<uint32_t>(kNumCounters);
+  Check(hipMemset(q.counters, 0, kNumCounters * sizeof(uint32_t)), "hipMemset counters");
+  q.w_track = nullptr;
+  q.w_track_null = nullptr;
+  h->stats.state_bytes = (size_t)N * (16 * 8 + 4 * 6 + 8 * 3 + 12);
+  const size_t npix = (size_t)h->npix;
+  h->fb = h->Alloc<double>(3 * npix);
+  h->fb_sq = h->Alloc<double>(3 * npix);
+  LaunchFill(h->fb, 3 * npix, 0.0, h->stream);
+  LaunchFill(h->fb_sq, 3 * npix, 0.0, h->stream);
+  Check(hipGetLastError(), "fill framebuffer");
+  Check(hipStreamSynchronize(h->stream), "sync");
+}
+
+int Fail(ttgpu_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define TTGPU_GUARD(h, ...)                                               \
+  try {                                                                   \
+    __VA_ARGS__                                                           \
+  } catch (const std::invalid_argument &e) {                              \
+    return Fail(h, TTGPU_ERR_INVALID_ARGUMENT, e.what());                 \
+  } catch (const HipError &e) {                                           \
+    return Fail(h, TTGPU_ERR_DEVICE, e.what());                           \
+  } catch (const std::bad_alloc &e) {                                     \
+    return Fail(h, TTGPU_ERR_OUT_OF_MEMORY, "host allocation failed");    \
+  } catch (const std::exception &e) {                                     \
+    return Fail(h, TTGPU_ERR_RUNTIME, e.what());                          \
+  }
+
+// Renders `spp` samples per pixel; returns false if a stop cut the pass short.
+bool RenderPass(ttgpu_handle *h, int spp) {
+  const uint64_t npix = (uint64_t)h->npix;
+  const uint64_t total = npix * (uint64_t)spp;
+  const uint32_t cap = h->capacity;
+  const int bounces = h->params.max_ray_depth - 1;
+  PathQueues &q = h->q;
+  auto t0 = std::chrono::steady_clock::now();
+  hipEvent_t ev0 = h->NextEvent(), ev1 = h->NextEvent();
+  Check(hipEventRecord(ev0, h->stream), "hipEventRecord");
+  bool complete = true;
+  for (uint64_t base = 0; base < total; base += cap) {
+    if (h->stop.load()) { complete = false; break; }
+    if (h->interrupt.exchange(false)) {
+      Check(hipStreamSynchronize(h->stream), "sync");
+      if (h->cb) h->cb(h->cb_user, 0);
+    }
+    const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - base);
+    h->Timed("raygen", [&] { LaunchRaygen(h->dsc, q, base, n, (uint32_t)npix, (uint32_t)h->total_spp, h->params.seed, h->stream); });
+    LaunchSetCounters(q.counters, n, 0, 0, h->stream);
+    uint32_t *cur_q = q.queue_a, *nxt_q = q.queue_b;
+    uint32_t *cur_c = q.counters + kCountA, *nxt_c = q.counters + kCountB;
+    int nxt_idx = kCountB, cur_idx = kCountA;
+    for (int b = 0; b < bounces; ++b) {
+      h->Timed("trace", [&] { LaunchTrace(h->dsc, q, cur_q, cur_c, n, h->stream); });
+      LaunchResetCounter(q.counters, nxt_idx, h->stream);
+      LaunchResetCounter(q.counters, kCountShadow, h->stream);
+      h->Timed("shade", [&] { LaunchShade(h->dsc, q, cur_q, cur_c, nxt_q, nxt_c, n, h->fb, h->fb_sq, h->stream); });
+      h->Timed("shadow", [&] { LaunchShadow(h->dsc, q, n, h->fb, h->fb_sq, h->stream); });
+      std::swap(cur_q, nxt_q);
+      std::swap(cur_c, nxt_c);
+      std::swap(cur_idx, nxt_idx);
+    }
+    Check(hipGetLastError(), "kernel launch");
+    // Per-pixel sample counts of this chunk (global index g -> pixel g % npix).
+    uint64_t full = n / npix, rem = n % npix, start = base % npix;
+    if (full)
+      for (auto &c : h->pixel_spp) c += full;
+    for (uint64_t k = 0; k < rem; ++k) h->pixel_spp[(start + k) % npix] += 1;
+    h->stats.paths_traced += n;
+  }
+  Check(hipEventRecord(ev1, h->stream), "hipEventRecord");
+  Check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  float ms = 0.f;
+  Check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+  h->stats.render_ms = ms;
+  (void)t0;
+  h->CollectTimers();
+  h->event_next = 0;
+  uint32_t counters[kNumCounters];
+  Check(hipMemcpy(counters, q.counters, sizeof(counters), hipMemcpyDeviceToHost), "read counters");
+  h->stats.rays_closest = counters[kStatClosest];
+  h->stats.rays_shadow = counters[kStatShadow];
+  if (complete) h->total_spp += spp;
+  h->stats.total_spp = h->total_spp;
+  h->stats.passes += 1;
+  return complete;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ttgpu_abi_version(void) { return TTGPU_ABI_VERSION; }
+
+const char *ttgpu_device_info(void) {
+  static std::string info;
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+    info = "no HIP device";
+  } else {
+    info = std::string(p.gcnArchName) + ", " + std::to_string(p.multiProcessorCount) + " CUs, " +
+           std::to_string(p.totalGlobalMem >> 30) + " GiB, " + p.name;
+  }
+  return info.c_str();
+}
+
+const char *ttgpu_create_error(void) { return g_create_error.c_str(); }
+
+int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_handle **out) {
+  if (!scene || !params || !out) {
+    g_create_error = "null argument";
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  ttgpu_handle *h = nullptr;
+  try {
+    Validate(*scene, *params);
+    EnsureTables();
+    h = new ttgpu_handle();
+    h->params = *params;
+    if (h->params.num_wavelengths == 0) h->params.num_wavelengths = 4;
+    h->npix = params->width * params->height;
+    h->pixel_spp.assign(h->npix, 0);
+    h->sched_spp = 1;
+    Check(hipStreamCreate(&h->stream), "hipStreamCreate");
+    auto t0 = std::chrono::steady_clock::now();
+    const ttgpu_scene &s = *scene;
+    std::vector<int64_t> surf, vol;
+    const int64_t nprim = s.num_triangles + s.num_spheres;
+    for (int64_t p = 0; p < nprim; ++p) {
+      int m = p < s.num_triangles ? s.tri_material[p] : s.sphere_material[p - s.num_triangles];
+      (s.materials[m].shader >= 0 ? surf : vol).push_back(p);
+    }
+    h->num_triangles = (int)s.num_triangles;
+    h->dsc.num_triangles = (int)s.num_triangles;
+    BuildGeometry(h, s, surf, true);
+    BuildGeometry(h, s, vol, false);
+    // Scene bounds over all primitives (src/scene.cxx:139-140).
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t p = 0; p < nprim; ++p) {
+      if (p < s.num_triangles) {
+        for (int j = 0; j < 3; ++j) {
+          const float *v = s.vertices + 3 * s.indices[3 * p + j];
+          for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], (double)v[a]); hi[a] = std::max(hi[a], (double)v[a]); }
+        }
+      } else {
+        const float *sp = s.spheres + 4 * (p - s.num_triangles);
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], (double)sp[a] - sp[3]); hi[a] = std::max(hi[a], (double)sp[a] + sp[3]); }
+      }
+    }
+    double diag = nprim ? std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]))
+                        : 1.0;
+    BuildTables(h, s, diag);
+    int64_t cap = params->wavefront_paths > 0 ? params->wavefront_paths : (int64_t)1 << 22;
+    cap = std::min<int64_t>(cap, (int64_t)1 << 26);
+    h->capacity = (uint32_t)cap;
+    h->stats.wavefront_paths = (int32_t)cap;
+    AllocateQueues(h);
+    auto t1 = std::chrono::steady_clock::now();
+    h->stats.upload_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() - h->stats.bvh_build_ms;
+    *out = h;
+    return TTGPU_OK;
+  } catch (const std::invalid_argument &e) {
+    g_create_error = e.what();
+    delete h;
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  } catch (const HipError &e) {
+    g_create_error = e.what();
+    delete h;
+    return TTGPU_ERR_DEVICE;
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    delete h;
+    return TTGPU_ERR_RUNTIME;
+  }
+}
+
+void ttgpu_destroy(ttgpu_handle *h) { delete h; }
+
+const char *ttgpu_last_error(const ttgpu_handle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int ttgpu_request_interrupt(ttgpu_handle *h) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  h->interrupt.store(true);
+  return TTGPU_OK;
+}
+
+int ttgpu_request_stop(ttgpu_handle *h) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  h->stop.store(true);
+  return TTGPU_OK;
+}
+
+int ttgpu_set_pass_callback(ttgpu_handle *h, ttgpu_pass_callback cb, void *user) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  h->cb = cb;
+  h->cb_user = user;
+  return TTGPU_OK;
+}
+
+int ttgpu_render_spp(ttgpu_handle *h, int32_t spp) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (spp <= 0) return Fail(h, TTGPU_ERR_INVALID_ARGUMENT, "spp must be positive");
+  TTGPU_GUARD(h, { return RenderPass(h, spp) ? TTGPU_OK : TTGPU_ERR_INTERRUPTED; })
+}
+
+int ttgpu_run(ttgpu_handle *h, int32_t max_passes) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(h, {
+    constexpr int kMaxSppPerIteration = 1024;
+    const int max_spp = h->params.max_samples_per_pixel;
+    int passes = 0;
+    // SamplesPerPixelSchedule (src/renderingalgorithms_simplebase.hxx:34-62).
+    if (max_spp > 0 && (int64_t)h->total_spp + h->sched_spp > max_spp) h->sched_spp = max_spp - (int)h->total_spp;
+    while (!h->stop.load() && h->sched_spp > 0 && (max_passes <= 0 || passes < max_passes)) {
+      RenderPass(h, h->sched_spp);
+      ++passes;
+      if (h->cb) h->cb(h->cb_user, 1);
+      if (h->sched_spp < kMaxSppPerIteration) h->sched_spp *= 2;
+      if (max_spp > 0 && (int64_t)h->total_spp + h->sched_spp > max_spp) h->sched_spp = max_spp - (int)h->total_spp;
+    }
+    return h->stop.load() ? TTGPU_ERR_INTERRUPTED : TTGPU_OK;
+  })
+}
+
+int ttgpu_reset_framebuffer(ttgpu_handle *h) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(h, {
+    LaunchFill(h->fb, 3 * (size_t)h->npix, 0.0, h->stream);
+    LaunchFill(h->fb_sq, 3 * (size_t)h->npix, 0.0, h->stream);
+    Check(hipGetLastError(), "fill");
+    Check(hipStreamSynchronize(h->stream), "sync");
+    std::fill(h->pixel_spp.begin(), h->pixel_spp.end(), 0);
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_read_framebuffer(ttgpu_handle *h, double *rgb, double *rgb_sq, uint64_t *spp) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(h, {
+    Check(hipStreamSynchronize(h->stream), "sync");
+    if (rgb) Check(hipMemcpy(rgb, h->fb, 3 * (size_t)h->npix * sizeof(double), hipMemcpyDeviceToHost), "read fb");
+    if (rgb_sq) Check(hipMemcpy(rgb_sq, h->fb_sq, 3 * (size_t)h->npix * sizeof(double), hipMemcpyDeviceToHost), "read fb_sq");
+    if (spp) std::memcpy(spp, h->pixel_spp.data(), h->pixel_spp.size() * sizeof(uint64_t));
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_generate_image(ttgpu_handle *h, uint8_t *rgb8) {
+  if (!h || !rgb8) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(h, {
+    std::vector<double> fb(3 * (size_t)h->npix);
+    Check(hipStreamSynchronize(h->stream), "sync");
+    Check(hipMemcpy(fb.data(), h->fb, fb.size() * sizeof(double), hipMemcpyDeviceToHost), "read fb");
+    const int W = h->params.width, H = h->params.height;
+    // framebuffer::ToImage (src/renderbuffer.hxx:99-125): average, clamp, sRGB, rows flipped.
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        const size_t p = (size_t)y * W + x;
+        const double n = (double)h->pixel_spp[p];
+        uint8_t *dst = rgb8 + 3 * ((size_t)(H - 1 - y) * W + x);
+        for (int c = 0; c < 3; ++c) {
+          double v = n > 0 ? fb[3 * p + c] / n : 0.0;
+          v = std::min(1.0, std::max(0.0, v));
+          if (!h->params.linear_output) v = v <= 0.0031308 ? 12.92 * v : 1.055 * std::pow(v, 1.0 / 2.4) - 0.055;
+          dst[c] = (uint8_t)(v * 255.999);
+        }
+      }
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_stats_get(ttgpu_handle *h, ttgpu_stats *out) {
+  if (!h || !out) return TTGPU_ERR_INVALID_ARGUMENT;
+  *out = h->stats;
+  return TTGPU_OK;
+}
+
+int ttgpu_enable_kernel_timing(ttgpu_handle *h, int32_t enable) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  h->timing = enable != 0;
+  h->timers.clear();
+  return TTGPU_OK;
+}
+
+int ttgpu_kernel_time(ttgpu_handle *h, const char *name, double *total_ms, int64_t *launches) {
+  if (!h || !name) return TTGPU_ERR_INVALID_ARGUMENT;
+  auto it = h->timers.find(name);
+  if (total_ms) *total_ms = it == h->timers.end() ? 0.0 : it->second.ms;
+  if (launches) *launches = it == h->timers.end() ? 0 : it->second.launches;
+  return TTGPU_OK;
+}
+
+int ttgpu_trace_closest(ttgpu_handle *h, int64_t count, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
+                        float *u, float *v) {
+  if (!h || count < 0 || (count && (!org || !dir || !tmax || !t || !prim || !u || !v))) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (count == 0) return TTGPU_OK;
+  TTGPU_GUARD(h, {
+    float *d_org = nullptr, *d_dir = nullptr, *d_tmax = nullptr, *d_t = nullptr, *d_u = nullptr, *d_v = nullptr;
+    int32_t *d_prim = nullptr;
+    size_t n = (size_t)count;
+    Check(hipMalloc(&d_org, 3 * n * 4), "hipMalloc"); Check(hipMalloc(&d_dir, 3 * n * 4), "hipMalloc");
+    Check(hipMalloc(&d_tmax, n * 4), "hipMalloc"); Check(hipMalloc(&d_t, n * 4), "hipMalloc");
+    Check(hipMalloc(&d_u, n * 4), "hipMalloc"); Check(hipMalloc(&d_v, n * 4), "hipMalloc");
+    Check(hipMalloc(&d_prim, n * 4), "hipMalloc");
+    Check(hipMemcpy(d_org, org, 3 * n * 4, hipMemcpyHostToDevice), "H2D");
+    Check(hipMemcpy(d_dir, dir, 3 * n * 4, hipMemcpyHostToDevice), "H2D");
+    Check(hipMemcpy(d_tmax, tmax, n * 4, hipMemcpyHostToDevice), "H2D");
+    if (h->dsc.num_prims > 0) {
+      LaunchTraceRays(h->dsc, count, d_org, d_dir, d_tmax, d_t, d_prim, d_u, d_v, nullptr, h->stream);
+      Check(hipGetLastError(), "launch");
+      Check(hipStreamSynchronize(h->stream), "sync");
+      Check(hipMemcpy(t, d_t, n * 4, hipMemcpyDeviceToHost), "D2H");
+      Check(hipMemcpy(prim, d_prim, n * 4, hipMemcpyDeviceToHost), "D2H");
+      Check(hipMemcpy(u, d_u, n * 4, hipMemcpyDeviceToHost), "D2H");
+      Check(hipMemcpy(v, d_v, n * 4, hipMemcpyDeviceToHost), "D2H");
+    } else {
+      for (size_t i = 0; i < n; ++i) { t[i] = INFINITY; prim[i] = -1; u[i] = v[i] = 0.f; }
+    }
+    hipFree(d_org); hipFree(d_dir); hipFree(d_tmax); hipFree(d_t); hipFree(d_u); hipFree(d_v); hipFree(d_prim);
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_trace_occluded(ttgpu_handle *h, int64_t count, const float *org, const float *dir, const float *tmax, int32_t *occluded) {
+  if (!h || count < 0 || (count && (!org || !dir || !tmax || !occluded))) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (count == 0) return TTGPU_OK;
+  TTGPU_GUARD(h, {
+    size_t n = (size_t)count;
+    float *d_org = nullptr, *d_dir = nullptr, *d_tmax = nullptr;
+    int32_t *d_occ = nullptr;
+    Check(hipMalloc(&d_org, 3 * n * 4), "hipMalloc"); Check(hipMalloc(&d_dir, 3 * n * 4), "hipMalloc");
+    Check(hipMalloc(&d_tmax, n * 4), "hipMalloc"); Check(hipMalloc(&d_occ, n * 4), "hipMalloc");
+    Check(hipMemcpy(d_org, org, 3 * n * 4, hipMemcpyHostToDevice), "H2D");
+    Check(hipMemcpy(d_dir, dir, 3 * n * 4, hipMemcpyHostToDevice), "H2D");
+    Check(hipMemcpy(d_tmax, tmax, n * 4, hipMemcpyHostToDevice), "H2D");
+    if (h->dsc.num_prims > 0) {
+      LaunchTraceRays(h->dsc, count, d_org, d_dir, d_tmax, nullptr, nullptr, nullptr, nullptr, d_occ, h->stream);
+      Check(hipGetLastError(), "launch");
+      Check(hipStreamSynchronize(h->stream), "sync");
+      Check(hipMemcpy(occluded, d_occ, n * 4, hipMemcpyDeviceToHost), "D2H");
+    } else {
+      for (size_t i = 0; i < n; ++i) occluded[i] = 0;
+    }
+    hipFree(d_org); hipFree(d_dir); hipFree(d_tmax); hipFree(d_occ);
+    return TTGPU_OK;
+  })
+}
+
+static DevShader ToDev(const ttgpu_shader &a) {
+  DevShader d;
+  d.kind = a.kind; d.alpha = a.alpha; d.alpha_min = a.alpha_min; d.ior_ratio = a.ior_ratio;
+  d.ior_lambda_coeff = a.ior_lambda_coeff; d.specular = a.specular;
+  std::memcpy(d.reflectance, a.reflectance, sizeof(d.reflectance));
+  return d;
+}
+
+static int SamplerCall(int64_t count, size_t out_floats_per, std::function<void(float *, float *, float *)> launch, float *od, float *ov,
+                       float *op, size_t dir_floats, size_t val_floats) {
+  float *dd = nullptr, *dv = nullptr, *dp = nullptr;
+  size_t n = (size_t)count;
+  try {
+    EnsureTables();
+    if (dir_floats) Check(hipMalloc(&dd, dir_floats * n * 4), "hipMalloc");
+    Check(hipMalloc(&dv, val_floats * n * 4), "hipMalloc");
+    Check(hipMalloc(&dp, n * 4), "hipMalloc");
+    launch(dd, dv, dp);
+    Check(hipGetLastError(), "launch");
+    Check(hipDeviceSynchronize(), "sync");
+    if (dir_floats) Check(hipMemcpy(od, dd, dir_floats * n * 4, hipMemcpyDeviceToHost), "D2H");
+    Check(hipMemcpy(ov, dv, val_floats * n * 4, hipMemcpyDeviceToHost), "D2H");
+    Check(hipMemcpy(op, dp, n * 4, hipMemcpyDeviceToHost), "D2H");
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    if (dd) hipFree(dd);
+    if (dv) hipFree(dv);
+    if (dp) hipFree(dp);
+    return TTGPU_ERR_DEVICE;
+  }
+  (void)out_floats_per;
+  if (dd) hipFree(dd);
+  hipFree(dv);
+  hipFree(dp);
+  return TTGPU_OK;
+}
+
+int ttgpu_sample_shader(const ttgpu_shader *shader, const float wi[3], const float ng[3], const float ns[3], const int32_t bins[4],
+                        float roughness, uint64_t seed, int64_t count, float *out_dir, float *out_value, float *out_pdf) {
+  if (!shader || !wi || !ng || !ns || !bins || count < 0 || (count && (!out_dir || !out_value || !out_pdf)))
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 4; ++i)
+    if (bins[i] < 0 || bins[i] >= kNBins) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (count == 0) return TTGPU_OK;
+  DevShader d = ToDev(*shader);
+  uint32_t pb = (uint32_t)bins[0] | ((uint32_t)bins[1] << 8) | ((uint32_t)bins[2] << 16) | ((uint32_t)bins[3] << 24);
+  return SamplerCall(count, 8, [&](float *dd, float *dv, float *dp) {
+    LaunchSampleShader(d, make_float3(wi[0], wi[1], wi[2]), make_float3(ng[0], ng[1], ng[2]), make_float3(ns[0], ns[1], ns[2]), pb,
+                       roughness, seed, count, dd, dv, dp, nullptr);
+  }, out_dir, out_value, out_pdf, 3, 4);
+}
+
+int ttgpu_eval_shader(const ttgpu_shader *shader, const float wi[3], const float ng[3], const float ns[3], const int32_t bins[4],
+                      float roughness, int64_t count, const float *dirs, float *out_value, float *out_pdf) {
+  if (!shader || !wi || !ng || !ns || !bins || count < 0 || (count && (!dirs || !out_value || !out_pdf))) return TTGPU_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 4; ++i)
+    if (bins[i] < 0 || bins[i] >= kNBins) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (count == 0) return TTGPU_OK;
+  DevShader d = ToDev(*shader);
+  uint32_t pb = (uint32_t)bins[0] | ((uint32_t)bins[1] << 8) | ((uint32_t)bins[2] << 16) | ((uint32_t)bins[3] << 24);
+  float *ddirs = nullptr;
+  if (hipMalloc(&ddirs, 3 * (size_t)count * 4) != hipSuccess) return TTGPU_ERR_DEVICE;
+  if (hipMemcpy(ddirs, dirs, 3 * (size_t)count * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(ddirs);
+    return TTGPU_ERR_DEVICE;
+  }
+  int rc = SamplerCall(count, 5, [&](float *, float *dv, float *dp) {
+    LaunchEvalShader(d, make_float3(wi[0], wi[1], wi[2]), make_float3(ng[0], ng[1], ng[2]), make_float3(ns[0], ns[1], ns[2]), pb,
+                     roughness, count, ddirs, dv, dp, nullptr);
+  }, nullptr, out_value, out_pdf, 0, 4);
+  hipFree(ddirs);
+  return rc;
+}
+
+int ttgpu_sample_phase(int32_t phase_kind, float g, const float wi[3], uint64_t seed, int64_t count, float *out_dir, float *out_value,
+                       float *out_pdf) {
+  if (!wi || count < 0 || (count && (!out_dir || !out_value || !out_pdf))) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (phase_kind < 0 || phase_kind > TTGPU_PHASE_RAYLEIGH) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (count == 0) return TTGPU_OK;
+  return SamplerCall(count, 5, [&](float *dd, float *dv, float *dp) {
+    LaunchSamplePhase(phase_kind, g, make_float3(wi[0], wi[1], wi[2]), seed, count, dd, dv, dp, nullptr);
+  }, out_dir, out_value, out_pdf, 3, 1);
+}
+
+}  // extern "C"

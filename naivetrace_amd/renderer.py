@@ -1,0 +1,226 @@
+"""Python face of the C ABI: a renderer object per ttgpu handle.
+
+`RenderingAlgo` mirrors ToyTrace's algorithm interface
+(src/renderingalgorithms_interface.hxx:13-30): InitializeScene, Run,
+RequestInterrupt, RequestFullStop, GenerateImage, SetInterruptCallback; and
+`RenderAlgorithmFactory` mirrors src/renderingalgorithms.cxx:27-42 for the
+two integrators this library implements, "pt2" and "ptg".
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class TTGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ttgpu error {code}: {msg}")
+        self.code = code
+
+
+def _f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class GpuRenderer:
+    """Owns one ttgpu handle.  Raises if the HIP library is not built."""
+
+    def __init__(self, scene, params):
+        self._lib = abi.load_library()
+        self.scene = scene
+        self.params = params
+        self._desc = scene.to_abi()
+        h = C.c_void_p()
+        rc = self._lib.ttgpu_create(C.byref(self._desc), C.byref(params), C.byref(h))
+        if rc != 0:
+            raise TTGpuError(rc, self._lib.ttgpu_create_error().decode())
+        self._h = h
+        self._cb = None
+
+    def _check(self, rc):
+        if rc != 0 and rc != 4:
+            raise TTGpuError(rc, self._lib.ttgpu_last_error(self._h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ttgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def npix(self):
+        return self.params.width * self.params.height
+
+    def run(self, max_passes=0):
+        return self._check(self._lib.ttgpu_run(self._h, max_passes))
+
+    def render_spp(self, spp):
+        return self._check(self._lib.ttgpu_render_spp(self._h, spp))
+
+    def request_interrupt(self):
+        self._check(self._lib.ttgpu_request_interrupt(self._h))
+
+    def request_stop(self):
+        self._check(self._lib.ttgpu_request_stop(self._h))
+
+    def set_pass_callback(self, fn):
+        if fn is None:
+            self._cb = abi.PASS_CALLBACK()
+        else:
+            self._cb = abi.PASS_CALLBACK(lambda user, complete: fn(bool(complete)))
+        self._check(self._lib.ttgpu_set_pass_callback(self._h, self._cb, None))
+
+    def reset(self):
+        self._check(self._lib.ttgpu_reset_framebuffer(self._h))
+
+    def framebuffer(self):
+        """(sum_rgb, sum_sq_rgb, spp) with shapes (h, w, 3), (h, w, 3), (h, w)."""
+        h, w = self.params.height, self.params.width
+        rgb = np.zeros((h, w, 3))
+        sq = np.zeros((h, w, 3))
+        spp = np.zeros((h, w), dtype=np.uint64)
+        self._check(self._lib.ttgpu_read_framebuffer(self._h, rgb.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     sq.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     spp.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return rgb, sq, spp
+
+    def mean_image(self):
+        rgb, _, spp = self.framebuffer()
+        return rgb / np.maximum(spp, 1)[..., None]
+
+    def generate_image(self):
+        img = np.zeros((self.params.height, self.params.width, 3), dtype=np.uint8)
+        self._check(self._lib.ttgpu_generate_image(self._h, img.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return img
+
+    def stats(self):
+        s = abi.Stats()
+        self._check(self._lib.ttgpu_stats_get(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in abi.Stats._fields_}
+
+    def enable_kernel_timing(self, on=True):
+        self._check(self._lib.ttgpu_enable_kernel_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self._lib.ttgpu_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def trace_closest(self, org, direction, tmax):
+        org = np.ascontiguousarray(org, dtype=np.float32)
+        direction = np.ascontiguousarray(direction, dtype=np.float32)
+        tmax = np.ascontiguousarray(tmax, dtype=np.float32)
+        n = len(org)
+        t = np.zeros(n, np.float32); u = np.zeros(n, np.float32); v = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        self._check(self._lib.ttgpu_trace_closest(self._h, n, _f32p(org), _f32p(direction), _f32p(tmax), _f32p(t),
+                                                  prim.ctypes.data_as(C.POINTER(C.c_int32)), _f32p(u), _f32p(v)))
+        return t, prim, u, v
+
+    def trace_occluded(self, org, direction, tmax):
+        org = np.ascontiguousarray(org, dtype=np.float32)
+        direction = np.ascontiguousarray(direction, dtype=np.float32)
+        tmax = np.ascontiguousarray(tmax, dtype=np.float32)
+        occ = np.zeros(len(org), np.int32)
+        self._check(self._lib.ttgpu_trace_occluded(self._h, len(org), _f32p(org), _f32p(direction), _f32p(tmax),
+                                                   occ.ctypes.data_as(C.POINTER(C.c_int32))))
+        return occ
+
+
+def sample_shader(shader, wi, ng, ns, bins, roughness, seed, count):
+    lib = abi.load_library()
+    d = np.zeros((count, 3), np.float32); v = np.zeros((count, 4), np.float32); p = np.zeros(count, np.float32)
+    a = [np.asarray(x, dtype=np.float32) for x in (wi, ng, ns)]
+    b = np.asarray(bins, dtype=np.int32)
+    rc = lib.ttgpu_sample_shader(C.byref(shader), _f32p(a[0]), _f32p(a[1]), _f32p(a[2]), b.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 roughness, seed, count, _f32p(d), _f32p(v), _f32p(p))
+    if rc:
+        raise TTGpuError(rc, lib.ttgpu_create_error().decode())
+    return d, v, p
+
+
+def eval_shader(shader, wi, ng, ns, bins, roughness, dirs):
+    lib = abi.load_library()
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32)
+    n = len(dirs)
+    v = np.zeros((n, 4), np.float32); p = np.zeros(n, np.float32)
+    a = [np.asarray(x, dtype=np.float32) for x in (wi, ng, ns)]
+    b = np.asarray(bins, dtype=np.int32)
+    rc = lib.ttgpu_eval_shader(C.byref(shader), _f32p(a[0]), _f32p(a[1]), _f32p(a[2]), b.ctypes.data_as(C.POINTER(C.c_int32)),
+                               roughness, n, _f32p(dirs), _f32p(v), _f32p(p))
+    if rc:
+        raise TTGpuError(rc, lib.ttgpu_create_error().decode())
+    return v, p
+
+
+def sample_phase(kind, g, wi, seed, count):
+    lib = abi.load_library()
+    d = np.zeros((count, 3), np.float32); v = np.zeros(count, np.float32); p = np.zeros(count, np.float32)
+    w = np.asarray(wi, dtype=np.float32)
+    rc = lib.ttgpu_sample_phase(kind, g, _f32p(w), seed, count, _f32p(d), _f32p(v), _f32p(p))
+    if rc:
+        raise TTGpuError(rc, lib.ttgpu_create_error().decode())
+    return d, v, p
+
+
+class RenderingAlgo:
+    """Mirror of ToyTrace's RenderingAlgo (src/renderingalgorithms_interface.hxx:13-30) over the GPU library."""
+
+    def __init__(self, scene, params):
+        self._scene = scene
+        self._params = params
+        self._r = None
+        self._cb = None
+
+    def InitializeScene(self, scene=None):
+        if scene is not None:
+            self._scene = scene
+        self._r = GpuRenderer(self._scene, self._params)
+        if self._cb:
+            self._r.set_pass_callback(self._cb)
+
+    def SetInterruptCallback(self, fn):
+        self._cb = fn
+        if self._r:
+            self._r.set_pass_callback(fn)
+
+    def Run(self):
+        if self._r is None:
+            self.InitializeScene()
+        return self._r.run(0)
+
+    def RequestInterrupt(self):
+        self._r.request_interrupt()
+
+    def RequestFullStop(self):
+        self._r.request_stop()
+
+    def GenerateImage(self):
+        return self._r.generate_image()
+
+    @property
+    def renderer(self):
+        return self._r
+
+
+def RenderAlgorithmFactory(scene, params, name=None):
+    """`name` in {"pt2", "ptg"} (src/renderingalgorithms.cxx:36,38); other integrators are not provided."""
+    name = name or ("ptg" if params.algo == abi.ALGO_PTG else "pt2")
+    if name == "pt2":
+        params.algo = abi.ALGO_PT2
+    elif name == "ptg":
+        params.algo = abi.ALGO_PTG
+    else:
+        raise ValueError(f"unknown or unsupported algorithm {name!r}; this library provides 'pt2' and 'ptg'")
+    return RenderingAlgo(scene, params)

@@ -1,0 +1,1111 @@
+// ORACLE — test infrastructure only (see ref_math.hpp).
+//
+// FP64 restatement of ToyTrace's pt2 integrator
+// (src/renderingalgorithms_pathtracing2.cxx) with its null-path spectral
+// tracking (src/media_integrator.cxx:317-508), medium tracker
+// (src/rendering_util.hxx:18-207, rendering_util.cxx:13-37), lights
+// (src/light.hxx, light.cxx), light picker (src/lightpicker_trivial.*) and
+// camera (src/camera.hxx).  Differences that do not change the estimator are
+// listed where they occur:
+//  * one PCG32 stream per path, seeded (seed, global sample index), instead of
+//    the reference's two default-seeded per-thread generators;
+//  * paths whose throughput is exactly zero stop early (every later
+//    contribution would be multiplied by zero).
+#include "ref_shading.hpp"
+#include "ttref.h"
+#include <thread>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <stdexcept>
+
+namespace ttref {
+
+void RGBToSpectrum(const double rgb[3], double out[NBINS]) {
+  const double red = rgb[0], green = rgb[1], blue = rgb[2];
+  enum { WHITE = 0, RED, GREEN, BLUE, YELLOW, CYAN, MAGENTA };
+  auto lin = [&](double a, int ia, double b, int ib, double c, int ic) {
+    for (int i = 0; i < NBINS; ++i) out[i] = a * kSmits[ia][i] + b * kSmits[ib][i] + c * kSmits[ic][i];
+  };
+  if (red <= green && red <= blue) {
+    if (green <= blue) lin(red, WHITE, green - red, CYAN, blue - green, BLUE);
+    else lin(red, WHITE, blue - red, CYAN, green - blue, GREEN);
+  } else if (green <= red && green <= blue) {
+    if (red <= blue) lin(green, WHITE, red - green, MAGENTA, blue - red, BLUE);
+    else lin(green, WHITE, blue - green, MAGENTA, red - blue, RED);
+  } else {
+    if (red <= green) lin(blue, WHITE, red - blue, YELLOW, green - red, GREEN);
+    else lin(blue, WHITE, green - blue, YELLOW, red - green, RED);
+  }
+}
+
+void SpectrumToRGB(const double spec[NBINS], double out[3]) {
+  for (int i = 0; i < 3; ++i) {
+    out[i] = 0.;
+    for (int b = 0; b < NBINS; ++b) out[i] += spec[b] * kSpecToRGB[i][b];
+  }
+}
+
+constexpr int VACUUM = -1, EMPTY = -2;
+constexpr int IDX_ENV = 0, IDX_AREA = 1, IDX_POINT = 2, IDX_VOLUME = 3;
+
+template <int NW> using Mat = std::array<std::array<double, NW>, NW>;  // rows: contribution, cols: pdf
+
+template <int NW> inline Mat<NW> MatOnes() { Mat<NW> m; for (auto &r : m) r.fill(1.); return m; }
+template <int NW> inline void FixNan(Mat<NW> &w) {
+  for (auto &r : w) for (auto &x : r) if (std::isnan(x)) x = Infinity;
+}
+// nullpath::AccumulateContributions (media_integrator.cxx:322-337)
+template <int NW> inline void Accumulate(Mat<NW> &w, const Mat<NW> &x) {
+  for (int i = 0; i < NW; ++i) for (int j = 0; j < NW; ++j) w[i][j] *= x[i][j];
+  FixNan<NW>(w);
+}
+template <int NW> inline void Accumulate(Mat<NW> &w, const Spec<NW> &f, const Spec<NW> &p) {
+  Mat<NW> t;
+  for (int i = 0; i < NW; ++i) for (int j = 0; j < NW; ++j) t[i][j] = (1. / f[i]) * p[j];
+  FixNan<NW>(t);
+  Accumulate<NW>(w, t);
+}
+template <int NW> inline Spec<NW> RowMean(const Mat<NW> &m, double scale) {
+  Spec<NW> r;
+  for (int i = 0; i < NW; ++i) {
+    double s = 0.;
+    for (int j = 0; j < NW; ++j) {
+      double v = m[i][j] * scale;
+      s += std::isnan(v) ? Infinity : v;
+    }
+    r[i] = s / NW;
+  }
+  return r;
+}
+
+struct ThroughputAndPdfsTag {};
+
+template <int NW> struct Engine {
+  const Scene &sc;
+  const ttgpu_params &prm;
+  Shading<NW> sh;
+  double type_probs[4] = {0, 0, 0, 0};
+  int counts[4] = {0, 0, 0, 0};
+  bool use_nee = true, use_emission = true;
+  // camera
+  M3 cam_frame;
+  V3 cam_pos;
+  double xmin, ymin, xper, yper, fish_delta;
+  int W, H;
+  // env maps
+  struct EnvMap {
+    M3 frame;
+    int w, h;
+    const float *tex;
+    std::vector<double> rows, cols;
+  };
+  std::vector<EnvMap> envmaps;  // parallel to sc.envs (only MAP entries filled)
+  struct SunData { M3 frame; double cos_open, pdf; double spectrum[NBINS]; };
+  std::vector<SunData> suns;
+  double bb_diag = 0;
+  std::vector<double> area_cmf, point_cmf;
+
+  Engine(const Scene &s, const ttgpu_params &p) : sc(s), prm(p), sh(s) {
+    if (p.sample_mode == TTGPU_SAMPLE_BSDF) use_nee = false;
+    if (p.sample_mode == TTGPU_SAMPLE_LIGHTS) use_emission = false;
+    counts[IDX_POINT] = (int)sc.points.size();
+    counts[IDX_AREA] = (int)sc.area_lights.size();
+    counts[IDX_ENV] = sc.envs.empty() ? 0 : 1;
+    double tot = counts[0] + counts[1] + counts[2] + counts[3];
+    for (int i = 0; i < 4; ++i) type_probs[i] = tot > 0 ? counts[i] / tot : 0.;
+    W = p.width; H = p.height;
+    const ttgpu_camera &c = sc.camera;
+    cam_pos = {c.pos[0], c.pos[1], c.pos[2]};
+    // Camera::MakeFrame (camera.hxx:43-58)
+    V3 up = Normalized(V3{c.up[0], c.up[1], c.up[2]}), dir = Normalized(V3{c.dir[0], c.dir[1], c.dir[2]});
+    double r = Dot(dir, up);
+    if (std::abs(r) <= 1.0 - Epsilon) up = Normalized(up - dir * r);
+    V3 right = Cross(up, dir);
+    cam_frame = {right, up, dir};
+    double fov = c.fov_deg * Pi / 180.;
+    double aspect = (double)H / W;
+    double a = std::tan(fov * 0.5), b = a * aspect;
+    xmin = -a; ymin = -b; xper = 2. * a / W; yper = 2. * b / H;
+    fish_delta = 2. / std::min(W, H);
+    bb_diag = sc.BoundingBoxDiagonal();
+    area_cmf.assign(counts[IDX_AREA], counts[IDX_AREA] ? 1. / counts[IDX_AREA] : 0.);
+    if (!area_cmf.empty()) ComputeNormalizedCumSum(area_cmf.data(), (long)area_cmf.size());
+    point_cmf.assign(counts[IDX_POINT], counts[IDX_POINT] ? 1. / counts[IDX_POINT] : 0.);
+    if (!point_cmf.empty()) ComputeNormalizedCumSum(point_cmf.data(), (long)point_cmf.size());
+    envmaps.resize(sc.envs.size());
+    suns.resize(sc.envs.size());
+    for (size_t i = 0; i < sc.envs.size(); ++i) {
+      const auto &e = sc.envs[i];
+      V3 d{e.dir[0], e.dir[1], e.dir[2]};
+      d = Normalized(d);
+      if (e.kind == TTGPU_ENV_MAP) BuildEnvMap(envmaps[i], e, d);
+      if (e.kind == TTGPU_ENV_SUN) {
+        SunData &s = suns[i];
+        s.cos_open = std::cos(Pi / 180. * e.opening_angle_deg);
+        s.pdf = 1. / (2. * Pi * (1. - s.cos_open));
+        s.frame = OrthogonalSystemZAligned(-d);
+        for (int k = 0; k < NBINS; ++k) s.spectrum[k] = kSunSpectrum[k] * e.total_power * s.pdf;
+      }
+    }
+    TrackerInitializePosition(cam_tracker, cam_pos);
+  }
+
+  // ---- env map (light.cxx:88-189) -------------------------------------------
+  void BuildEnvMap(EnvMap &m, const ttgpu_env_light &e, V3 up) {
+    m.frame = OrthogonalSystemZAligned(up);
+    m.w = e.tex_width; m.h = e.tex_height; m.tex = e.texture_rgb;
+    m.rows.assign(m.h, 0.);
+    m.cols.assign((size_t)m.w * m.h, 0.);
+    for (int y = 0; y < m.h; ++y) {
+      double row = 0.;
+      for (int x = 0; x < m.w; ++x) {
+        const float *px = m.tex + 3 * ((size_t)y * m.w + x);
+        // PixelCenterToUv (texture.hxx) -> theta = pi * v
+        double v = ((double)(m.h - 1 - y) + 0.5) / m.h;
+        double theta = v * Pi;
+        double wgt = std::sin(theta) * ((double)px[0] + px[1] + px[2]) / 3.;
+        row += wgt;
+        m.cols[(size_t)y * m.w + x] = row;
+      }
+      for (int x = 0; x < m.w; ++x) m.cols[(size_t)y * m.w + x] /= row;
+      m.rows[y] = row;
+    }
+    ComputeNormalizedCumSum(m.rows.data(), m.h);
+  }
+  void EnvMapToPixel(const EnvMap &m, V3 dir_out, int &x, int &y) const {
+    double phi, theta;
+    KartesianToSpherical(m.frame.TMul(dir_out), phi, theta);
+    float u = (float)(phi / (2. * Pi)), v = (float)(theta / Pi);
+    float dummy;
+    u = std::modf(u, &dummy);
+    v = std::modf(v, &dummy);
+    if (u < 0.f) u = 1.f + u;
+    if (v < 0.f) v = 1.f + v;
+    const float ef = std::numeric_limits<float>::epsilon();
+    if (u > 1.f - ef) u -= ef;
+    if (v > 1.f - ef) v -= ef;
+    x = (int)(u * m.w);
+    y = (int)(v * m.h);
+    y = m.h - y - 1;
+  }
+  void EnvPixelBounds(const EnvMap &m, int x, int y, double &phi0, double &z0, double &phi1, double &z1) const {
+    double ul = (double)x / m.w, vl = 1. - (double)y / m.h;
+    double uu = ul + 1. / m.w, vu = 1. - ((double)y + 1.) / m.h;
+    phi0 = ul * 2. * Pi; phi1 = uu * 2. * Pi;
+    z0 = std::cos(vl * Pi); z1 = std::cos(vu * Pi);
+  }
+
+  // ---- environment radiance field ------------------------------------------
+  Spec<NW> EnvEvalOne(int i, V3 dir_out, const PathContext<NW> &ctx) const {
+    const auto &e = sc.envs[i];
+    switch (e.kind) {
+      case TTGPU_ENV_SUN: {
+        const SunData &s = suns[i];
+        if (Dot(dir_out, s.frame.c2) > s.cos_open) {
+          Spec<NW> v; for (int k = 0; k < NW; ++k) v[k] = s.spectrum[ctx.bins[k]];
+          return v;
+        }
+        return SpecConst<NW>(0.);
+      }
+      case TTGPU_ENV_DOME: {
+        V3 up = Normalized(V3{e.dir[0], e.dir[1], e.dir[2]});
+        return Dot(dir_out, -up) > 0. ? Take<NW>(e.spectrum, ctx) : SpecConst<NW>(0.);
+      }
+      case TTGPU_ENV_DIRECTIONAL: return SpecConst<NW>(0.);
+      case TTGPU_ENV_MAP: {
+        int x, y;
+        EnvMapToPixel(envmaps[i], dir_out, x, y);
+        const float *px = envmaps[i].tex + 3 * ((size_t)y * envmaps[i].w + x);
+        double rgb[3] = {px[0], px[1], px[2]};
+        return RGBToSpectralSelection<NW>(rgb, ctx);
+      }
+    }
+    return SpecConst<NW>(0.);
+  }
+  double EnvPdfOne(int i, V3 dir_out) const {
+    const auto &e = sc.envs[i];
+    switch (e.kind) {
+      case TTGPU_ENV_SUN: return Dot(dir_out, suns[i].frame.c2) > suns[i].cos_open ? suns[i].pdf : 0.;
+      case TTGPU_ENV_DOME: {
+        V3 up = Normalized(V3{e.dir[0], e.dir[1], e.dir[2]});
+        return Dot(dir_out, -up) > 0. ? 1. / UnitHalfSphereSurfaceArea : 0.;
+      }
+      case TTGPU_ENV_DIRECTIONAL: return 0.;
+      case TTGPU_ENV_MAP: {
+        const EnvMap &m = envmaps[i];
+        int x, y;
+        EnvMapToPixel(m, dir_out, x, y);
+        double py = ProbabilityFromCmf(m.rows.data(), y), px = ProbabilityFromCmf(&m.cols[(size_t)y * m.w], x);
+        double p0, z0, p1, z1;
+        EnvPixelBounds(m, x, y, p0, z0, p1, z1);
+        return py * px / ((z1 - z0) * (p1 - p0));
+      }
+    }
+    return 0.;
+  }
+  // TakeDirectionSample of one env light; dir_out = direction the light travels.
+  void EnvSampleOne(int i, Sampler &smp, const PathContext<NW> &ctx, V3 &dir_out, Spec<NW> &val, Pdf &pdf) const {
+    const auto &e = sc.envs[i];
+    double r[2];
+    switch (e.kind) {
+      case TTGPU_ENV_SUN: {
+        const SunData &s = suns[i];
+        smp.UniformUnitSquare(r);
+        dir_out = s.frame * ToUniformSphereSection(s.cos_open, r);
+        for (int k = 0; k < NW; ++k) val[k] = s.spectrum[ctx.bins[k]];
+        pdf = Pdf(s.pdf);
+        return;
+      }
+      case TTGPU_ENV_DOME: {
+        V3 up = Normalized(V3{e.dir[0], e.dir[1], e.dir[2]});
+        smp.UniformUnitSquare(r);
+        dir_out = OrthogonalSystemZAligned(-up) * ToUniformHemisphere(r);
+        val = Take<NW>(e.spectrum, ctx);
+        pdf = Pdf(1. / UnitHalfSphereSurfaceArea);
+        return;
+      }
+      case TTGPU_ENV_DIRECTIONAL: {
+        V3 d = Normalized(V3{e.dir[0], e.dir[1], e.dir[2]});
+        dir_out = -d;
+        val = Take<NW>(e.spectrum, ctx);
+        pdf = Pdf::Delta(1.);
+        return;
+      }
+      case TTGPU_ENV_MAP: {
+        const EnvMap &m = envmaps[i];
+        smp.UniformUnitSquare(r);
+        long y = TowerSamplingBisection(m.rows.data(), m.h, r[1]);
+        double py = ProbabilityFromCmf(m.rows.data(), y);
+        const double *row = &m.cols[(size_t)y * m.w];
+        long x = TowerSamplingBisection(row, m.w, r[0]);
+        double px = ProbabilityFromCmf(row, x);
+        double p0, z0, p1, z1;
+        EnvPixelBounds(m, (int)x, (int)y, p0, z0, p1, z1);
+        double r2[2];
+        smp.UniformUnitSquare(r2);
+        dir_out = m.frame * ToUniformSphereSection(r2, p0, z0, p1, z1);
+        const float *pxv = m.tex + 3 * ((size_t)y * m.w + x);
+        double rgb[3] = {pxv[0], pxv[1], pxv[2]};
+        val = RGBToSpectralSelection<NW>(rgb, ctx);
+        pdf = Pdf(py * px / ((z1 - z0) * (p1 - p0)));
+        return;
+      }
+    }
+  }
+  // TotalEnvironmentalRadianceField (light.cxx:29-84)
+  Spec<NW> EnvEval(V3 dir_out, const PathContext<NW> &ctx) const {
+    Spec<NW> s = SpecConst<NW>(0.);
+    for (size_t i = 0; i < sc.envs.size(); ++i) s = s + EnvEvalOne((int)i, dir_out, ctx);
+    return s;
+  }
+  double EnvPdf(V3 dir_out) const {
+    double sel = sc.envs.empty() ? 1. : 1. / sc.envs.size();
+    double sum = 0.;
+    for (size_t i = 0; i < sc.envs.size(); ++i) sum += EnvPdfOne((int)i, dir_out);
+    return sum * sel;
+  }
+  void EnvSample(Sampler &smp, const PathContext<NW> &ctx, V3 &dir_out, Spec<NW> &val, Pdf &pdf) const {
+    int n = (int)sc.envs.size();
+    int i = smp.UniformInt(0, n - 1);
+    EnvSampleOne(i, smp, ctx, dir_out, val, pdf);
+    pdf = Pdf(std::copysign(pdf.val() / n, pdf.v));
+  }
+
+  // ---- light picker (lightpicker_trivial.cxx:35-51, .hxx:19-38,138-176) ----
+  double PmfArea() const { return type_probs[IDX_AREA] * (counts[IDX_AREA] ? ProbabilityFromCmf(area_cmf.data(), 0) : 0.); }
+  double PmfEnv() const { return type_probs[IDX_ENV]; }
+
+  // ---- media ------------------------------------------------------------------
+  int Priority(int m) const { return m == VACUUM ? -1 : sc.media[m].priority; }
+  int EmptySpace() const { return sc.empty_space_medium >= 0 ? sc.empty_space_medium : VACUUM; }
+
+  struct Tracker {
+    int current;
+    int slots[4];
+  };
+  Tracker cam_tracker;
+  void TrackerInit(Tracker &t) const { t.current = EmptySpace(); for (int &s : t.slots) s = EMPTY; }
+  void GoingThroughSurface(Tracker &t, V3 dir, V3 n, int material) const {
+    const auto &mat = sc.materials[material];
+    if (mat.medium < 0 && mat.outer_medium < 0) return;
+    int to_enter = mat.medium, to_exit = mat.outer_medium;
+    // Dot in float as the reference does (rendering_util.hxx:70).
+    float d = (float)dir.x * (float)n.x + (float)dir.y * (float)n.y + (float)dir.z * (float)n.z;
+    if (d > 0.f) std::swap(to_enter, to_exit);
+    if (to_exit >= 0) {
+      for (int &s : t.slots) if (s == to_exit) { s = EMPTY; break; }
+      if (to_exit == t.current) {
+        int best = EmptySpace();
+        for (int s : t.slots) if (s != EMPTY && Priority(best) < Priority(s)) best = s;
+        t.current = best;
+      }
+    }
+    if (to_enter >= 0) {
+      bool inserted = false;
+      for (int &s : t.slots) if (s == EMPTY) { s = to_enter; inserted = true; break; }
+      t.current = (inserted && Priority(to_enter) > Priority(t.current)) ? to_enter : t.current;
+    }
+  }
+  // MediumTracker::initializePosition (rendering_util.cxx:13-37)
+  void TrackerInitializePosition(Tracker &t, V3 pos) const {
+    TrackerInit(t);
+    if (!sc.InBox(pos)) return;
+    double dist = 2. * sc.BoundingBoxMaxExtent();
+    V3 org = pos + V3{dist, 0., 0.}, dir{-1., 0., 0.};
+    std::vector<BoundaryHit> hs;
+    sc.IntersectionsInOrder(org, dir, 0., dist, false, hs);
+    for (auto &h : hs) GoingThroughSurface(t, dir, h.n, sc.prims[h.prim].material);
+    sc.IntersectionsInOrder(org, dir, 0., dist, true, hs);
+    for (auto &h : hs) GoingThroughSurface(t, dir, h.n, sc.prims[h.prim].material);
+  }
+  // MaybeGoingThroughSurface (rendering_util.hxx:130-137)
+  void MaybeGoingThroughSurface(Tracker &t, V3 dir, const SurfaceInteraction &si) const {
+    if (Dot(dir, si.normal) < 0.) GoingThroughSurface(t, dir, si.geometry_normal, si.material);
+  }
+
+  double Altitude(const ttgpu_medium &m, V3 p) const {
+    return Length(p - V3{m.planet_center[0], m.planet_center[1], m.planet_center[2]}) - m.planet_radius;
+  }
+  double AtmoLowerCutoff(const ttgpu_medium &m) const { return std::max(-m.rayleigh_scale_height, -m.mie_scale_height); }
+  void AtmoCoeffs(const ttgpu_medium &m, double alt, const PathContext<NW> &ctx, Spec<NW> &ss_ray, Spec<NW> &ss_mie, Spec<NW> &sa) const {
+    alt = alt > AtmoLowerCutoff(m) ? alt : AtmoLowerCutoff(m);
+    double rr = std::exp(-alt / m.rayleigh_scale_height), rm = std::exp(-alt / m.mie_scale_height);
+    for (int k = 0; k < NW; ++k) {
+      ss_ray[k] = m.rayleigh_sigma_s[ctx.bins[k]] * rr;
+      ss_mie[k] = m.mie_sigma_s[ctx.bins[k]] * rm;
+      sa[k] = m.mie_sigma_a[ctx.bins[k]] * rm;
+    }
+  }
+  void Coeffs(int mid, V3 pos, const PathContext<NW> &ctx, Spec<NW> &ss, Spec<NW> &st) const {
+    if (mid == VACUUM) { ss = st = SpecConst<NW>(0.); return; }
+    const auto &m = sc.media[mid];
+    if (m.kind == TTGPU_MEDIUM_HOMOGENEOUS) {
+      for (int k = 0; k < NW; ++k) {
+        ss[k] = m.sigma_s[ctx.bins[k]];
+        st[k] = (double)m.sigma_s[ctx.bins[k]] + m.sigma_a[ctx.bins[k]];
+      }
+      return;
+    }
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+      Spec<NW> a, b, c;
+      AtmoCoeffs(m, Altitude(m, pos), ctx, a, b, c);
+      ss = a + b;
+      st = ss + c;
+      return;
+    }
+    ss = st = SpecConst<NW>(0.);
+  }
+  // Majorant over a segment (ComputeMajorante; atmosphere: lowest point, atmosphere.cxx:84-91).
+  Spec<NW> Majorant(int mid, V3 org, V3 dir, double len, const PathContext<NW> &ctx) const {
+    if (mid == VACUUM) return SpecConst<NW>(0.);
+    const auto &m = sc.media[mid];
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+      V3 c{m.planet_center[0], m.planet_center[1], m.planet_center[2]};
+      double tl = -Dot(org - c, dir);
+      if (tl > len) tl = len; else if (tl < 0.) tl = 0.;
+      double alt = std::max(AtmoLowerCutoff(m), Altitude(m, org + dir * tl));
+      Spec<NW> a, b, cc;
+      AtmoCoeffs(m, alt, ctx, a, b, cc);
+      return a + b + cc;
+    }
+    Spec<NW> ss, st;
+    Coeffs(mid, org, ctx, ss, st);
+    return st;
+  }
+  void MediumPhaseSample(int mid, V3 wi, V3 pos, Sampler &smp, const PathContext<NW> &ctx, ScatterSample<NW> &s) const {
+    if (mid == VACUUM) { s = {-wi, SpecConst<NW>(1.), Pdf(1.)}; return; }
+    const auto &m = sc.media[mid];
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+      // SimpleCombined of Rayleigh and HG weighted by the constituents' sigma_s (phasefunctions.cxx:199-270)
+      Spec<NW> w[2], sa;
+      AtmoCoeffs(m, Altitude(m, pos), ctx, w[0], w[1], sa);
+      double sel[2];
+      Spec<NW> norm = w[0] + w[1];
+      double tot = 0.;
+      for (int c = 0; c < 2; ++c) { sel[c] = 0.; for (int k = 0; k < NW; ++k) sel[c] += w[c][k]; tot += sel[c]; }
+      if (tot > 0.) {
+        for (int c = 0; c < 2; ++c) { sel[c] /= tot; for (int k = 0; k < NW; ++k) w[c][k] /= norm[k]; }
+      } else {
+        sel[0] = sel[1] = 0.5;
+      }
+      int kinds[2] = {TTGPU_PHASE_RAYLEIGH, TTGPU_PHASE_HENYEY_GREENSTEIN};
+      double gs[2] = {0., (double)m.mie_g};
+      int c = TowerSampling<2>(sel, smp.Uniform01());
+      int o = 1 - c;
+      V3 dir;
+      double val;
+      PhaseSample(kinds[c], gs[c], wi, smp, dir, val);
+      double other = PhaseEval(kinds[o], gs[o], wi, dir);
+      Spec<NW> v;
+      for (int k = 0; k < NW; ++k) v[k] = w[c][k] * val + w[o][k] * other;
+      double pdfs[2];
+      pdfs[c] = val; pdfs[o] = other;
+      s = {dir, v, Pdf(sel[0] * pdfs[0] + sel[1] * pdfs[1])};
+      return;
+    }
+    V3 dir;
+    double val;
+    PhaseSample(m.phase_kind, m.phase_g, wi, smp, dir, val);
+    s = {dir, SpecConst<NW>(val), Pdf(val)};
+  }
+  Spec<NW> MediumPhaseEval(int mid, V3 wi, V3 pos, V3 wo, const PathContext<NW> &ctx, double *pdf) const {
+    if (mid == VACUUM) { *pdf = 1.; return SpecConst<NW>(0.); }
+    const auto &m = sc.media[mid];
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+      Spec<NW> w[2], sa;
+      AtmoCoeffs(m, Altitude(m, pos), ctx, w[0], w[1], sa);
+      double sel[2];
+      Spec<NW> norm = w[0] + w[1];
+      double tot = 0.;
+      for (int c = 0; c < 2; ++c) { sel[c] = 0.; for (int k = 0; k < NW; ++k) sel[c] += w[c][k]; tot += sel[c]; }
+      if (tot > 0.) {
+        for (int c = 0; c < 2; ++c) { sel[c] /= tot; for (int k = 0; k < NW; ++k) w[c][k] /= norm[k]; }
+      } else {
+        sel[0] = sel[1] = 0.5;
+      }
+      double p0 = PhaseEval(TTGPU_PHASE_RAYLEIGH, 0., wi, wo), p1 = PhaseEval(TTGPU_PHASE_HENYEY_GREENSTEIN, m.mie_g, wi, wo);
+      *pdf = sel[0] * p0 + sel[1] * p1;
+      Spec<NW> v;
+      for (int k = 0; k < NW; ++k) v[k] = w[0][k] * p0 + w[1][k] * p1;
+      return v;
+    }
+    double v = PhaseEval(m.phase_kind, m.phase_g, wi, wo);
+    *pdf = v;
+    return SpecConst<NW>(v);
+  }
+
+  struct TandP {
+    Mat<NW> nulls = MatOnes<NW>(), track = MatOnes<NW>();
+  };
+
+  // nullpath::Tracking(medium,...) (media_integrator.cxx:339-385)
+  bool TrackMedium(int mid, V3 org, V3 dir, double len, Sampler &smp, const PathContext<NW> &ctx, TandP &w, double &t_out,
+                   Spec<NW> &sigma_s_out) const {
+    const Spec<NW> maj = Majorant(mid, org, dir, len, ctx);
+    const double mfp = 1. / maj[0];
+    double t = 0.;
+    for (int it = 1; it < 10000; ++it) {
+      double v = -std::log(1.0 - smp.Uniform01()) * mfp;
+      double tlocal = (v < LargeNumber) ? v : LargeNumber;  // std::min(LargeNumber, v) with NaN -> LargeNumber
+      Spec<NW> tr;
+      double seg = std::min(tlocal, len - t);
+      for (int k = 0; k < NW; ++k) tr[k] = std::exp(-seg * maj[k]);
+      t += tlocal;
+      Spec<NW> ss, st;
+      V3 pos = org + dir * t;
+      Coeffs(mid, pos, ctx, ss, st);
+      if (t >= len) {
+        Accumulate<NW>(w.track, tr, tr);
+        Accumulate<NW>(w.nulls, tr, tr);
+        t_out = t;
+        return false;
+      }
+      Spec<NW> sn = maj - st;
+      const double prob_n = sn[0] * mfp;
+      if (smp.Uniform01() < prob_n) {
+        Accumulate<NW>(w.track, tr * sn, tr * sn);
+        Accumulate<NW>(w.nulls, tr * sn, tr * maj);
+      } else {
+        Accumulate<NW>(w.track, tr * st, tr * st);
+        Accumulate<NW>(w.nulls, tr * st, tr);
+        t_out = t;
+        sigma_s_out = ss;
+        return true;
+      }
+    }
+    t_out = len;
+    return false;
+  }
+  // nullpath::Transmission(medium,...) (media_integrator.cxx:387-426)
+  void TransmitMedium(int mid, V3 org, V3 dir, double len, Sampler &smp, const PathContext<NW> &ctx, TandP &w) const {
+    const Spec<NW> maj = Majorant(mid, org, dir, len, ctx);
+    const double mfp = 1. / maj[0];
+    double t = 0.;
+    for (int it = 1; it < 10000; ++it) {
+      double v = -std::log(1.0 - smp.Uniform01()) * mfp;
+      double tlocal = (v < LargeNumber) ? v : LargeNumber;
+      Spec<NW> tr;
+      double seg = std::min(tlocal, len - t);
+      for (int k = 0; k < NW; ++k) tr[k] = std::exp(-seg * maj[k]);
+      t += tlocal;
+      if (t >= len) {
+        Accumulate<NW>(w.track, tr, tr);
+        Accumulate<NW>(w.nulls, tr, tr);
+        return;
+      }
+      Spec<NW> ss, st;
+      Coeffs(mid, org + dir * t, ctx, ss, st);
+      Spec<NW> sn = maj - st;
+      Accumulate<NW>(w.track, tr * sn, tr * sn);
+      Accumulate<NW>(w.nulls, tr * sn, tr * maj);
+      bool all_large = true;
+      for (int k = 0; k < NW; ++k) all_large &= w.nulls[k][k] > LargeNumber;
+      if (all_large) {
+        for (auto &r : w.nulls) r.fill(Infinity);
+        return;
+      }
+    }
+  }
+
+  struct Interaction {
+    int kind = 0;  // 0 none (escaped), 1 surface, 2 volume
+    SurfaceInteraction si;
+    V3 vpos;
+    int vmedium = VACUUM;
+    Spec<NW> vsigma_s;
+  };
+
+  SurfaceInteraction MakeSurfaceInteraction(const Hit &h, V3 ray_dir) const {
+    SurfaceInteraction si;
+    si.hit = h;
+    sc.LocalGeometry(h, si.pos, si.geometry_normal, si.smooth_normal, si.pos_bounds);
+    si.material = sc.prims[h.prim].material;
+    si.SetOrientedNormals(ray_dir);
+    return si;
+  }
+
+  // nullpath::Tracking(scene,...) (media_integrator.cxx:429-475)
+  TandP TrackScene(V3 org, V3 dir, Sampler &smp, Tracker &tracker, const PathContext<NW> &ctx, Interaction &ia) const {
+    TandP w;
+    Hit hit;
+    bool has_hit = sc.FirstIntersection(org, dir, 0., LargeNumber, hit);
+    double tfar = has_hit ? hit.t : LargeNumber;
+    double seg_end = tfar * 0.9999;
+    thread_local std::vector<BoundaryHit> bh;
+    sc.IntersectionsInOrder(org, dir, 0., seg_end, false, bh);
+    double tnear = 0.;
+    size_t k = 0;
+    while (tnear != seg_end) {
+      double sfar = k < bh.size() ? bh[k].t : seg_end;
+      TandP s;
+      double t;
+      Spec<NW> ss;
+      V3 sorg = org + dir * tnear;
+      bool inter = TrackMedium(tracker.current, sorg, dir, sfar - tnear, smp, ctx, s, t, ss);
+      Accumulate<NW>(w.nulls, s.nulls);
+      Accumulate<NW>(w.track, s.track);
+      if (inter) {
+        ia.kind = 2;
+        ia.vpos = org + dir * (tnear + t);
+        ia.vmedium = tracker.current;
+        ia.vsigma_s = ss;
+        return w;
+      }
+      if (k >= bh.size()) { tnear = seg_end; }
+      else {
+        GoingThroughSurface(tracker, dir, bh[k].n, sc.prims[bh[k].prim].material);
+        tnear = bh[k].t;
+        ++k;
+      }
+    }
+    if (has_hit) {
+      ia.kind = 1;
+      ia.si = MakeSurfaceInteraction(hit, dir);
+    } else {
+      ia.kind = 0;
+    }
+    return w;
+  }
+
+  // nullpath::Transmission(scene,...) (media_integrator.cxx:478-508)
+  TandP TransmitScene(V3 org, V3 dir, double length, Sampler &smp, Tracker &tracker, const PathContext<NW> &ctx) const {
+    TandP w;
+    length *= 0.9999;
+    if (sc.IsOccluded(org, dir, 0., length)) {
+      for (auto &r : w.nulls) r.fill(Infinity);
+      for (auto &r : w.track) r.fill(Infinity);
+      return w;
+    }
+    thread_local std::vector<BoundaryHit> bh;
+    sc.IntersectionsInOrder(org, dir, 0., length, false, bh);
+    double tnear = 0.;
+    size_t k = 0;
+    while (tnear != length) {
+      double sfar = k < bh.size() ? bh[k].t : length;
+      TandP s;
+      TransmitMedium(tracker.current, org + dir * tnear, dir, sfar - tnear, smp, ctx, s);
+      Accumulate<NW>(w.nulls, s.nulls);
+      Accumulate<NW>(w.track, s.track);
+      if (k >= bh.size()) tnear = length;
+      else {
+        GoingThroughSurface(tracker, dir, bh[k].n, sc.prims[bh[k].prim].material);
+        tnear = bh[k].t;
+        ++k;
+      }
+    }
+    return w;
+  }
+
+  // ---- path state -----------------------------------------------------------
+  struct PathState {
+    PathContext<NW> ctx;
+    int pixel;
+    Tracker tracker;
+    V3 org, dir;
+    Mat<NW> weights_track, weights_track_then_null;
+    Spec<NW> path_weights;
+    bool has_last_pdf = false;
+    Pdf last_pdf;
+    double shader_roughness = 0.;
+    int node_count = 0;
+    double rgb[3] = {0, 0, 0};
+  };
+
+  // MisWeight (pt2.cxx:452-476); returns (weight, per-wavelength denominator).
+  void MisWeight(bool has_pdf, Pdf scatter_pdf, double other, const Mat<NW> &dp, const Mat<NW> &dpo, double &w, Spec<NW> &den) const {
+    if (!has_pdf || scatter_pdf.IsDelta() || !(use_nee && use_emission)) {
+      w = 1.;
+      den = RowMean<NW>(dp, 1.);
+      return;
+    }
+    Spec<NW> a = RowMean<NW>(dp, scatter_pdf.val()), b = RowMean<NW>(dpo, other);
+    w = scatter_pdf.val();
+    den = a + b;
+  }
+
+  void Record(const Spec<NW> &m, PathState &ps) const {  // RecordMeasurementToCurrentPixel (pt2.cxx:695-700)
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < NW; ++j) ps.rgb[c] += m[j] * kSpecToRGB[c][ps.ctx.bins[j]];
+  }
+
+  // LambdaSelectionStrategy::WithWeights (shader_util.hxx:83-91) + camera (pt2.cxx:381-407).
+  void InitializePath(PathState &p, int px, int py, Sampler &smp) const {
+    constexpr int strata = NBINS / NW;
+    int main_idx = smp.UniformInt(0, strata - 1);
+    int idx[NW];
+    for (int i = 0; i < NW; ++i) idx[i] = main_idx + i * strata;
+    int rot = smp.UniformInt(0, NW - 1);
+    std::swap(idx[0], idx[rot]);
+    for (int i = 0; i < NW; ++i) {
+      p.ctx.bins[i] = idx[i];
+      double c = kBinCenters[idx[i]], hw = 0.5 * (LambdaMax - LambdaMin) / NBINS;
+      p.ctx.wavelengths[i] = (c - hw) + smp.Uniform01() * (2. * hw);
+    }
+    p.pixel = py * W + px;
+    p.node_count = 0;
+    p.path_weights = SpecConst<NW>((double)strata);
+    p.weights_track = MatOnes<NW>();
+    p.weights_track_then_null = MatOnes<NW>();
+    p.has_last_pdf = false;
+    p.shader_roughness = 0.;
+    p.rgb[0] = p.rgb[1] = p.rgb[2] = 0.;
+    p.node_count = 1;  // position sample: delta, weight 1
+    double r[2];
+    smp.UniformUnitSquare(r);
+    V3 dir;
+    if (sc.camera.kind == TTGPU_CAMERA_PINHOLE) {
+      V3 v{xmin + (r[0] + px) * xper, ymin + (r[1] + py) * yper, 1.0};
+      v = Normalized(v);
+      dir = cam_frame * v;  // value == pdf: weight 1
+    } else {
+      double u = (r[0] + px) * fish_delta - 1.0, vv = (r[1] + py) * fish_delta - 1.0;
+      double rho = u * u + vv * vv;
+      V3 w{2. * u, 2. * vv, rho - 1.};
+      w = w * (1. / (1. + rho));
+      w.z = -w.z;
+      if (rho <= 1.) dir = cam_frame * w;
+      else { dir = cam_frame.c2; p.path_weights = SpecConst<NW>(0.); }
+    }
+    p.org = cam_pos;
+    p.dir = dir;
+    // MediumTracker::initializePosition depends only on the camera position, which is fixed:
+    // computed once per render instead of once per path (same result).
+    p.tracker = cam_tracker;
+    p.node_count = 2;
+  }
+
+  // AddDirectLighting (pt2.cxx:479-556)
+  void AddDirectLighting(const Interaction &ia, PathState &ps, Sampler &smp) const {
+    if (!use_nee) return;
+    if (counts[0] + counts[1] + counts[2] + counts[3] == 0) return;
+    int kind = TowerSampling<4>(type_probs, smp.Uniform01());
+    double u2 = smp.Uniform01();
+    V3 ray_org, ray_dir;
+    double length;
+    Pdf pdf;
+    Spec<NW> light_weight;
+    double prob;
+    const V3 here = ia.kind == 1 ? ia.si.pos : ia.vpos;
+    auto SegmentToPoint = [&](V3 target) {
+      V3 d = target - here;
+      length = Length(d);
+      if (length > 0.) d = d / length;
+      else { length = Epsilon; d = ia.kind == 1 ? ia.si.normal : V3{1., 0., 0.}; }
+      ray_dir = d;
+      ray_org = here;
+      if (ia.kind == 1) ray_org = ray_org + AntiSelfIntersectionOffset(ia.si, d);
+    };
+    if (kind == IDX_AREA) {
+      int n = counts[IDX_AREA];
+      const double *cmf = area_cmf.data();
+      long li = TowerSamplingBisection(cmf, n, u2);
+      prob = type_probs[IDX_AREA] * ProbabilityFromCmf(cmf, li);
+      int prim = sc.area_lights[li];
+      double r[2];
+      smp.UniformUnitSquare(r);
+      Hit lh = sc.SampleUniformPosition(prim, r);
+      V3 lpos, lgn, lsn, lpb;
+      sc.LocalGeometry(lh, lpos, lgn, lsn, lpb);
+      double area_pdf = 1. / sc.PrimArea(prim);
+      SegmentToPoint(lpos);
+      const auto &em = sc.emitters[sc.materials[sc.prims[prim].material].emitter];
+      Spec<NW> val;
+      if (em.kind == TTGPU_AREA_UNIFORM) {
+        double vis = Dot(lgn, -ray_dir) > 0. ? 1. : 0.;
+        val = (vis / UnitHalfSphereSurfaceArea) * Take<NW>(em.spectrum, ps.ctx);
+      } else {
+        val = SpecConst<NW>(0.);
+      }
+      double dfactor = std::abs(Dot(lsn, ray_dir));
+      light_weight = (dfactor / (area_pdf * prob)) * val;
+      light_weight = (1. / Sqr(length)) * light_weight;
+      double p_sa = AreaToSolidAngle(length, ray_dir, lgn) * area_pdf;
+      pdf = Pdf(p_sa * (prm.mis_mode == TTGPU_MIS_CONSISTENT ? prob : 1.));
+    } else if (kind == IDX_ENV) {
+      prob = type_probs[IDX_ENV];
+      V3 dir_out;
+      Spec<NW> val;
+      EnvSample(smp, ps.ctx, dir_out, val, pdf);
+      ray_dir = -dir_out;
+      ray_org = here;
+      if (ia.kind == 1) ray_org = ray_org + AntiSelfIntersectionOffset(ia.si, ray_dir);
+      length = 10. * bb_diag;
+      light_weight = (1. / (pdf.val() * prob)) * val;
+      if (prm.mis_mode == TTGPU_MIS_CONSISTENT) pdf = Pdf(std::copysign(pdf.val() * prob, pdf.v));
+    } else if (kind == IDX_POINT) {
+      int n = counts[IDX_POINT];
+      const double *cmf = point_cmf.data();
+      long li = TowerSamplingBisection(cmf, n, u2);
+      prob = type_probs[IDX_POINT] * ProbabilityFromCmf(cmf, li);
+      const auto &pl = sc.points[li];
+      SegmentToPoint(V3{pl.pos[0], pl.pos[1], pl.pos[2]});
+      Spec<NW> val = (1. / UnitSphereSurfaceArea) * Take<NW>(pl.spectrum, ps.ctx);
+      light_weight = (1. / prob) * val;
+      light_weight = (1. / Sqr(length)) * light_weight;
+      pdf = Pdf::Delta(1.);
+    } else {
+      return;
+    }
+    Spec<NW> pw = ps.path_weights;
+    Tracker tr = ps.tracker;
+    double bsdf_pdf = 0.;
+    if (ia.kind == 1) {
+      const SurfaceInteraction &si = ia.si;
+      pw = DFactorPBRT(si, ray_dir) * pw;
+      const auto &shd = sc.shaders[sc.materials[si.material].shader];
+      Spec<NW> f = sh.Evaluate(shd, -ps.dir, si, ps.ctx, ps.shader_roughness, ray_dir, &bsdf_pdf);
+      pw = pw * f;
+      MaybeGoingThroughSurface(tr, ray_dir, si);
+    } else {
+      Spec<NW> f = MediumPhaseEval(ia.vmedium, -ps.dir, ia.vpos, ray_dir, ps.ctx, &bsdf_pdf);
+      Spec<NW> ss, st;
+      Coeffs(ia.vmedium, ia.vpos, ps.ctx, ss, st);
+      pw = pw * f;
+      for (int k = 0; k < NW; ++k) pw[k] *= ss[k] / (st[k] + Epsilon);
+    }
+    TandP f = TransmitScene(ray_org, ray_dir, length, smp, tr, ps.ctx);
+    Accumulate<NW>(f.track, ps.weights_track);
+    Accumulate<NW>(f.nulls, ps.weights_track);
+    double w;
+    Spec<NW> den;
+    MisWeight(true, pdf, bsdf_pdf, f.nulls, f.track, w, den);
+    Spec<NW> m;
+    for (int k = 0; k < NW; ++k) m[k] = w * (pw[k] * light_weight[k]) / den[k];
+    Record(m, ps);
+  }
+
+  // MaybeAddEmission (pt2.cxx:627-658)
+  void MaybeAddEmission(const SurfaceInteraction &si, PathState &ps, double hit_dist) const {
+    if (!use_emission) return;
+    const auto &mat = sc.materials[si.material];
+    if (mat.emitter < 0) return;
+    const auto &em = sc.emitters[mat.emitter];
+    Spec<NW> rad = SpecConst<NW>(0.);
+    if (em.kind == TTGPU_AREA_UNIFORM) {
+      double vis = Dot(si.geometry_normal, -ps.dir) > 0. ? 1. : 0.;
+      rad = (vis / UnitHalfSphereSurfaceArea) * Take<NW>(em.spectrum, ps.ctx);
+    }
+    double w;
+    Spec<NW> den;
+    if (ps.has_last_pdf) {
+      const double prob = PmfArea();
+      const double area_pdf = 1. / sc.PrimArea(si.hit.prim);
+      const double cvt = AreaToSolidAngle(hit_dist, ps.dir, si.normal);
+      MisWeight(true, ps.last_pdf, prob * area_pdf * cvt, ps.weights_track, ps.weights_track_then_null, w, den);
+    } else {
+      MisWeight(false, Pdf(), 1., ps.weights_track, ps.weights_track_then_null, w, den);
+    }
+    Spec<NW> c;
+    for (int k = 0; k < NW; ++k) c[k] = w * (rad[k] * ps.path_weights[k]) / den[k];
+    Record(c, ps);
+  }
+
+  // AddEnvEmission (pt2.cxx:661-684)
+  void AddEnvEmission(PathState &ps) const {
+    if (!use_emission || sc.envs.empty()) return;
+    Spec<NW> rad = EnvEval(-ps.dir, ps.ctx);
+    double w;
+    Spec<NW> den;
+    if (ps.has_last_pdf) {
+      MisWeight(true, ps.last_pdf, EnvPdf(-ps.dir) * PmfEnv(), ps.weights_track, ps.weights_track_then_null, w, den);
+    } else {
+      MisWeight(false, Pdf(), 1., ps.weights_track, ps.weights_track_then_null, w, den);
+    }
+    Spec<NW> c;
+    for (int k = 0; k < NW; ++k) c[k] = w * (rad[k] * ps.path_weights[k]) / den[k];
+    Record(c, ps);
+  }
+
+  // MaybeScatter (pt2.cxx:605-622) with PrepareStateForAfterScattering (:583-599).
+  bool MaybeScatter(const Interaction &ia, PathState &ps, Sampler &smp) const {
+    ScatterSample<NW> s;
+    if (ia.kind == 1) {
+      const auto &shd = sc.shaders[sc.materials[ia.si.material].shader];
+      ps.shader_roughness = std::max(sh.MyRoughness(shd), ps.shader_roughness);
+      s = sh.Sample(shd, -ps.dir, ia.si, ps.ctx, ps.shader_roughness, smp);
+    } else {
+      MediumPhaseSample(ia.vmedium, -ps.dir, ia.vpos, smp, ps.ctx, s);
+      Spec<NW> ss, st;
+      Coeffs(ia.vmedium, ia.vpos, ps.ctx, ss, st);
+      for (int k = 0; k < NW; ++k) s.value[k] *= ss[k] / (st[k] + Epsilon);
+    }
+    ps.node_count++;
+    if (ps.node_count > prm.max_ray_depth) return false;
+    if (ia.kind == 1) {
+      double d = DFactorPBRT(ia.si, s.dir);
+      for (int k = 0; k < NW; ++k) ps.path_weights[k] *= s.value[k] * d / s.pdf.val();
+      ps.dir = s.dir;
+      ps.org = ia.si.pos + AntiSelfIntersectionOffset(ia.si, s.dir);
+      MaybeGoingThroughSurface(ps.tracker, s.dir, ia.si);
+    } else {
+      for (int k = 0; k < NW; ++k) ps.path_weights[k] *= s.value[k] / s.pdf.val();
+      ps.dir = s.dir;
+      ps.org = ia.vpos;
+    }
+    ps.last_pdf = s.pdf;
+    ps.has_last_pdf = true;
+    bool all_zero = true;
+    for (int k = 0; k < NW; ++k) all_zero &= ps.path_weights[k] == 0.;
+    return !all_zero;
+  }
+
+  // TrackToNextInteractionAndRecordPixel (pt2.cxx:410-443)
+  bool Step(PathState &ps, Sampler &smp) const {
+    Interaction ia;
+    TandP f = TrackScene(ps.org, ps.dir, smp, ps.tracker, ps.ctx, ia);
+    ps.weights_track_then_null = ps.weights_track;
+    Accumulate<NW>(ps.weights_track_then_null, f.nulls);
+    Accumulate<NW>(ps.weights_track, f.track);
+    if (ia.kind == 0) {
+      AddEnvEmission(ps);
+      return false;
+    }
+    if (ia.kind == 1) {
+      double hit_dist = Length(ps.org - ia.si.pos);
+      MaybeAddEmission(ia.si, ps, hit_dist);
+    }
+    AddDirectLighting(ia, ps, smp);
+    return MaybeScatter(ia, ps, smp);
+  }
+
+  void RenderPath(int px, int py, uint64_t sample_index, double rgb[3]) const {
+    Sampler smp;
+    smp.rng.Seed(prm.seed, sample_index);
+    PathState ps;
+    InitializePath(ps, px, py, smp);
+    bool zero = true;
+    for (int k = 0; k < NW; ++k) zero &= ps.path_weights[k] == 0.;
+    if (!zero)
+      while (Step(ps, smp)) {}
+    rgb[0] = ps.rgb[0]; rgb[1] = ps.rgb[1]; rgb[2] = ps.rgb[2];
+  }
+};
+
+}  // namespace ttref
+
+// ---------------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------------
+using namespace ttref;
+
+static thread_local std::string g_err;
+
+extern "C" const char *ttref_last_error(void) { return g_err.c_str(); }
+
+template <int NW>
+static void RenderImpl(const Scene &sc, const ttgpu_params &p, int spp, int spp_offset, int threads, double *rgb, double *rgb_sq) {
+  Engine<NW> eng(sc, p);
+  const int W = p.width, H = p.height;
+  const int64_t npix = (int64_t)W * H;
+  std::atomic<int> next_row{0};
+  auto worker = [&]() {
+    for (;;) {
+      int y = next_row.fetch_add(1);
+      if (y >= H) break;
+      for (int x = 0; x < W; ++x) {
+        double acc[3] = {0, 0, 0}, acc2[3] = {0, 0, 0};
+        for (int s = 0; s < spp; ++s) {
+          uint64_t gidx = (uint64_t)(s + spp_offset) * (uint64_t)npix + (uint64_t)(y * W + x);
+          double c[3];
+          eng.RenderPath(x, y, gidx, c);
+          for (int k = 0; k < 3; ++k) { acc[k] += c[k]; acc2[k] += c[k] * c[k]; }
+        }
+        int64_t pi = (int64_t)y * W + x;
+        for (int k = 0; k < 3; ++k) {
+          rgb[3 * pi + k] += acc[k];
+          if (rgb_sq) rgb_sq[3 * pi + k] += acc2[k];
+        }
+      }
+    }
+  };
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  std::vector<std::thread> ts;
+  for (int i = 0; i < threads; ++i) ts.emplace_back(worker);
+  for (auto &t : ts) t.join();
+}
+
+extern "C" int ttref_render(const ttgpu_scene *scene, const ttgpu_params *params, int32_t spp, int32_t spp_offset,
+                            int32_t threads, double *rgb, double *rgb_sq) {
+  try {
+    if (!scene || !params || !rgb || spp < 0) throw std::invalid_argument("null argument");
+    if (params->width <= 0 || params->height <= 0) throw std::invalid_argument("bad resolution");
+    Scene sc(*scene);
+    if (params->num_wavelengths == 3) RenderImpl<3>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);
+    else if (params->num_wavelengths == 4) RenderImpl<4>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);
+    else throw std::invalid_argument("num_wavelengths must be 3 or 4");
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Shader sampling / evaluation at a synthetic surface point for the statistical tests.
+static SurfaceInteraction MakeTestSurface(const double ng[3], const double ns[3]) {
+  SurfaceInteraction si;
+  si.geometry_normal = si.normal = Normalized(V3{ng[0], ng[1], ng[2]});
+  si.smooth_normal = si.shading_normal = Normalized(V3{ns[0], ns[1], ns[2]});
+  si.pos = V3{0, 0, 0};
+  si.pos_bounds = V3{0, 0, 0};
+  return si;
+}
+
+extern "C" int ttref_sample_shader(const ttgpu_shader *shader, const double wi[3], const double ng[3], const double ns[3],
+                                   const int32_t bins[4], double roughness, uint64_t seed, int64_t count, double *out_dir,
+                                   double *out_value, double *out_pdf) {
+  try {
+    std::vector<ttgpu_material> mats;
+    ttgpu_scene s{};
+    Scene sc(s);
+    Shading<4> sh(sc);
+    SurfaceInteraction si = MakeTestSurface(ng, ns);
+    PathContext<4> ctx;
+    for (int i = 0; i < 4; ++i) { ctx.bins[i] = bins[i]; ctx.wavelengths[i] = kBinCenters[bins[i]]; }
+    Sampler smp;
+    smp.rng.Seed(seed, 0);
+    V3 w{wi[0], wi[1], wi[2]};
+    for (int64_t i = 0; i < count; ++i) {
+      ScatterSample<4> r = sh.Sample(*shader, w, si, ctx, roughness, smp);
+      out_dir[3 * i] = r.dir.x; out_dir[3 * i + 1] = r.dir.y; out_dir[3 * i + 2] = r.dir.z;
+      for (int k = 0; k < 4; ++k) out_value[4 * i + k] = r.value[k];
+      out_pdf[i] = r.pdf.v;
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+extern "C" int ttref_eval_shader(const ttgpu_shader *shader, const double wi[3], const double ng[3], const double ns[3],
+                                 const int32_t bins[4], double roughness, int64_t count, const double *dirs, double *out_value,
+                                 double *out_pdf) {
+  try {
+    ttgpu_scene s{};
+    Scene sc(s);
+    Shading<4> sh(sc);
+    SurfaceInteraction si = MakeTestSurface(ng, ns);
+    PathContext<4> ctx;
+    for (int i = 0; i < 4; ++i) { ctx.bins[i] = bins[i]; ctx.wavelengths[i] = kBinCenters[bins[i]]; }
+    V3 w{wi[0], wi[1], wi[2]};
+    for (int64_t i = 0; i < count; ++i) {
+      double pdf = 0.;
+      Spec<4> v = sh.Evaluate(*shader, w, si, ctx, roughness, V3{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]}, &pdf);
+      for (int k = 0; k < 4; ++k) out_value[4 * i + k] = v[k];
+      out_pdf[i] = pdf;
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+extern "C" int ttref_sample_phase(int32_t kind, double g, const double wi[3], uint64_t seed, int64_t count, double *out_dir,
+                                  double *out_value, double *out_pdf) {
+  Sampler smp;
+  smp.rng.Seed(seed, 0);
+  V3 w{wi[0], wi[1], wi[2]};
+  for (int64_t i = 0; i < count; ++i) {
+    V3 d;
+    double v;
+    PhaseSample(kind, g, w, smp, d, v);
+    out_dir[3 * i] = d.x; out_dir[3 * i + 1] = d.y; out_dir[3 * i + 2] = d.z;
+    out_value[i] = v;
+    out_pdf[i] = v;
+  }
+  return 0;
+}
+
+extern "C" double ttref_eval_phase(int32_t kind, double g, const double wi[3], const double wo[3]) {
+  return PhaseEval(kind, g, V3{wi[0], wi[1], wi[2]}, V3{wo[0], wo[1], wo[2]});
+}
+
+extern "C" int ttref_trace_closest(const ttgpu_scene *scene, int64_t count, const double *org, const double *dir, const double *tmax,
+                                   double *t, int32_t *prim) {
+  try {
+    Scene sc(*scene);
+    for (int64_t i = 0; i < count; ++i) {
+      Hit h;
+      V3 o{org[3 * i], org[3 * i + 1], org[3 * i + 2]}, d{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+      if (sc.FirstIntersection(o, d, 0., tmax[i], h)) {
+        t[i] = h.t;
+        prim[i] = h.prim;
+      } else {
+        t[i] = Infinity;
+        prim[i] = -1;
+      }
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+extern "C" void ttref_pcg32(uint64_t initstate, uint64_t initseq, int32_t n, uint32_t *out) {
+  Pcg32 r(initstate, initseq);
+  for (int i = 0; i < n; ++i) out[i] = r.NextUInt();
+}
+
+extern "C" void ttref_pcg32_double(uint64_t initstate, uint64_t initseq, int32_t n, double *out) {
+  Pcg32 r(initstate, initseq);
+  for (int i = 0; i < n; ++i) out[i] = r.NextDouble();
+}
+
+extern "C" void ttref_rgb_to_spectrum(const double rgb[3], double out[36]) { RGBToSpectrum(rgb, out); }
+extern "C" void ttref_spectrum_to_rgb(const double spec[36], double out[3]) { SpectrumToRGB(spec, out); }
+extern "C" void ttref_tables(double *centers, double *smits, double *to_rgb, double *sun) {
+  std::memcpy(centers, kBinCenters, sizeof(kBinCenters));
+  std::memcpy(smits, kSmits, sizeof(kSmits));
+  std::memcpy(to_rgb, kSpecToRGB, sizeof(kSpecToRGB));
+  std::memcpy(sun, kSunSpectrum, sizeof(kSunSpectrum));
+}
+extern "C" int ttref_quadratic_err(float a, float b, float c, float ea, float eb, float ec, float *out) {
+  float t0, t1, e0, e1;
+  if (!QuadraticErr<float>(a, b, c, ea, eb, ec, t0, t1, e0, e1)) return 0;
+  out[0] = t0; out[1] = t1; out[2] = e0; out[3] = e1;
+  return 1;
+}
+extern "C" long ttref_bisection(const double *vals, long n, double r) { return BisectionSearch(vals, n, r); }

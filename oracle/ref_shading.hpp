@@ -1,0 +1,594 @@
+// ORACLE — test infrastructure only (see ref_math.hpp).
+//
+// Shaders, phase functions, media coefficients and lights of the pt2 path in
+// FP64, restated from the reference.  Spectral quantities carry NW wavelengths
+// (the reference's Spectral3 has NW = 3; the configurations here use NW = 4).
+#pragma once
+#include "ref_scene.hpp"
+
+namespace ttref {
+
+template <int NW> using Spec = std::array<double, NW>;
+
+template <int NW> inline Spec<NW> SpecConst(double v) { Spec<NW> s; s.fill(v); return s; }
+template <size_t N> inline std::array<double, N> operator*(const std::array<double, N> &a, const std::array<double, N> &b) { std::array<double, N> r; for (size_t i = 0; i < N; ++i) r[i] = a[i] * b[i]; return r; }
+template <size_t N> inline std::array<double, N> operator*(double s, const std::array<double, N> &a) { std::array<double, N> r; for (size_t i = 0; i < N; ++i) r[i] = s * a[i]; return r; }
+template <size_t N> inline std::array<double, N> operator+(const std::array<double, N> &a, const std::array<double, N> &b) { std::array<double, N> r; for (size_t i = 0; i < N; ++i) r[i] = a[i] + b[i]; return r; }
+template <size_t N> inline std::array<double, N> operator-(const std::array<double, N> &a, const std::array<double, N> &b) { std::array<double, N> r; for (size_t i = 0; i < N; ++i) r[i] = a[i] - b[i]; return r; }
+template <size_t N> inline std::array<double, N> operator/(const std::array<double, N> &a, const std::array<double, N> &b) { std::array<double, N> r; for (size_t i = 0; i < N; ++i) r[i] = a[i] / b[i]; return r; }
+template <int NW> inline double Mean(const Spec<NW> &a) { double s = 0; for (int i = 0; i < NW; ++i) s += a[i]; return s / NW; }
+template <int NW> inline double MaxCoeff(const Spec<NW> &a) { double s = a[0]; for (int i = 1; i < NW; ++i) s = std::max(s, a[i]); return s; }
+
+template <int NW> struct PathContext {
+  int bins[NW];
+  double wavelengths[NW];
+};
+
+template <int NW> inline Spec<NW> Take(const float *spectrum, const PathContext<NW> &ctx) {
+  Spec<NW> s;
+  for (int i = 0; i < NW; ++i) s[i] = spectrum[ctx.bins[i]];
+  return s;
+}
+
+// RGBToSpectralSelection (spectral.cxx:143-177): Smits for selected bins only.
+template <int NW> Spec<NW> RGBToSpectralSelection(const double rgb[3], const PathContext<NW> &ctx) {
+  double full[NBINS];
+  RGBToSpectrum(rgb, full);
+  Spec<NW> s;
+  for (int i = 0; i < NW; ++i) s[i] = full[ctx.bins[i]];
+  return s;
+}
+
+struct SurfaceInteraction {
+  Hit hit;
+  V3 pos, geometry_normal, smooth_normal, normal, shading_normal, pos_bounds;
+  int material = -1;
+  void SetOrientedNormals(V3 incident) {  // scene.cxx:239-244
+    double sign = -std::copysign(1.0, Dot(incident, geometry_normal));
+    normal = geometry_normal * sign;
+    shading_normal = smooth_normal * sign;
+  }
+};
+
+// scene.cxx:256-261
+inline V3 AntiSelfIntersectionOffset(const SurfaceInteraction &si, V3 exitant) {
+  const V3 n = si.geometry_normal;
+  // Dot of float bounds with |normal| cast to float, as the reference does.
+  float delta = (float)si.pos_bounds.x * std::abs((float)n.x) + (float)si.pos_bounds.y * std::abs((float)n.y) +
+                (float)si.pos_bounds.z * std::abs((float)n.z);
+  return n * ((double)delta * (Dot(exitant, n) > 0. ? 1. : -1.));
+}
+
+inline double DFactorPBRT(const SurfaceInteraction &si, V3 dir) { return std::abs(Dot(si.shading_normal, dir)); }
+
+template <int NW> struct ScatterSample {
+  V3 dir;
+  Spec<NW> value;
+  Pdf pdf;
+};
+
+// ---- shader physics (shader_physics.hxx) ---------------------------------
+inline V3 Reflected(V3 wi, V3 n) { return n * (2. * Dot(wi, n)) - wi; }
+inline bool Refracted(V3 wi, V3 m, double eta, V3 &out) {
+  const double c = Dot(m, wi);
+  const double t2 = 1.0 - eta * eta * (1. - c * c);
+  if (t2 < 0.) return false;
+  const double s = Dot(wi, m) >= 0. ? 1.0 : -1.0;
+  out = m * (eta * c - s * std::sqrt(t2)) - wi * eta;
+  return true;
+}
+inline V3 HalfVector(V3 wi, V3 wo) {
+  V3 r = wo + wi;
+  double n = Length(r);
+  if (n > 0) return r / n;
+  return OrthogonalSystemZAligned(wi).c0;
+}
+inline bool HalfVectorRefracted(V3 wi, V3 wo, double eta, V3 &out) {
+  V3 r = -(wi * eta + wo);
+  double n = Length(r);
+  if (n > 0) r = r / n; else r = wi;
+  if (Dot(r, wi) * Dot(r, wo) < 0.) { out = r; return true; }
+  return false;
+}
+inline double HalfVectorPdfToReflectedPdf(double pdf_wh, double wh_dot_in) { return pdf_wh * 0.25 / (std::abs(wh_dot_in) + Epsilon); }
+inline double HalfVectorPdfToTransmittedPdf(double pdf_wh, double eta, double dwi, double dwo) {
+  double d = eta * dwi + dwo;
+  if (d == 0.) return pdf_wh;
+  return pdf_wh * std::abs(dwo) / (d * d);
+}
+inline double Schlick(double ks, double c) { return ks + (1. - ks) * std::pow(1 - c, 5.); }
+inline double AverageProjectedSchlick(double ks) { return ks + (1. - ks) * 2. / 42.; }
+inline double FresnelReflectivity(double cs_i, double cs_t, double eta) {
+  double rs_nom = eta * cs_i - cs_t, rs_den = eta * cs_i + cs_t;
+  double rp_nom = eta * cs_t - cs_i, rp_den = eta * cs_t + cs_i;
+  return 0.5 * (Sqr(rs_nom / rs_den) + Sqr(rp_nom / rp_den));
+}
+inline double FresnelReflectivity(double cos_n_wi, double eta) {  // Walter et al. 2007
+  double c = std::abs(cos_n_wi);
+  double tmp = Sqr(1.0 / eta) - 1.0 + c * c;
+  if (tmp < 0.) return 1.;
+  double g = std::sqrt(tmp);
+  if (g == 0. && c == 0.) return 1.;
+  double nom = c * (g + c) - 1., den = c * (g - c) + 1.;
+  return 0.5 * Sqr((g - c) / (g + c)) * (1. + Sqr(nom / den));
+}
+
+// Beckmann NDF (normaldistributionfunction.hxx:8-43)
+struct Beckmann {
+  double alpha;
+  double Eval(double c) const {
+    if (c <= 0.) return 0.;
+    const double t1 = (c * c - 1.) / (c * c * alpha * alpha);
+    const double t2 = alpha * alpha * c * c * c * c * Pi;
+    return std::exp(t1) / t2;
+  }
+  V3 SampleHalfVector(const double r[2]) const {
+    const double t1 = -alpha * alpha * std::log(r[0]);
+    const double t = 1 / (t1 + 1);
+    const double z = std::sqrt(t), rho = std::sqrt(1 - t), om = 2 * Pi * r[1];
+    return {std::cos(om) * rho, std::sin(om) * rho, z};
+  }
+};
+inline double G1VCavity(double cvm, double cvn, double cnm) {
+  if (cvm * cvn < 0.) return 0.;
+  return std::min(1., 2. * std::abs(cnm * cvn) / (std::abs(cvm) + Epsilon));
+}
+inline double G2VCavity(double whi, double who, double ni, double no, double nh) {
+  return std::min(G1VCavity(whi, ni, nh), G1VCavity(who, no, nh));
+}
+inline double G2VCavityTransmissive(double whi, double who, double ni, double no, double nh) {
+  return std::max(G1VCavity(whi, ni, nh) + G1VCavity(who, no, nh) - 1., 0.);
+}
+constexpr double kWiBelowHorizonFix = 1.e-6;
+inline void VndfVCavitySample(V3 &wh, V3 wi, double r) {
+  V3 whp{-wh.x, -wh.y, wh.z};
+  double prob = ClampDot(wi, whp) / (ClampDot(wi, wh) + ClampDot(wi, whp) + kWiBelowHorizonFix);
+  if (r < prob) wh = whp;
+}
+inline double VndfVCavityPdf(double ndf, V3 wh, V3 wi) {
+  V3 whp{-wh.x, -wh.y, wh.z};
+  const double a = ClampDot(wi, whp), b = ClampDot(wh, wi);
+  const double prob = (2. * b + kWiBelowHorizonFix) / (b + a + kWiBelowHorizonFix);
+  return ndf * std::abs(wh.z) * prob;
+}
+
+// GGX (Trowbridge-Reitz) for the new GGX metal: D, Smith Lambda, height-correlated G2,
+// and exact visible-normal sampling after Heitz (2018) "Sampling the GGX Distribution of
+// Visible Normals".  Not in the reference (it only has Beckmann); defined here from the
+// published formulas.
+struct GGX {
+  double alpha;
+  double D(V3 h) const {
+    if (h.z <= 0.) return 0.;
+    double a2 = alpha * alpha;
+    double t = h.x * h.x / a2 + h.y * h.y / a2 + h.z * h.z;
+    return 1. / (Pi * a2 * t * t);
+  }
+  double Lambda(V3 w) const {
+    double z2 = w.z * w.z;
+    if (z2 <= 0.) return Infinity;
+    double t2 = (1. - z2) / z2;
+    return 0.5 * (-1. + std::sqrt(1. + alpha * alpha * t2));
+  }
+  double G1(V3 w) const { return 1. / (1. + Lambda(w)); }
+  double G2(V3 wi, V3 wo) const { return 1. / (1. + Lambda(wi) + Lambda(wo)); }
+  V3 SampleVisible(V3 wi, const double r[2]) const {
+    V3 vh = Normalized(V3{alpha * wi.x, alpha * wi.y, wi.z});
+    double lensq = vh.x * vh.x + vh.y * vh.y;
+    V3 T1 = lensq > 0 ? V3{-vh.y, vh.x, 0} / std::sqrt(lensq) : V3{1, 0, 0};
+    V3 T2 = Cross(vh, T1);
+    double rr = std::sqrt(r[0]);
+    double phi = 2. * Pi * r[1];
+    double t1 = rr * std::cos(phi), t2 = rr * std::sin(phi);
+    double s = 0.5 * (1. + vh.z);
+    t2 = (1. - s) * std::sqrt(std::max(0., 1. - t1 * t1)) + s * t2;
+    V3 nh = T1 * t1 + T2 * t2 + vh * std::sqrt(std::max(0., 1. - t1 * t1 - t2 * t2));
+    return Normalized(V3{alpha * nh.x, alpha * nh.y, std::max(0., nh.z)});
+  }
+  // Density of the visible normal h given wi (wi.z > 0).
+  double PdfVisible(V3 wi, V3 h) const {
+    if (wi.z <= 0.) return 0.;
+    return G1(wi) * ClampDot(wi, h) * D(h) / wi.z;
+  }
+};
+
+// ---- surface shaders -----------------------------------------------------
+template <int NW> struct Shading {
+  const Scene &sc;
+  explicit Shading(const Scene &s) : sc(s) {}
+
+  double MyRoughness(const ttgpu_shader &s) const {  // Shader::MyRoughness overrides
+    switch (s.kind) {
+      case TTGPU_SHADER_BECKMANN_METAL:
+      case TTGPU_SHADER_GGX_METAL: return s.alpha;
+      case TTGPU_SHADER_GLOSSY_DIELECTRIC: return s.alpha_min + (s.alpha - s.alpha_min);
+      default: return 0.;
+    }
+  }
+
+  ScatterSample<NW> Sample(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx,
+                           double rough, Sampler &smp) const {
+    switch (s.kind) {
+      case TTGPU_SHADER_DIFFUSE: return DiffuseSample(s, wi, si, ctx, smp);
+      case TTGPU_SHADER_MIRROR: return MirrorSample(s, wi, si, ctx);
+      case TTGPU_SHADER_DIELECTRIC:
+        return rough > 0. ? DielectricMollifiedSample(s, wi, si, ctx, rough, smp) : DielectricSample(s, wi, si, ctx, smp);
+      case TTGPU_SHADER_BECKMANN_METAL: return BeckmannSample(s, wi, si, ctx, rough, smp);
+      case TTGPU_SHADER_GGX_METAL: return GgxSample(s, wi, si, ctx, rough, smp);
+      case TTGPU_SHADER_INVISIBLE:
+        return {-wi, SpecConst<NW>(-1. / std::abs(Dot(wi, si.shading_normal))), Pdf::Delta(1.)};
+      case TTGPU_SHADER_GLOSSY_DIELECTRIC: return GlossyDielectricSample(s, wi, si, ctx, rough, smp);
+      case TTGPU_SHADER_DENSE_DIELECTRIC: return DenseSample(s, wi, si, ctx, smp);
+    }
+    return {wi, SpecConst<NW>(0.), Pdf(1.)};
+  }
+
+  Spec<NW> Evaluate(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx, double rough,
+                    V3 wo, double *pdf) const {
+    switch (s.kind) {
+      case TTGPU_SHADER_DIFFUSE: return DiffuseEval(s, wi, si, ctx, wo, pdf);
+      case TTGPU_SHADER_DIELECTRIC:
+        if (rough > 0.) return DielectricMollifiedEval(s, wi, si, ctx, rough, wo, pdf);
+        if (pdf) *pdf = 0.;
+        return SpecConst<NW>(0.);
+      case TTGPU_SHADER_BECKMANN_METAL: return BeckmannEval(s, wi, si, ctx, rough, wo, pdf);
+      case TTGPU_SHADER_GGX_METAL: return GgxEval(s, wi, si, ctx, rough, wo, pdf);
+      case TTGPU_SHADER_GLOSSY_DIELECTRIC: return GlossyDielectricEval(s, wi, si, ctx, rough, wo, pdf);
+      case TTGPU_SHADER_DENSE_DIELECTRIC: return DenseEval(s, wi, si, ctx, wo, pdf);
+      default:
+        if (pdf) *pdf = 0.;
+        return SpecConst<NW>(0.);
+    }
+  }
+
+  // DiffuseShader (shader.cxx:251-318), kr_d = reflectance / pi.
+  Spec<NW> DiffuseEval(const ttgpu_shader &s, V3, const SurfaceInteraction &si, const PathContext<NW> &ctx, V3 wo, double *pdf) const {
+    Spec<NW> r = SpecConst<NW>(0.);
+    if (Dot(si.normal, wo) > 0.) r = (1. / Pi) * Take<NW>(s.reflectance, ctx);
+    if (pdf) *pdf = std::max(0., Dot(si.shading_normal, wo)) / Pi;
+    return r;
+  }
+  ScatterSample<NW> DiffuseSample(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx, Sampler &smp) const {
+    M3 m = OrthogonalSystemZAligned(si.shading_normal);
+    double r[2];
+    smp.UniformUnitSquare(r);
+    V3 v = ToCosHemisphere(r);
+    double pdf = v.z / Pi;
+    V3 out = m * v;
+    if (Dot(si.normal, out) > 0) return {out, (1. / Pi) * Take<NW>(s.reflectance, ctx), Pdf(pdf)};
+    return {out, SpecConst<NW>(0.), Pdf(pdf)};
+  }
+
+  // SpecularReflectiveShader (shader.cxx:322-366)
+  ScatterSample<NW> MirrorSample(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx) const {
+    V3 r = Reflected(wi, si.shading_normal);
+    double cos_rn = Dot(si.normal, r);
+    if (cos_rn < 0.) return {r, SpecConst<NW>(0.), Pdf::Delta(1.)};
+    double c = Dot(si.shading_normal, r);
+    Spec<NW> v = Take<NW>(s.reflectance, ctx);
+    for (auto &x : v) x /= c;
+    return {r, v, Pdf::Delta(1.)};
+  }
+
+  double EtaIOverT(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx) const {
+    bool entering = Dot(si.geometry_normal, wi) > 0.;
+    double ior = s.ior_ratio;
+    if (s.ior_lambda_coeff != 0.f) ior = s.ior_ratio + s.ior_lambda_coeff * ctx.wavelengths[0];
+    return entering ? 1. / ior : ior;
+  }
+
+  // SpecularTransmissiveDielectricShader::SampleBsdfRegular (shader.cxx:400-478)
+  ScatterSample<NW> DielectricSample(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx, Sampler &smp) const {
+    double abs_shn_dot_i = std::abs(Dot(si.shading_normal, wi));
+    double eta = EtaIOverT(s, wi, si, ctx);
+    double radiance_weight = Sqr(eta);
+    double fr = 1.;
+    V3 wt;
+    bool has_wt = Refracted(wi, si.shading_normal, eta, wt);
+    if (has_wt) fr = FresnelReflectivity(abs_shn_dot_i, std::abs(Dot(wt, si.shading_normal)), eta);
+    const double prob_refl = has_wt ? std::max(0.1, std::min(0.9, fr)) : 1.;
+    bool do_refl = has_wt ? smp.Uniform01() < prob_refl : true;
+    ScatterSample<NW> smpl;
+    double val;
+    if (do_refl) {
+      smpl.dir = Reflected(wi, si.shading_normal);
+      smpl.pdf = Pdf::Delta(prob_refl);
+      val = fr;
+    } else {
+      smpl.dir = wt;
+      smpl.pdf = Pdf::Delta(1. - prob_refl);
+      val = 1. - fr;
+    }
+    val /= std::abs(Dot(smpl.dir, si.shading_normal));
+    if (Dot(smpl.dir, si.normal) < 0) val *= radiance_weight;
+    smpl.value = SpecConst<NW>(val);
+    if (s.ior_lambda_coeff != 0.f)
+      for (int i = 1; i < NW; ++i) smpl.value[i] = 0.;
+    return smpl;
+  }
+
+  // SampleBsdfMollified (shader.cxx:482-547)
+  ScatterSample<NW> DielectricMollifiedSample(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx,
+                                              double rough, Sampler &smp) const {
+    const double open = 1. - rough;
+    double abs_shn_dot_i = std::abs(Dot(si.shading_normal, wi));
+    double eta = EtaIOverT(s, wi, si, ctx);
+    double radiance_weight = Sqr(eta);
+    double fr = 1.;
+    V3 wt;
+    bool has_wt = Refracted(wi, si.shading_normal, eta, wt);
+    if (has_wt) fr = FresnelReflectivity(abs_shn_dot_i, std::abs(Dot(wt, si.shading_normal)), eta);
+    const double prob_refl = has_wt ? std::max(0.1, std::min(0.9, fr)) : 1.;
+    bool do_refl = has_wt ? smp.Uniform01() < prob_refl : true;
+    const double sp = UniformSphereSectionPdf(open);
+    ScatterSample<NW> smpl;
+    double val;
+    double r[2];
+    if (do_refl) {
+      V3 c = Reflected(wi, si.shading_normal);
+      smp.UniformUnitSquare(r);
+      smpl.dir = OrthogonalSystemZAligned(c) * ToUniformSphereSection(open, r);
+      smpl.pdf = Pdf(prob_refl * sp);
+      val = fr * sp;
+    } else {
+      smp.UniformUnitSquare(r);
+      smpl.dir = OrthogonalSystemZAligned(wt) * ToUniformSphereSection(open, r);
+      smpl.pdf = Pdf((1. - prob_refl) * sp);
+      val = (1. - fr) * sp;
+    }
+    val /= std::abs(Dot(smpl.dir, si.shading_normal));
+    if (Dot(smpl.dir, si.normal) < 0) val *= radiance_weight;
+    smpl.value = SpecConst<NW>(val);
+    if (s.ior_lambda_coeff != 0.f)
+      for (int i = 1; i < NW; ++i) smpl.value[i] = 0.;
+    return smpl;
+  }
+
+  // EvaluateBsdfMollified (shader.cxx:550-604)
+  Spec<NW> DielectricMollifiedEval(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx,
+                                   double rough, V3 wo, double *pdf) const {
+    const double open = 1. - rough;
+    double eta = EtaIOverT(s, wi, si, ctx);
+    double radiance_weight = Sqr(eta);
+    const double abs_shn_dot_i = std::abs(Dot(si.shading_normal, wi));
+    double fr = 1.;
+    V3 wt;
+    bool has_wt = Refracted(wi, si.shading_normal, eta, wt);
+    if (has_wt) fr = FresnelReflectivity(abs_shn_dot_i, std::abs(Dot(wt, si.shading_normal)), eta);
+    const double prob_refl = has_wt ? std::max(0.1, std::min(0.9, fr)) : 1.;
+    const double sp = UniformSphereSectionPdf(open);
+    const V3 refl = Reflected(wi, si.shading_normal);
+    const bool maybe_refl = Dot(refl, wo) > open;
+    double total_pdf = maybe_refl ? sp * prob_refl : 0.;
+    double total_val = maybe_refl ? prob_refl * fr : 0.;
+    if (has_wt) {
+      const bool maybe_refr = Dot(wt, wo) > open;
+      total_val += maybe_refr ? (1. - fr) * sp : 0.;
+      total_pdf += maybe_refr ? (1. - prob_refl) * sp : 0.;
+    }
+    if (Dot(wo, si.normal) < 0) total_val *= radiance_weight;
+    total_val /= std::abs(Dot(wo, si.shading_normal));
+    if (pdf) *pdf = total_pdf;
+    return SpecConst<NW>(total_val);
+  }
+
+  // MicrofacetShader with Beckmann NDF and V-cavity VNDF (shader.cxx:67-123, 634-707)
+  Spec<NW> BeckmannWrapperEval(const Beckmann &ndf, V3 ng_local, const Spec<NW> &color, V3 wi, V3 wh, V3 wo, double *pdf) const {
+    const double n_dot_out = Dot(ng_local, wo);
+    const double ns_out = wo.z, ns_in = wi.z, ns_wh = wh.z;
+    const double wh_out = Dot(wo, wh), wh_in = Dot(wi, wh);
+    double D = ndf.Eval(std::abs(ns_wh));
+    const double G = G2VCavity(wh_in, wh_out, ns_in, ns_out, ns_wh);
+    if (pdf) {
+      V3 whf = wh.z < 0 ? -wh : wh;
+      double sp = VndfVCavityPdf(D, whf, wi);
+      *pdf = HalfVectorPdfToReflectedPdf(sp, Dot(whf, wi));
+    }
+    if (n_dot_out <= 0.) return SpecConst<NW>(0.);
+    Spec<NW> f;
+    for (int i = 0; i < NW; ++i) f[i] = Schlick(color[i], std::abs(wh_in));
+    double mono = G * D * 0.25 / (std::abs(ns_in * ns_out) + Epsilon);
+    return mono * f;
+  }
+  Spec<NW> BeckmannEval(const ttgpu_shader &s, V3 wi_w, const SurfaceInteraction &si, const PathContext<NW> &ctx, double rough,
+                        V3 wo_w, double *pdf) const {
+    M3 m = OrthogonalSystemZAligned(si.shading_normal);
+    V3 ng = m.TMul(si.normal);
+    Beckmann ndf{std::max((double)s.alpha, rough)};
+    V3 wi = m.TMul(wi_w), wo = m.TMul(wo_w);
+    V3 wh = Normalized(wi + wo);
+    return BeckmannWrapperEval(ndf, ng, Take<NW>(s.reflectance, ctx), wi, wh, wo, pdf);
+  }
+  ScatterSample<NW> BeckmannSample(const ttgpu_shader &s, V3 wi_w, const SurfaceInteraction &si, const PathContext<NW> &ctx,
+                                   double rough, Sampler &smp) const {
+    M3 m = OrthogonalSystemZAligned(si.shading_normal);
+    V3 ng = m.TMul(si.normal);
+    Beckmann ndf{std::max((double)s.alpha, rough)};
+    V3 wi = m.TMul(wi_w);
+    double r[2];
+    smp.UniformUnitSquare(r);
+    V3 wh = ndf.SampleHalfVector(r);
+    VndfVCavitySample(wh, wi, smp.Uniform01());
+    V3 wo = Reflected(wi, wh);
+    double pdf;
+    Spec<NW> c = BeckmannWrapperEval(ndf, ng, Take<NW>(s.reflectance, ctx), wi, wh, wo, &pdf);
+    return {m * wo, c, Pdf(pdf)};
+  }
+
+  // GGX metal (new): f = F_schlick(|wh.wi|) D G2 / (4 |n.wi| |n.wo|), sampled with the visible-normal
+  // distribution; value zero below the geometric surface, like the Beckmann wrapper.
+  Spec<NW> GgxCore(const GGX &g, V3 ng_local, const Spec<NW> &color, V3 wi, V3 wo, double *pdf) const {
+    V3 wh = HalfVector(wi, wo);
+    if (wh.z < 0) wh = -wh;
+    if (pdf) {
+      double pv = g.PdfVisible(wi, wh);
+      *pdf = pv * 0.25 / (std::abs(Dot(wi, wh)) + Epsilon);
+    }
+    if (Dot(ng_local, wo) <= 0. || wi.z <= 0. || wo.z <= 0.) return SpecConst<NW>(0.);
+    double D = g.D(wh), G = g.G2(wi, wo);
+    double whi = std::abs(Dot(wh, wi));
+    Spec<NW> f;
+    for (int i = 0; i < NW; ++i) f[i] = Schlick(color[i], whi);
+    return (D * G * 0.25 / (std::abs(wi.z * wo.z) + Epsilon)) * f;
+  }
+  Spec<NW> GgxEval(const ttgpu_shader &s, V3 wi_w, const SurfaceInteraction &si, const PathContext<NW> &ctx, double rough, V3 wo_w,
+                   double *pdf) const {
+    M3 m = OrthogonalSystemZAligned(si.shading_normal);
+    GGX g{std::max((double)s.alpha, rough)};
+    return GgxCore(g, m.TMul(si.normal), Take<NW>(s.reflectance, ctx), m.TMul(wi_w), m.TMul(wo_w), pdf);
+  }
+  ScatterSample<NW> GgxSample(const ttgpu_shader &s, V3 wi_w, const SurfaceInteraction &si, const PathContext<NW> &ctx, double rough,
+                              Sampler &smp) const {
+    M3 m = OrthogonalSystemZAligned(si.shading_normal);
+    GGX g{std::max((double)s.alpha, rough)};
+    V3 wi = m.TMul(wi_w);
+    double r[2];
+    smp.UniformUnitSquare(r);
+    V3 wip = wi.z >= 0 ? wi : V3{wi.x, wi.y, 1e-7};
+    V3 wh = g.SampleVisible(Normalized(wip), r);
+    V3 wo = Reflected(wi, wh);
+    double pdf;
+    Spec<NW> c = GgxCore(g, m.TMul(si.normal), Take<NW>(s.reflectance, ctx), wi, wo, &pdf);
+    return {m * wo, c, Pdf(pdf)};
+  }
+
+  // GlossyTransmissiveDielectricShader (shader.cxx:125-195, 710-787, 937-941)
+  double GlossyDielectricCore(V3 wi, V3 ng, const Beckmann &ndf, const Beckmann &bndf, double eta, V3 wo, double *pdf) const {
+    const double n_dot_out = Dot(ng, wo);
+    const double ns_out = wo.z, ns_in = wi.z;
+    if (pdf) {
+      double pwot = 0.;
+      V3 wht;
+      if (HalfVectorRefracted(wi, wo, eta, wht)) {
+        double frt = FresnelReflectivity(AbsDot(wht, wi), eta);
+        double nt = bndf.Eval(std::abs(wht.z)) * std::abs(wht.z);
+        pwot = HalfVectorPdfToTransmittedPdf(nt, eta, Dot(wht, wi), Dot(wht, wo)) * (1.0 - frt);
+      }
+      V3 whr = HalfVector(wi, wo);
+      double frr = FresnelReflectivity(AbsDot(whr, wi), eta);
+      double nr = bndf.Eval(std::abs(whr.z)) * std::abs(whr.z);
+      *pdf = HalfVectorPdfToReflectedPdf(nr, Dot(whr, wi)) * frr + pwot;
+    }
+    if (n_dot_out >= 0) {
+      V3 whr = HalfVector(wi, wo);
+      double fr = FresnelReflectivity(std::abs(Dot(whr, wi)), eta);
+      double D = ndf.Eval(std::abs(whr.z));
+      double G = G2VCavity(Dot(whr, wi), Dot(wo, whr), ns_in, ns_out, whr.z);
+      return fr * G * D * 0.25 / (std::abs(ns_in * ns_out) + Epsilon);
+    }
+    V3 w;
+    if (!HalfVectorRefracted(wi, wo, eta, w)) return 0.;
+    V3 wht = w.z < 0. ? -w : w;
+    const double whi = Dot(wi, wht), who = Dot(wo, wht);
+    double fr = FresnelReflectivity(std::abs(whi), eta);
+    double D = ndf.Eval(std::abs(wht.z));
+    double J = HalfVectorPdfToTransmittedPdf(1.0, eta, whi, who);
+    double G = G2VCavityTransmissive(whi, who, ns_in, ns_out, wht.z);
+    return (1.0 - fr) * std::abs(whi) * G * D * J / (std::abs(ns_in * ns_out) + Epsilon);
+  }
+  Spec<NW> GlossyDielectricEval(const ttgpu_shader &s, V3 wi_w, const SurfaceInteraction &si, const PathContext<NW> &, double rough,
+                                V3 wo_w, double *pdf) const {
+    M3 m = OrthogonalSystemZAligned(si.shading_normal);
+    V3 wi = m.TMul(wi_w), wo = m.TMul(wo_w), ng = m.TMul(si.normal);
+    double eta = Dot(si.geometry_normal, wi_w) < 0. ? s.ior_ratio : 1.0 / s.ior_ratio;
+    double alpha = std::max(rough, MyRoughness(s));
+    Beckmann ndf{alpha}, b{alpha * (1.2 - 0.2 * std::abs(wi.z))};
+    return SpecConst<NW>(GlossyDielectricCore(wi, ng, ndf, b, eta, wo, pdf));
+  }
+  ScatterSample<NW> GlossyDielectricSample(const ttgpu_shader &s, V3 wi_w, const SurfaceInteraction &si, const PathContext<NW> &,
+                                           double rough, Sampler &smp) const {
+    M3 m = OrthogonalSystemZAligned(si.shading_normal);
+    V3 wi = m.TMul(wi_w), ng = m.TMul(si.normal);
+    double eta = Dot(si.geometry_normal, wi_w) < 0. ? s.ior_ratio : 1.0 / s.ior_ratio;
+    double alpha = std::max(rough, MyRoughness(s));
+    Beckmann ndf{alpha}, b{alpha * (1.2 - 0.2 * std::abs(wi.z))};
+    double r[2];
+    smp.UniformUnitSquare(r);
+    double r2 = smp.Uniform01();
+    V3 wh = b.SampleHalfVector(r);
+    double fr = FresnelReflectivity(AbsDot(wh, wi), eta);
+    V3 wt;
+    if (!Refracted(wi, wh, eta, wt)) fr = 1.0;
+    V3 wo = r2 < fr ? Reflected(wi, wh) : wt;
+    double pdf;
+    double v = GlossyDielectricCore(wi, ng, ndf, b, eta, wo, &pdf);
+    return {m * wo, SpecConst<NW>(v), Pdf(pdf)};
+  }
+
+  // SpecularDenseDielectricShader (shader.cxx:977-1081)
+  Spec<NW> DenseEval(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx, V3 wo, double *pdf) const {
+    if (Dot(si.normal, wo) > 0.) {
+      double ce = std::max(0., Dot(si.shading_normal, wo)), ci = std::max(0., Dot(si.shading_normal, wi));
+      double rf = Schlick(s.specular, ci), other = Schlick(s.specular, ce), avg = AverageProjectedSchlick(s.specular);
+      Spec<NW> v = DiffuseEval(s, wi, si, ctx, wo, pdf);
+      v = ((1. - rf) * (1. - other) / (1. - avg)) * v;
+      if (pdf) *pdf *= (1. - rf);
+      return v;
+    }
+    if (pdf) *pdf = 0.;
+    return SpecConst<NW>(0.);
+  }
+  ScatterSample<NW> DenseSample(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx, Sampler &smp) const {
+    double ci = std::max(0., Dot(si.shading_normal, wi));
+    double rf = Schlick(s.specular, ci);
+    double dv = smp.Uniform01();
+    if (dv < rf) {
+      V3 rd = Reflected(wi, si.shading_normal);
+      if (Dot(si.normal, rd) >= 0.) return {rd, SpecConst<NW>(rf / (ci + Epsilon)), Pdf::Delta(rf)};
+      return {rd, SpecConst<NW>(0.), Pdf::Delta(rf)};
+    }
+    ScatterSample<NW> smpl = DiffuseSample(s, wi, si, ctx, smp);
+    double ce = std::max(0., Dot(si.shading_normal, smpl.dir));
+    double other = Schlick(s.specular, ce), avg = AverageProjectedSchlick(s.specular);
+    smpl.value = ((1. - rf) * (1. - other) / (1. - avg)) * smpl.value;
+    smpl.pdf = Pdf(smpl.pdf.val() * (1. - rf));
+    return smpl;
+  }
+};
+
+// ---- phase functions (phasefunctions.cxx) --------------------------------
+inline V3 MuToDirection(double mu, double r2) {
+  const double sn = std::sin(Pi * 2. * r2), cs = std::cos(Pi * 2. * r2);
+  const double rho = std::sqrt(1. - mu * mu);
+  return {sn * rho, cs * rho, mu};
+}
+inline double HgSampleMu(double r, double g) {
+  double s = 2.0 * r - 1.0, g2 = g * g;
+  if (std::abs(g) > 1.0e-3) {
+    double t1 = (1.0 - g2) / (1.0 + g * s);
+    return (1.0 + g2 - t1 * t1) / (2.0 * g);
+  }
+  double s2 = s * s;
+  return s + 1.5 * g * (1.0 - s2) - 2.0 * g2 * s * (1.0 - s2);
+}
+inline double HgValue(double mu, double g) {
+  double g2 = g * g, term = 1.0 + g2 - 2.0 * g * mu;
+  return 1.0 / (4. * Pi) * (1.0 - g2) / (std::sqrt(term) * term);
+}
+inline double RayleighSampleMu(double r) {
+  double a = std::pow(-2. + 4. * r + std::sqrt(5. - 16. * r + 16. * r * r), 1. / 3.);
+  return a - 1. / a;
+}
+inline double RayleighValue(double mu) { return (3. / (16. * Pi)) * (1. + mu * mu); }
+
+// Sample a phase function; returns direction, value (== pdf for these normalised functions).
+inline void PhaseSample(int kind, double g, V3 wi, Sampler &smp, V3 &dir, double &val) {
+  double r[2];
+  smp.UniformUnitSquare(r);
+  if (kind == TTGPU_PHASE_UNIFORM) {
+    dir = ToUniformSphere(r);
+    val = 1. / UnitSphereSurfaceArea;
+    return;
+  }
+  M3 m = OrthogonalSystemZAligned(-wi);
+  double mu = kind == TTGPU_PHASE_RAYLEIGH ? RayleighSampleMu(r[0]) : HgSampleMu(r[0], g);
+  dir = m * MuToDirection(mu, r[1]);
+  val = kind == TTGPU_PHASE_RAYLEIGH ? RayleighValue(mu) : HgValue(mu, g);
+}
+inline double PhaseEval(int kind, double g, V3 wi, V3 wo) {
+  if (kind == TTGPU_PHASE_UNIFORM) return 1. / UnitSphereSurfaceArea;
+  double mu = Dot(-wi, wo);
+  return kind == TTGPU_PHASE_RAYLEIGH ? RayleighValue(mu) : HgValue(mu, g);
+}
+
+}  // namespace ttref

@@ -1,0 +1,35 @@
+/* ORACLE — test infrastructure only.  C entry points of the FP64 CPU
+ * restatement of ToyTrace's pt2 path, for tests/ and bench.py's cpu_baseline. */
+#ifndef TTREF_H
+#define TTREF_H
+#include "../include/ttgpu.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+const char *ttref_last_error(void);
+/* Adds the sums of `spp` paths per pixel (sample indices spp_offset ..) into rgb
+ * (and squared per-path sums into rgb_sq when non-NULL). */
+int ttref_render(const ttgpu_scene *scene, const ttgpu_params *params, int32_t spp, int32_t spp_offset,
+                 int32_t threads, double *rgb, double *rgb_sq);
+int ttref_sample_shader(const ttgpu_shader *shader, const double wi[3], const double ng[3], const double ns[3],
+                        const int32_t bins[4], double roughness, uint64_t seed, int64_t count, double *out_dir,
+                        double *out_value, double *out_pdf);
+int ttref_eval_shader(const ttgpu_shader *shader, const double wi[3], const double ng[3], const double ns[3],
+                      const int32_t bins[4], double roughness, int64_t count, const double *dirs, double *out_value,
+                      double *out_pdf);
+int ttref_sample_phase(int32_t kind, double g, const double wi[3], uint64_t seed, int64_t count, double *out_dir,
+                       double *out_value, double *out_pdf);
+double ttref_eval_phase(int32_t kind, double g, const double wi[3], const double wo[3]);
+int ttref_trace_closest(const ttgpu_scene *scene, int64_t count, const double *org, const double *dir,
+                        const double *tmax, double *t, int32_t *prim);
+void ttref_pcg32(uint64_t initstate, uint64_t initseq, int32_t n, uint32_t *out);
+void ttref_pcg32_double(uint64_t initstate, uint64_t initseq, int32_t n, double *out);
+void ttref_rgb_to_spectrum(const double rgb[3], double out[36]);
+void ttref_spectrum_to_rgb(const double spec[36], double out[3]);
+void ttref_tables(double *centers, double *smits, double *to_rgb, double *sun);
+int ttref_quadratic_err(float a, float b, float c, float ea, float eb, float ec, float *out);
+long ttref_bisection(const double *vals, long n, double r);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,51 @@
+"""Writes converged oracle renders used by the GPU parity tests.
+
+The oracle is the FP64 CPU restatement of ToyTrace's pt2 path (oracle/).  Each
+fixture holds the per-pixel mean of the linear RGB path estimate and its
+standard error, for a configuration named in the file.  Run from the
+repository root:  python tests/golden/make_oracle_images.py [name ...]
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle_lib  # noqa: E402
+from naivetrace_amd import abi, configs  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+FIXTURES = {
+    # name: (scene builder, width, height, spp, max_depth, mis_mode, sample_mode)
+    "cornell64": (lambda w, h: configs.cornell_box(w, h), 64, 64, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+    "cornell32_consistent": (lambda w, h: configs.cornell_box(w, h), 32, 32, 8192, 8, abi.MIS_CONSISTENT, abi.SAMPLE_BOTH),
+    "cornell32_bsdf": (lambda w, h: configs.cornell_box(w, h), 32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BSDF),
+}
+
+
+def make(name):
+    build, w, h, spp, depth, mis, mode = FIXTURES[name]
+    scene = build(w, h)
+    p = abi.default_params(w, h, max_depth=depth, seed=7)
+    p.mis_mode = mis
+    p.sample_mode = mode
+    t = time.time()
+    rgb, sq = oracle_lib.render(scene, p, spp)
+    mean = rgb / spp
+    var = np.maximum(sq / spp - mean ** 2, 0.0) * spp / (spp - 1)
+    stderr = np.sqrt(var / spp)
+    out = os.path.join(HERE, f"{name}_oracle.npz")
+    np.savez_compressed(out, mean=mean, stderr=stderr, spp=spp, width=w, height=h, max_depth=depth, mis_mode=mis,
+                        sample_mode=mode, seed=7)
+    print(f"{name}: {time.time() - t:.1f}s -> {out}")
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(FIXTURES)
+    for n in names:
+        make(n)

@@ -1,0 +1,153 @@
+"""ctypes binding of the oracle (oracle/libttref.so): the FP64 CPU restatement of
+the pt2 path.  Test infrastructure only: used by tests/ and bench.py's CPU
+baseline, never by the product."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from naivetrace_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "libttref.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            build()
+        L = C.CDLL(ORACLE_LIB)
+        P = C.POINTER
+        f64p, i32p = P(C.c_double), P(C.c_int32)
+        L.ttref_last_error.restype = C.c_char_p
+        L.ttref_render.argtypes = [P(abi.SceneDesc), P(abi.Params), C.c_int32, C.c_int32, C.c_int32, f64p, f64p]
+        L.ttref_sample_shader.argtypes = [P(abi.Shader), f64p, f64p, f64p, i32p, C.c_double, C.c_uint64, C.c_int64,
+                                          f64p, f64p, f64p]
+        L.ttref_eval_shader.argtypes = [P(abi.Shader), f64p, f64p, f64p, i32p, C.c_double, C.c_int64, f64p, f64p, f64p]
+        L.ttref_sample_phase.argtypes = [C.c_int32, C.c_double, f64p, C.c_uint64, C.c_int64, f64p, f64p, f64p]
+        L.ttref_eval_phase.argtypes = [C.c_int32, C.c_double, f64p, f64p]
+        L.ttref_eval_phase.restype = C.c_double
+        L.ttref_trace_closest.argtypes = [P(abi.SceneDesc), C.c_int64, f64p, f64p, f64p, f64p, i32p]
+        L.ttref_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, P(C.c_uint32)]
+        L.ttref_pcg32_double.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, f64p]
+        L.ttref_rgb_to_spectrum.argtypes = [f64p, f64p]
+        L.ttref_spectrum_to_rgb.argtypes = [f64p, f64p]
+        L.ttref_tables.argtypes = [f64p, f64p, f64p, f64p]
+        L.ttref_quadratic_err.argtypes = [C.c_float] * 6 + [P(C.c_float)]
+        L.ttref_bisection.argtypes = [f64p, C.c_long, C.c_double]
+        L.ttref_bisection.restype = C.c_long
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def render(scene, params, spp, spp_offset=0, threads=0, with_sq=True):
+    """Returns (sum_rgb, sum_sq_rgb) per pixel as (h, w, 3) float64 arrays."""
+    d = scene.to_abi()
+    h, w = params.height, params.width
+    rgb = np.zeros((h, w, 3))
+    sq = np.zeros((h, w, 3)) if with_sq else None
+    rc = lib().ttref_render(C.byref(d), C.byref(params), spp, spp_offset, threads, _p(rgb), _p(sq) if with_sq else None)
+    if rc != 0:
+        raise RuntimeError(lib().ttref_last_error().decode())
+    return rgb, sq
+
+
+def sample_shader(shader, wi, ng, ns, bins, roughness, seed, count):
+    L = lib()
+    d = np.zeros((count, 3)); v = np.zeros((count, 4)); p = np.zeros(count)
+    args = [np.asarray(x, dtype=np.float64) for x in (wi, ng, ns)]
+    b = np.asarray(bins, dtype=np.int32)
+    rc = L.ttref_sample_shader(C.byref(shader), _p(args[0]), _p(args[1]), _p(args[2]), _p(b, C.c_int32), roughness, seed,
+                               count, _p(d), _p(v), _p(p))
+    assert rc == 0, L.ttref_last_error()
+    return d, v, p
+
+
+def eval_shader(shader, wi, ng, ns, bins, roughness, dirs):
+    L = lib()
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+    n = len(dirs)
+    v = np.zeros((n, 4)); p = np.zeros(n)
+    args = [np.asarray(x, dtype=np.float64) for x in (wi, ng, ns)]
+    b = np.asarray(bins, dtype=np.int32)
+    rc = L.ttref_eval_shader(C.byref(shader), _p(args[0]), _p(args[1]), _p(args[2]), _p(b, C.c_int32), roughness, n,
+                             _p(dirs), _p(v), _p(p))
+    assert rc == 0, L.ttref_last_error()
+    return v, p
+
+
+def sample_phase(kind, g, wi, seed, count):
+    d = np.zeros((count, 3)); v = np.zeros(count); p = np.zeros(count)
+    w = np.asarray(wi, dtype=np.float64)
+    lib().ttref_sample_phase(kind, g, _p(w), seed, count, _p(d), _p(v), _p(p))
+    return d, v, p
+
+
+def eval_phase(kind, g, wi, wo):
+    return lib().ttref_eval_phase(kind, g, _p(np.asarray(wi, dtype=np.float64)), _p(np.asarray(wo, dtype=np.float64)))
+
+
+def trace_closest(scene, org, direction, tmax):
+    d = scene.to_abi()
+    org = np.ascontiguousarray(org, dtype=np.float64)
+    direction = np.ascontiguousarray(direction, dtype=np.float64)
+    tmax = np.ascontiguousarray(tmax, dtype=np.float64)
+    n = len(org)
+    t = np.zeros(n); prim = np.zeros(n, dtype=np.int32)
+    rc = lib().ttref_trace_closest(C.byref(d), n, _p(org), _p(direction), _p(tmax), _p(t), _p(prim, C.c_int32))
+    assert rc == 0, lib().ttref_last_error()
+    return t, prim
+
+
+def pcg32(initstate, initseq, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().ttref_pcg32(initstate, initseq, n, _p(out, C.c_uint32))
+    return out
+
+
+def pcg32_double(initstate, initseq, n):
+    out = np.zeros(n)
+    lib().ttref_pcg32_double(initstate, initseq, n, _p(out))
+    return out
+
+
+def tables():
+    c = np.zeros(36); s = np.zeros((7, 36)); m = np.zeros((3, 36)); sun = np.zeros(36)
+    lib().ttref_tables(_p(c), _p(s), _p(m), _p(sun))
+    return c, s, m, sun
+
+
+def rgb_to_spectrum(rgb):
+    out = np.zeros(36)
+    lib().ttref_rgb_to_spectrum(_p(np.asarray(rgb, dtype=np.float64)), _p(out))
+    return out
+
+
+def spectrum_to_rgb(spec):
+    out = np.zeros(3)
+    lib().ttref_spectrum_to_rgb(_p(np.asarray(spec, dtype=np.float64)), _p(out))
+    return out
+
+
+def quadratic_err(a, b, c, ea, eb, ec):
+    out = (C.c_float * 4)()
+    ok = lib().ttref_quadratic_err(a, b, c, ea, eb, ec, out)
+    return bool(ok), list(out)
+
+
+def bisection(vals, r):
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    return lib().ttref_bisection(_p(v), len(v), r)

@@ -1,0 +1,146 @@
+"""GPU parity: the HIP path against the oracle, called through the C ABI.
+
+* BVH queries: closest-hit primitive ids and distances equal the oracle's
+  brute-force-free FP64 BVH (watertight tests on both sides).
+* Renders: per-pixel means of the GPU render within the stated statistical
+  bound of the oracle's converged FP64 render of the same scene
+  (tests/golden/*_oracle.npz, written by tests/golden/make_oracle_images.py).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from naivetrace_amd import abi, configs
+from naivetrace_amd.renderer import GpuRenderer
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def _random_rays(n, rng, lo, hi):
+    org = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return org, d
+
+
+def test_device_info(gpu_lib):
+    info = gpu_lib.ttgpu_device_info().decode()
+    assert "gfx950" in info
+
+
+def test_closest_hit_matches_oracle(gpu_lib, oracle):
+    s = configs.cornell_box(16, 16)
+    p = abi.default_params(16, 16)
+    rng = np.random.default_rng(3)
+    org, d = _random_rays(20000, rng, [-0.95, 0.05, -0.95], [0.95, 1.95, 0.95])
+    tmax = np.full(len(org), 1e30)
+    with GpuRenderer(s, p) as r:
+        t, prim, u, v = r.trace_closest(org, d, tmax)
+    t_ref, prim_ref = oracle.trace_closest(s, org, d, tmax)
+    hit = prim_ref >= 0
+    assert hit.mean() > 0.99  # closed room
+    same = prim == prim_ref
+    # Primitive ids agree except for rays grazing a shared edge.
+    assert same.mean() > 0.999, f"{(~same).sum()} mismatches"
+    np.testing.assert_allclose(t[same & hit], t_ref[same & hit], rtol=2e-5, atol=2e-6)
+    assert np.all((u >= -1e-6) & (v >= -1e-6) & (u + v <= 1 + 1e-5))
+
+
+def test_closest_hit_empty_and_miss(gpu_lib):
+    s = configs.cornell_box(8, 8)
+    p = abi.default_params(8, 8)
+    with GpuRenderer(s, p) as r:
+        t, prim, _, _ = r.trace_closest(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))
+        assert len(t) == 0
+        # Rays leaving the open front of the box miss everything.
+        t, prim, _, _ = r.trace_closest([[0, 1, 5]], [[0, 0, 1]], [1e30])
+        assert prim[0] == -1 and np.isinf(t[0])
+        # tmax shorter than the wall distance: no hit.
+        t, prim, _, _ = r.trace_closest([[0, 1, 0]], [[0, 1, 0]], [0.5])
+        assert prim[0] == -1
+        occ = r.trace_occluded([[0, 1, 0], [0, 1, 0]], [[0, 1, 0], [0, 1, 0]], [0.5, 3.0])
+        assert occ.tolist() == [0, 1]
+
+
+def _zscores(gpu_mean, gpu_se, ref_mean, ref_se):
+    se = np.sqrt(gpu_se ** 2 + ref_se ** 2)
+    ok = se > 0
+    z = np.zeros_like(gpu_mean)
+    z[ok] = (gpu_mean[ok] - ref_mean[ok]) / se[ok]
+    # Pixels with zero variance on both sides must agree exactly (up to float rounding).
+    assert np.all(np.abs(gpu_mean[~ok] - ref_mean[~ok]) < 1e-6)
+    return z
+
+
+def _render_parity(name, spp, mis=abi.MIS_REFERENCE, mode=abi.SAMPLE_BOTH):
+    f = np.load(os.path.join(GOLDEN, f"{name}_oracle.npz"))
+    w, h, depth = int(f["width"]), int(f["height"]), int(f["max_depth"])
+    s = configs.cornell_box(w, h)
+    p = abi.default_params(w, h, max_depth=depth, seed=11)
+    p.mis_mode = mis
+    p.sample_mode = mode
+    with GpuRenderer(s, p) as r:
+        r.render_spp(spp)
+        rgb, sq, n = r.framebuffer()
+    assert np.all(n == spp)
+    mean = rgb / spp
+    var = np.maximum(sq / spp - mean ** 2, 0) * spp / (spp - 1)
+    z = _zscores(mean, np.sqrt(var / spp), f["mean"], f["stderr"])
+    frac3 = float(np.mean(np.abs(z) > 3))
+    print(f"{name}: mean z {z.mean():+.4f}, |z|>3 fraction {frac3:.4f}, max |z| {np.abs(z).max():.2f}", flush=True)
+    # Gaussian tails give 0.27 % beyond 3 sigma; heavy-tailed path estimates allow up to 1.5 %.
+    assert frac3 < 0.015
+    # No systematic bias: the mean z-score over all pixel channels is ~N(0, 1/sqrt(n)).
+    assert abs(z.mean()) < 5.0 / np.sqrt(z.size) + 0.02
+    # Image-average radiance agrees to within 0.5 %.
+    np.testing.assert_allclose(mean.mean(axis=(0, 1)), f["mean"].mean(axis=(0, 1)), rtol=5e-3)
+
+
+def test_cornell_parity_reference_mis(gpu_lib):
+    _render_parity("cornell64", 4096)
+
+
+def test_cornell_parity_consistent_mis(gpu_lib):
+    _render_parity("cornell32_consistent", 4096, mis=abi.MIS_CONSISTENT)
+
+
+def test_cornell_parity_bsdf_only(gpu_lib):
+    _render_parity("cornell32_bsdf", 4096, mode=abi.SAMPLE_BSDF)
+
+
+def test_run_schedule_and_image(gpu_lib):
+    """Run(): doubling passes 1,2,4,... capped by max spp (SamplesPerPixelSchedule)."""
+    s = configs.cornell_box(32, 24)
+    p = abi.default_params(32, 24, spp=20)
+    calls = []
+    with GpuRenderer(s, p) as r:
+        r.set_pass_callback(lambda complete: calls.append(complete))
+        r.run()
+        _, _, n = r.framebuffer()
+        assert np.all(n == 20)  # 1 + 2 + 4 + 8 + 5
+        assert calls == [True] * 5
+        img = r.generate_image()
+        assert img.shape == (24, 32, 3) and img.dtype == np.uint8 and img.max() > 0
+        st = r.stats()
+        assert st["total_spp"] == 20 and st["paths_traced"] == 32 * 24 * 20
+
+
+def test_stop_before_run(gpu_lib):
+    s = configs.cornell_box(16, 16)
+    p = abi.default_params(16, 16, spp=64)
+    with GpuRenderer(s, p) as r:
+        r.request_stop()
+        rc = r.run()
+        assert rc == 4  # TTGPU_ERR_INTERRUPTED: no pass was started
+        _, _, n = r.framebuffer()
+        assert n.sum() == 0
+
+
+def test_invalid_scene_rejected(gpu_lib):
+    s = configs.cornell_box(8, 8)
+    s.tri_material[0] = s.tri_material[0].copy()
+    s.tri_material[0][0] = 99
+    with pytest.raises(RuntimeError, match="material"):
+        GpuRenderer(s, abi.default_params(8, 8))

@@ -286,6 +286,10 @@ int ttgpu_trace_closest(ttgpu_handle *h, int64_t count, const float *org, const 
                         const float *tmax, float *t, int32_t *prim, float *u, float *v);
 int ttgpu_trace_occluded(ttgpu_handle *h, int64_t count, const float *org, const float *dir,
                          const float *tmax, int32_t *occluded);
+/* Same as ttgpu_trace_closest but testing every primitive without the BVH
+ * (test entry point: the traversal's own reference on the device). */
+int ttgpu_trace_closest_bruteforce(ttgpu_handle *h, int64_t count, const float *org, const float *dir,
+                                   const float *tmax, float *t, int32_t *prim, float *u, float *v);
 
 /* ---- sampler entry points for the statistical tests ------------------------
  * Run the GPU BSDF sampler of `shader` at a surface with geometric normal ng

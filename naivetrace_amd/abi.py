@@ -109,6 +109,7 @@ EXPORTS = [
     "ttgpu_render_spp", "ttgpu_request_interrupt", "ttgpu_request_stop", "ttgpu_set_pass_callback",
     "ttgpu_reset_framebuffer", "ttgpu_read_framebuffer", "ttgpu_generate_image", "ttgpu_stats_get",
     "ttgpu_kernel_time", "ttgpu_enable_kernel_timing", "ttgpu_trace_closest", "ttgpu_trace_occluded",
+    "ttgpu_trace_closest_bruteforce",
     "ttgpu_sample_shader", "ttgpu_eval_shader", "ttgpu_sample_phase", "ttgpu_device_info",
     "ttgpu_abi_version",
 ]
@@ -138,6 +139,7 @@ def _declare(lib):
         "ttgpu_enable_kernel_timing": (C.c_int, [h, C.c_int32]),
         "ttgpu_trace_closest": (C.c_int, [h, C.c_int64, f32p, f32p, f32p, f32p, i32p, f32p, f32p]),
         "ttgpu_trace_occluded": (C.c_int, [h, C.c_int64, f32p, f32p, f32p, i32p]),
+        "ttgpu_trace_closest_bruteforce": (C.c_int, [h, C.c_int64, f32p, f32p, f32p, f32p, i32p, f32p, f32p]),
         "ttgpu_sample_shader": (C.c_int, [P(Shader), f32p, f32p, f32p, i32p, C.c_float, C.c_uint64,
                                           C.c_int64, f32p, f32p, f32p]),
         "ttgpu_eval_shader": (C.c_int, [P(Shader), f32p, f32p, f32p, i32p, C.c_float, C.c_int64,

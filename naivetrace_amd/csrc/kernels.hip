@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DeviceScene sc, PathQueue
     r.tmax = 3.0e38f;
     HitRec h;
     bool any = traverse<false>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
-    q.hit[i] = any ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim)) : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    q.hit[i] = any ? make_float4(h.t, h.u, h.v, as_float(h.prim)) : make_float4(0.f, 0.f, 0.f, as_float(-1));
   }
   if (gtid() == 0) atomicAdd(stats + kStatClosest, count);
 }
@@ -249,7 +249,7 @@ __device__ __forceinline__ SurfHit make_hit(const DeviceScene &sc, int prim, flo
   const PrimRecord pr = sc.prims[prim];
   h.prim = prim;
   h.material = sc.prim_material[prim];
-  if (__float_as_int(pr.b.w) == 0) {
+  if (as_int(pr.b.w) == 0) {
     const float3 v0 = xyz(pr.a), v1 = xyz(pr.b), v2 = xyz(pr.c);
     const float b0 = 1.f - u - v;
     h.pos = v0 * b0 + v1 * u + v2 * v;
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, con
       const bool has_last = (flags & kFlagHasLastPdf) != 0;
       const bool last_delta = (flags & kFlagLastPdfDelta) != 0;
       Pcg rng = load_rng(q, i);
-      const int prim = __float_as_int(hitv.w);
+      const int prim = as_int(hitv.w);
       if (prim < 0) {
         // AddEnvEmission (pt2.cxx:661-684)
         if (use_emission && sc.num_envs > 0) {
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, con
             const PrimRecord pr = sc.prims[lp];
             const float r0 = rng.uniform(), r1 = rng.uniform();
             float3 lpos, lng, lns;
-            if (__float_as_int(pr.b.w) == 0) {
+            if (as_int(pr.b.w) == 0) {
               const float3 v0 = xyz(pr.a), v1 = xyz(pr.b), v2 = xyz(pr.c);
               const float s = sqrtf(r0);
               const float b0 = 1.f - s, b1 = (1.f - r1) * s, b2 = r1 * s;
@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DeviceScene sc, PathQueu
 
 // ---- debug / test kernels -------------------------------------------------------------
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DeviceScene sc, int64_t n, const float *org, const float *dir, const float *tmax,
-                                                            float *t, int32_t *prim, float *u, float *v, int32_t *occl) {
+                                                            float *t, int32_t *prim, float *u, float *v, int32_t *occl, int brute) {
   __shared__ int lds_stack[kLdsStack * kTraceBlock];
   for (int64_t i = gtid(); i < n; i += grid_stride()) {
     Ray r;
@@ -511,12 +511,26 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DeviceScene sc, int6
     r.dir = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
     r.tmax = tmax[i];
     HitRec h;
-    if (occl) {
+    if (brute) {
+      // Every primitive, no BVH: the reference the traversal is checked against.
+      RayPre p = precompute(r);
+      float best = r.tmax;
+      int bi = -1;
+      float bu = 0.f, bv = 0.f;
+      for (int k = 0; k < sc.num_prims; ++k) {
+        float tt, uu, vv;
+        if (intersect_prim(sc.prims[k], p, r, best, tt, uu, vv)) { best = tt; bi = k; bu = uu; bv = vv; }
+      }
+      t[i] = bi >= 0 ? best : as_float(0x7f800000);
+      prim[i] = bi >= 0 ? as_int(sc.prims[bi].a.w) : -1;
+      u[i] = bu;
+      v[i] = bv;
+    } else if (occl) {
       occl[i] = traverse<true>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h) ? 1 : 0;
     } else {
       bool any = traverse<false>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
-      t[i] = any ? h.t : __int_as_float(0x7f800000);
-      prim[i] = any ? __float_as_int(sc.prims[h.prim].a.w) : -1;
+      t[i] = any ? h.t : as_float(0x7f800000);
+      prim[i] = any ? as_int(sc.prims[h.prim].a.w) : -1;
       u[i] = any ? h.u : 0.f;
       v[i] = any ? h.v : 0.f;
     }
@@ -614,9 +628,9 @@ void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count
   hipLaunchKernelGGL(k_shadow, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq, q.counters);
 }
 void LaunchTraceRays(const DeviceScene &sc, int64_t n, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
-                     float *u, float *v, int32_t *occl, hipStream_t st) {
+                     float *u, float *v, int32_t *occl, int brute, hipStream_t st) {
   hipLaunchKernelGGL(k_trace_rays, dim3(Blocks(n, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, n, org, dir, tmax, t, prim, u, v,
-                     occl);
+                     occl, brute);
 }
 void LaunchSampleShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, uint64_t seed, int64_t n,
                         float *d, float *v, float *p, hipStream_t st) {

@@ -17,7 +17,7 @@ void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
                  uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st);
 void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st);
 void LaunchTraceRays(const DeviceScene &sc, int64_t n, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
-                     float *u, float *v, int32_t *occl, hipStream_t st);
+                     float *u, float *v, int32_t *occl, int brute, hipStream_t st);
 void LaunchSampleShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, uint64_t seed, int64_t n,
                         float *d, float *v, float *p, hipStream_t st);
 void LaunchEvalShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, int64_t n, const float *dirs,

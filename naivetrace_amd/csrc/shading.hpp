@@ -28,12 +28,12 @@ struct Scatter {
   bool delta;
 };
 
-__device__ __forceinline__ float3 offset_origin(const SurfHit &h, float3 dir) {
+TT_HD float3 offset_origin(const SurfHit &h, float3 dir) {
   return h.pos + h.ng * (dot(dir, h.ng) > 0.f ? h.offset : -h.offset);
 }
 
-__device__ __forceinline__ float3 reflected(float3 wi, float3 n) { return n * (2.f * dot(wi, n)) - wi; }
-__device__ __forceinline__ bool refracted(float3 wi, float3 m, float eta, float3 &out) {
+TT_HD float3 reflected(float3 wi, float3 n) { return n * (2.f * dot(wi, n)) - wi; }
+TT_HD bool refracted(float3 wi, float3 m, float eta, float3 &out) {
   const float c = dot(m, wi);
   const float t2 = 1.f - eta * eta * (1.f - c * c);
   if (t2 < 0.f) return false;
@@ -41,30 +41,30 @@ __device__ __forceinline__ bool refracted(float3 wi, float3 m, float eta, float3
   out = m * (eta * c - s * sqrtf(t2)) - wi * eta;
   return true;
 }
-__device__ __forceinline__ float3 half_vector(float3 wi, float3 wo) {
+TT_HD float3 half_vector(float3 wi, float3 wo) {
   float3 r = wo + wi;
   float n = length(r);
   if (n > 0.f) return r * (1.f / n);
   return make_frame(wi).x;
 }
-__device__ __forceinline__ bool half_vector_refracted(float3 wi, float3 wo, float eta, float3 &out) {
+TT_HD bool half_vector_refracted(float3 wi, float3 wo, float eta, float3 &out) {
   float3 r = -(wi * eta + wo);
   float n = length(r);
   r = n > 0.f ? r * (1.f / n) : wi;
   if (dot(r, wi) * dot(r, wo) < 0.f) { out = r; return true; }
   return false;
 }
-__device__ __forceinline__ float schlick(float ks, float c) {
+TT_HD float schlick(float ks, float c) {
   float m = 1.f - c;
   float m2 = m * m;
   return ks + (1.f - ks) * (m2 * m2 * m);
 }
-__device__ __forceinline__ float fresnel(float cs_i, float cs_t, float eta) {
+TT_HD float fresnel(float cs_i, float cs_t, float eta) {
   float rs = (eta * cs_i - cs_t) / (eta * cs_i + cs_t);
   float rp = (eta * cs_t - cs_i) / (eta * cs_t + cs_i);
   return 0.5f * (rs * rs + rp * rp);
 }
-__device__ __forceinline__ float fresnel_walter(float cos_n_wi, float eta) {
+TT_HD float fresnel_walter(float cos_n_wi, float eta) {
   float c = fabsf(cos_n_wi);
   float tmp = 1.f / (eta * eta) - 1.f + c * c;
   if (tmp < 0.f) return 1.f;
@@ -75,45 +75,45 @@ __device__ __forceinline__ float fresnel_walter(float cos_n_wi, float eta) {
 }
 
 // Beckmann NDF (normaldistributionfunction.hxx:8-43).
-__device__ __forceinline__ float beckmann_d(float alpha, float c) {
+TT_HD float beckmann_d(float alpha, float c) {
   if (c <= 0.f) return 0.f;
   float c2 = c * c, a2 = alpha * alpha;
-  return __expf((c2 - 1.f) / (c2 * a2)) / (a2 * c2 * c2 * kPi);
+  return fast_exp((c2 - 1.f) / (c2 * a2)) / (a2 * c2 * c2 * kPi);
 }
-__device__ __forceinline__ float3 beckmann_sample(float alpha, float r0, float r1) {
-  float t1 = -alpha * alpha * __logf(r0);
+TT_HD float3 beckmann_sample(float alpha, float r0, float r1) {
+  float t1 = -alpha * alpha * fast_log(r0);
   float t = 1.f / (t1 + 1.f);
   float z = sqrtf(t), rho = sqrtf(fmaxf(0.f, 1.f - t));
   float sn, cs;
-  __sincosf(2.f * kPi * r1, &sn, &cs);
+  fast_sincos(2.f * kPi * r1, &sn, &cs);
   return mk(cs * rho, sn * rho, z);
 }
-__device__ __forceinline__ float g1_vcavity(float cvm, float cvn, float cnm) {
+TT_HD float g1_vcavity(float cvm, float cvn, float cnm) {
   if (cvm * cvn < 0.f) return 0.f;
   return fminf(1.f, 2.f * fabsf(cnm * cvn) / (fabsf(cvm) + kDblEps));
 }
 constexpr float kWiBelowHorizonFix = 1.e-6f;
 
 // GGX NDF, Smith Lambda, visible-normal sampling (Heitz 2018).
-__device__ __forceinline__ float ggx_d(float a, float3 h) {
+TT_HD float ggx_d(float a, float3 h) {
   if (h.z <= 0.f) return 0.f;
   float a2 = a * a;
   float t = (h.x * h.x + h.y * h.y) / a2 + h.z * h.z;
   return 1.f / (kPi * a2 * t * t);
 }
-__device__ __forceinline__ float ggx_lambda(float a, float3 w) {
+TT_HD float ggx_lambda(float a, float3 w) {
   float z2 = w.z * w.z;
   if (z2 <= 0.f) return 1e30f;
   return 0.5f * (-1.f + sqrtf(1.f + a * a * (1.f - z2) / z2));
 }
-__device__ __forceinline__ float3 ggx_sample_visible(float a, float3 wi, float r0, float r1) {
+TT_HD float3 ggx_sample_visible(float a, float3 wi, float r0, float r1) {
   float3 vh = normalize(mk(a * wi.x, a * wi.y, wi.z));
   float lensq = vh.x * vh.x + vh.y * vh.y;
   float3 T1 = lensq > 0.f ? mk(-vh.y, vh.x, 0.f) * (1.f / sqrtf(lensq)) : mk(1.f, 0.f, 0.f);
   float3 T2 = cross(vh, T1);
   float rr = sqrtf(r0);
   float sn, cs;
-  __sincosf(2.f * kPi * r1, &sn, &cs);
+  fast_sincos(2.f * kPi * r1, &sn, &cs);
   float t1 = rr * cs, t2 = rr * sn;
   float s = 0.5f * (1.f + vh.z);
   t2 = (1.f - s) * sqrtf(fmaxf(0.f, 1.f - t1 * t1)) + s * t2;
@@ -122,16 +122,16 @@ __device__ __forceinline__ float3 ggx_sample_visible(float a, float3 wi, float r
 }
 
 // ---- per-shader sampling / evaluation ---------------------------------------
-__device__ __forceinline__ float my_roughness(const DevShader &s) {
+TT_HD float my_roughness(const DevShader &s) {
   return (s.kind == TTGPU_SHADER_BECKMANN_METAL || s.kind == TTGPU_SHADER_GGX_METAL || s.kind == TTGPU_SHADER_GLOSSY_DIELECTRIC)
              ? s.alpha : 0.f;
 }
 
-__device__ __forceinline__ float4 diffuse_eval(const DevShader &s, const SurfHit &h, uint32_t bins, float3 wo, float *pdf) {
+TT_HD float4 diffuse_eval(const DevShader &s, const SurfHit &h, uint32_t bins, float3 wo, float *pdf) {
   *pdf = fmaxf(0.f, dot(h.ns, wo)) * kInvPi;
   return dot(h.n, wo) > 0.f ? kInvPi * take(s.reflectance, bins) : f4(0.f);
 }
-__device__ __forceinline__ Scatter diffuse_sample(const DevShader &s, const SurfHit &h, uint32_t bins, Pcg &rng) {
+TT_HD Scatter diffuse_sample(const DevShader &s, const SurfHit &h, uint32_t bins, Pcg &rng) {
   float r0 = rng.uniform(), r1 = rng.uniform();
   float3 v = to_cos_hemisphere(r0, r1);
   Scatter out;
@@ -143,7 +143,7 @@ __device__ __forceinline__ Scatter diffuse_sample(const DevShader &s, const Surf
 }
 
 // Microfacet metal: Beckmann with V-cavity VNDF (shader.cxx:67-123, 634-707), or GGX.
-__device__ __forceinline__ float4 metal_core(const DevShader &s, float alpha, float3 ng_l, float3 wi, float3 wh, float3 wo,
+TT_HD float4 metal_core(const DevShader &s, float alpha, float3 ng_l, float3 wi, float3 wh, float3 wo,
                                              uint32_t bins, float *pdf) {
   float4 col = take(s.reflectance, bins);
   if (s.kind == TTGPU_SHADER_BECKMANN_METAL) {
@@ -174,7 +174,7 @@ __device__ __forceinline__ float4 metal_core(const DevShader &s, float alpha, fl
   float fr = fabsf(dot(h, wi));
   return make_float4(f * schlick(col.x, fr), f * schlick(col.y, fr), f * schlick(col.z, fr), f * schlick(col.w, fr));
 }
-__device__ __forceinline__ float4 metal_eval(const DevShader &s, const SurfHit &h, float3 wi_w, uint32_t bins, float rough, float3 wo_w,
+TT_HD float4 metal_eval(const DevShader &s, const SurfHit &h, float3 wi_w, uint32_t bins, float rough, float3 wo_w,
                                              float *pdf) {
   Frame f = make_frame(h.ns);
   float3 wi = f.to_local(wi_w), wo = f.to_local(wo_w);
@@ -182,7 +182,7 @@ __device__ __forceinline__ float4 metal_eval(const DevShader &s, const SurfHit &
   float3 wh = s.kind == TTGPU_SHADER_BECKMANN_METAL ? normalize(wi + wo) : half_vector(wi, wo);
   return metal_core(s, alpha, f.to_local(h.n), wi, wh, wo, bins, pdf);
 }
-__device__ __forceinline__ Scatter metal_sample(const DevShader &s, const SurfHit &h, float3 wi_w, uint32_t bins, float rough, Pcg &rng) {
+TT_HD Scatter metal_sample(const DevShader &s, const SurfHit &h, float3 wi_w, uint32_t bins, float rough, Pcg &rng) {
   Frame f = make_frame(h.ns);
   float3 wi = f.to_local(wi_w);
   float alpha = fmaxf(s.alpha, rough);
@@ -209,12 +209,12 @@ __device__ __forceinline__ Scatter metal_sample(const DevShader &s, const SurfHi
 }
 
 // Specular dielectric, regular and mollified (shader.cxx:400-604).
-__device__ __forceinline__ float eta_i_over_t(const DevShader &s, const SurfHit &h, float3 wi, float lambda0) {
+TT_HD float eta_i_over_t(const DevShader &s, const SurfHit &h, float3 wi, float lambda0) {
   bool entering = dot(h.ng, wi) > 0.f;
   float ior = s.ior_lambda_coeff != 0.f ? s.ior_ratio + s.ior_lambda_coeff * lambda0 : s.ior_ratio;
   return entering ? 1.f / ior : ior;
 }
-__device__ __forceinline__ Scatter dielectric_sample(const DevShader &s, const SurfHit &h, float3 wi, float rough, float lambda0, Pcg &rng) {
+TT_HD Scatter dielectric_sample(const DevShader &s, const SurfHit &h, float3 wi, float rough, float lambda0, Pcg &rng) {
   float abs_ni = fabsf(dot(h.ns, wi));
   float eta = eta_i_over_t(s, h, wi, lambda0);
   float fr = 1.f;
@@ -246,7 +246,7 @@ __device__ __forceinline__ Scatter dielectric_sample(const DevShader &s, const S
   if (s.ior_lambda_coeff != 0.f) out.value.y = out.value.z = out.value.w = 0.f;
   return out;
 }
-__device__ __forceinline__ float4 dielectric_eval(const DevShader &s, const SurfHit &h, float3 wi, float rough, float lambda0, float3 wo,
+TT_HD float4 dielectric_eval(const DevShader &s, const SurfHit &h, float3 wi, float rough, float lambda0, float3 wo,
                                                   float *pdf) {
   if (!(rough > 0.f)) { *pdf = 0.f; return f4(0.f); }
   const float open = 1.f - rough;
@@ -272,7 +272,7 @@ __device__ __forceinline__ float4 dielectric_eval(const DevShader &s, const Surf
 }
 
 // Glossy transmissive dielectric (shader.cxx:125-195, 710-787).
-__device__ __forceinline__ float glossy_diel_core(float3 wi, float3 ng, float alpha, float balpha, float eta, float3 wo, float *pdf) {
+TT_HD float glossy_diel_core(float3 wi, float3 ng, float alpha, float balpha, float eta, float3 wo, float *pdf) {
   const float ns_out = wo.z, ns_in = wi.z;
   float pwot = 0.f;
   float3 wht;
@@ -304,14 +304,14 @@ __device__ __forceinline__ float glossy_diel_core(float3 wi, float3 ng, float al
   float G = fmaxf(g1_vcavity(whi, ns_in, wt.z) + g1_vcavity(who, ns_out, wt.z) - 1.f, 0.f);
   return (1.f - fr) * fabsf(whi) * G * D * J / (fabsf(ns_in * ns_out) + kDblEps);
 }
-__device__ __forceinline__ float4 glossy_diel_eval(const DevShader &s, const SurfHit &h, float3 wi_w, float rough, float3 wo_w, float *pdf) {
+TT_HD float4 glossy_diel_eval(const DevShader &s, const SurfHit &h, float3 wi_w, float rough, float3 wo_w, float *pdf) {
   Frame f = make_frame(h.ns);
   float3 wi = f.to_local(wi_w), wo = f.to_local(wo_w);
   float eta = dot(h.ng, wi_w) < 0.f ? s.ior_ratio : 1.f / s.ior_ratio;
   float alpha = fmaxf(rough, s.alpha);
   return f4(glossy_diel_core(wi, f.to_local(h.n), alpha, alpha * (1.2f - 0.2f * fabsf(wi.z)), eta, wo, pdf));
 }
-__device__ __forceinline__ Scatter glossy_diel_sample(const DevShader &s, const SurfHit &h, float3 wi_w, float rough, Pcg &rng) {
+TT_HD Scatter glossy_diel_sample(const DevShader &s, const SurfHit &h, float3 wi_w, float rough, Pcg &rng) {
   Frame f = make_frame(h.ns);
   float3 wi = f.to_local(wi_w);
   float eta = dot(h.ng, wi_w) < 0.f ? s.ior_ratio : 1.f / s.ior_ratio;
@@ -331,9 +331,9 @@ __device__ __forceinline__ Scatter glossy_diel_sample(const DevShader &s, const 
   return out;
 }
 
-__device__ __forceinline__ float average_projected_schlick(float ks) { return ks + (1.f - ks) * 2.f / 42.f; }
+TT_HD float average_projected_schlick(float ks) { return ks + (1.f - ks) * 2.f / 42.f; }
 
-__device__ Scatter sample_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, Pcg &rng) {
+TT_HD Scatter sample_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, Pcg &rng) {
   switch (s.kind) {
     case TTGPU_SHADER_DIFFUSE: return diffuse_sample(s, h, bins, rng);
     case TTGPU_SHADER_MIRROR: {
@@ -380,7 +380,7 @@ __device__ Scatter sample_bsdf(const DevShader &s, const SurfHit &h, float3 wi, 
   return o;
 }
 
-__device__ float4 eval_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, float3 wo, float *pdf) {
+TT_HD float4 eval_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, float3 wo, float *pdf) {
   switch (s.kind) {
     case TTGPU_SHADER_DIFFUSE: return diffuse_eval(s, h, bins, wo, pdf);
     case TTGPU_SHADER_DIELECTRIC: return dielectric_eval(s, h, wi, rough, lambda0, wo, pdf);
@@ -404,13 +404,13 @@ __device__ float4 eval_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uin
 }
 
 // ---- phase functions (phasefunctions.cxx) -------------------------------------
-__device__ __forceinline__ float3 mu_to_direction(float mu, float r2) {
+TT_HD float3 mu_to_direction(float mu, float r2) {
   float sn, cs;
-  __sincosf(2.f * kPi * r2, &sn, &cs);
+  fast_sincos(2.f * kPi * r2, &sn, &cs);
   float rho = sqrtf(fmaxf(0.f, 1.f - mu * mu));
   return mk(sn * rho, cs * rho, mu);
 }
-__device__ __forceinline__ float hg_sample_mu(float r, float g) {
+TT_HD float hg_sample_mu(float r, float g) {
   float s = 2.f * r - 1.f, g2 = g * g;
   if (fabsf(g) > 1.0e-3f) {
     float t1 = (1.f - g2) / (1.f + g * s);
@@ -419,17 +419,17 @@ __device__ __forceinline__ float hg_sample_mu(float r, float g) {
   float s2 = s * s;
   return s + 1.5f * g * (1.f - s2) - 2.f * g2 * s * (1.f - s2);
 }
-__device__ __forceinline__ float hg_value(float mu, float g) {
+TT_HD float hg_value(float mu, float g) {
   float g2 = g * g, term = 1.f + g2 - 2.f * g * mu;
   return (1.f / (4.f * kPi)) * (1.f - g2) / (sqrtf(term) * term);
 }
-__device__ __forceinline__ float rayleigh_sample_mu(float r) {
+TT_HD float rayleigh_sample_mu(float r) {
   float a = cbrtf(-2.f + 4.f * r + sqrtf(5.f - 16.f * r + 16.f * r * r));
   return a - 1.f / a;
 }
-__device__ __forceinline__ float rayleigh_value(float mu) { return (3.f / (16.f * kPi)) * (1.f + mu * mu); }
+TT_HD float rayleigh_value(float mu) { return (3.f / (16.f * kPi)) * (1.f + mu * mu); }
 
-__device__ __forceinline__ void phase_sample(int kind, float g, float3 wi, Pcg &rng, float3 &dir, float &val) {
+TT_HD void phase_sample(int kind, float g, float3 wi, Pcg &rng, float3 &dir, float &val) {
   float r0 = rng.uniform(), r1 = rng.uniform();
   if (kind == TTGPU_PHASE_UNIFORM) {
     dir = to_uniform_sphere(r0, r1);
@@ -442,7 +442,7 @@ __device__ __forceinline__ void phase_sample(int kind, float g, float3 wi, Pcg &
   dir = f.to_world(mu_to_direction(mu, r1));
   val = kind == TTGPU_PHASE_RAYLEIGH ? rayleigh_value(mu) : hg_value(mu, g);
 }
-__device__ __forceinline__ float phase_eval(int kind, float g, float3 wi, float3 wo) {
+TT_HD float phase_eval(int kind, float g, float3 wi, float3 wo) {
   if (kind == TTGPU_PHASE_UNIFORM) return 1.f / (4.f * kPi);
   float mu = dot(-wi, wo);
   return kind == TTGPU_PHASE_RAYLEIGH ? rayleigh_value(mu) : hg_value(mu, g);

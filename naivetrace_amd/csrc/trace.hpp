@@ -29,7 +29,7 @@ struct RayPre {
   float Sx, Sy, Sz;
 };
 
-__device__ __forceinline__ RayPre precompute(const Ray &r) {
+TT_HD RayPre precompute(const Ray &r) {
   RayPre p;
   const float tiny = 1e-30f;
   float dx = fabsf(r.dir.x) > tiny ? r.dir.x : copysignf(tiny, r.dir.x);
@@ -51,10 +51,10 @@ __device__ __forceinline__ RayPre precompute(const Ray &r) {
   return p;
 }
 
-__device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+TT_HD float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
 // Watertight test; on a hit within (0, tmax) writes t and barycentrics (b1, b2) of v1, v2.
-__device__ __forceinline__ bool intersect_triangle(const RayPre &p, float3 org, float3 v0, float3 v1, float3 v2, float tmax, float &t,
+TT_HD bool intersect_triangle(const RayPre &p, float3 org, float3 v0, float3 v1, float3 v2, float tmax, float &t,
                                                    float &b1, float &b2) {
   float3 A = v0 - org, B = v1 - org, C = v2 - org;
   const float Akz = comp(A, p.kz), Bkz = comp(B, p.kz), Ckz = comp(C, p.kz);
@@ -72,10 +72,25 @@ __device__ __forceinline__ bool intersect_triangle(const RayPre &p, float3 org, 
   if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
   const float det = U + V + W;
   if (det == 0.f) return false;
-  const float T = U * (p.Sz * Akz) + V * (p.Sz * Bkz) + W * (p.Sz * Ckz);
+  const float Az = p.Sz * Akz, Bz = p.Sz * Bkz, Cz = p.Sz * Ckz;
+  const float T = U * Az + V * Bz + W * Cz;
   const float inv = 1.f / det;
   const float tt = T * inv;
   if (!(tt > 0.f && tt < tmax)) return false;
+  // Conservative lower bound on t (PBRT-v3 Triangle::Intersect): rays leaving a surface do not
+  // re-hit it at a distance below the rounding error of t.
+  {
+    constexpr float e = kFltEps;
+    const float g2 = 2.f * e / (1.f - 2.f * e), g3 = 3.f * e / (1.f - 3.f * e), g5 = 5.f * e / (1.f - 5.f * e);
+    const float maxZt = fmaxf(fabsf(Az), fmaxf(fabsf(Bz), fabsf(Cz)));
+    const float maxXt = fmaxf(fabsf(Ax), fmaxf(fabsf(Bx), fabsf(Cx)));
+    const float maxYt = fmaxf(fabsf(Ay), fmaxf(fabsf(By), fabsf(Cy)));
+    const float deltaZ = g3 * maxZt, deltaX = g5 * (maxXt + maxZt), deltaY = g5 * (maxYt + maxZt);
+    const float deltaE = 2.f * (g2 * maxXt * maxYt + deltaY * maxXt + deltaX * maxYt);
+    const float maxE = fmaxf(fabsf(U), fmaxf(fabsf(V), fabsf(W)));
+    const float deltaT = 3.f * (g3 * maxE * maxZt + deltaE * maxZt + deltaZ * maxE) * fabsf(inv);
+    if (tt <= deltaT) return false;
+  }
   t = tt;
   b1 = V * inv;
   b2 = W * inv;
@@ -83,7 +98,7 @@ __device__ __forceinline__ bool intersect_triangle(const RayPre &p, float3 org, 
 }
 
 // Sphere: nearest root in (tmin_eps, tmax), numerically stable form.
-__device__ __forceinline__ bool intersect_sphere(float3 org, float3 dir, float4 cr, float tmax, float &t) {
+TT_HD bool intersect_sphere(float3 org, float3 dir, float4 cr, float tmax, float &t) {
   float3 oc = org - mk(cr.x, cr.y, cr.z);
   float b = dot(oc, dir);
   float3 qc = oc - dir * b;
@@ -100,15 +115,15 @@ __device__ __forceinline__ bool intersect_sphere(float3 org, float3 dir, float4 
   return false;
 }
 
-__device__ __forceinline__ bool intersect_prim(const PrimRecord &pr, const RayPre &p, const Ray &r, float tmax, float &t, float &u, float &v) {
-  if (__float_as_int(pr.b.w) == 0)
+TT_HD bool intersect_prim(const PrimRecord &pr, const RayPre &p, const Ray &r, float tmax, float &t, float &u, float &v) {
+  if (as_int(pr.b.w) == 0)
     return intersect_triangle(p, r.org, xyz(pr.a), xyz(pr.b), xyz(pr.c), tmax, t, u, v);
   u = v = 0.f;
   return intersect_sphere(r.org, r.dir, make_float4(pr.a.x, pr.a.y, pr.a.z, pr.b.x), tmax, t);
 }
 
 // Slab test of both children; returns entry distances (or +inf when missed).
-__device__ __forceinline__ void box2(const BvhNode &n, const RayPre &p, float tmax, float &tl, float &tr) {
+TT_HD void box2(const BvhNode &n, const RayPre &p, float tmax, float &tl, float &tr) {
   float lx0 = n.lxy.x * p.inv.x - p.oinv.x, lx1 = n.lxy.y * p.inv.x - p.oinv.x;
   float ly0 = n.lxy.z * p.inv.y - p.oinv.y, ly1 = n.lxy.w * p.inv.y - p.oinv.y;
   float lz0 = n.lrz.x * p.inv.z - p.oinv.z, lz1 = n.lrz.y * p.inv.z - p.oinv.z;
@@ -121,13 +136,13 @@ __device__ __forceinline__ void box2(const BvhNode &n, const RayPre &p, float tm
   float lmax = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax)) * pad;
   float rmin = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), 0.f));
   float rmax = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax)) * pad;
-  tl = lmin <= lmax ? lmin : __int_as_float(0x7f800000);
-  tr = rmin <= rmax ? rmin : __int_as_float(0x7f800000);
+  tl = lmin <= lmax ? lmin : as_float(0x7f800000);
+  tr = rmin <= rmax ? rmin : as_float(0x7f800000);
 }
 
 // Generic traversal.  `stack` points at this lane's LDS column (stride kTraceBlock).
 template <bool kAnyHit>
-__device__ __forceinline__ bool traverse(const BvhNode *__restrict__ nodes, const PrimRecord *__restrict__ prims, const Ray &r,
+TT_HD bool traverse(const BvhNode *__restrict__ nodes, const PrimRecord *__restrict__ prims, const Ray &r,
                                          int *stack, HitRec &hit) {
   RayPre p = precompute(r);
   float tmax = r.tmax;

@@ -276,6 +276,7 @@ void BuildGeometry(ttgpu_handle *h, const ttgpu_scene &s, const std::vector<int6
       if (smooth)
         for (int j = 0; j < 3; ++j) normals[3 * k + j] = make_float4(0.f, 0.f, 1.f, 0.f);
     }
+    recs[k] = r;
     areas[k] = PrimAreaF(s, pid);
   }
   if (bvh.num_nodes == 0) {
@@ -733,8 +734,8 @@ int ttgpu_kernel_time(ttgpu_handle *h, const char *name, double *total_ms, int64
   return TTGPU_OK;
 }
 
-int ttgpu_trace_closest(ttgpu_handle *h, int64_t count, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
-                        float *u, float *v) {
+static int TraceClosestImpl(ttgpu_handle *h, int64_t count, const float *org, const float *dir, const float *tmax, float *t,
+                            int32_t *prim, float *u, float *v, int brute) {
   if (!h || count < 0 || (count && (!org || !dir || !tmax || !t || !prim || !u || !v))) return TTGPU_ERR_INVALID_ARGUMENT;
   if (count == 0) return TTGPU_OK;
   TTGPU_GUARD(h, {
@@ -749,7 +750,7 @@ int ttgpu_trace_closest(ttgpu_handle *h, int64_t count, const float *org, const 
     Check(hipMemcpy(d_dir, dir, 3 * n * 4, hipMemcpyHostToDevice), "H2D");
     Check(hipMemcpy(d_tmax, tmax, n * 4, hipMemcpyHostToDevice), "H2D");
     if (h->dsc.num_prims > 0) {
-      LaunchTraceRays(h->dsc, count, d_org, d_dir, d_tmax, d_t, d_prim, d_u, d_v, nullptr, h->stream);
+      LaunchTraceRays(h->dsc, count, d_org, d_dir, d_tmax, d_t, d_prim, d_u, d_v, nullptr, brute, h->stream);
       Check(hipGetLastError(), "launch");
       Check(hipStreamSynchronize(h->stream), "sync");
       Check(hipMemcpy(t, d_t, n * 4, hipMemcpyDeviceToHost), "D2H");
@@ -762,6 +763,16 @@ int ttgpu_trace_closest(ttgpu_handle *h, int64_t count, const float *org, const 
     hipFree(d_org); hipFree(d_dir); hipFree(d_tmax); hipFree(d_t); hipFree(d_u); hipFree(d_v); hipFree(d_prim);
     return TTGPU_OK;
   })
+}
+
+int ttgpu_trace_closest(ttgpu_handle *h, int64_t count, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
+                        float *u, float *v) {
+  return TraceClosestImpl(h, count, org, dir, tmax, t, prim, u, v, 0);
+}
+
+int ttgpu_trace_closest_bruteforce(ttgpu_handle *h, int64_t count, const float *org, const float *dir, const float *tmax, float *t,
+                                   int32_t *prim, float *u, float *v) {
+  return TraceClosestImpl(h, count, org, dir, tmax, t, prim, u, v, 1);
 }
 
 int ttgpu_trace_occluded(ttgpu_handle *h, int64_t count, const float *org, const float *dir, const float *tmax, int32_t *occluded) {
@@ -777,7 +788,7 @@ int ttgpu_trace_occluded(ttgpu_handle *h, int64_t count, const float *org, const
     Check(hipMemcpy(d_dir, dir, 3 * n * 4, hipMemcpyHostToDevice), "H2D");
     Check(hipMemcpy(d_tmax, tmax, n * 4, hipMemcpyHostToDevice), "H2D");
     if (h->dsc.num_prims > 0) {
-      LaunchTraceRays(h->dsc, count, d_org, d_dir, d_tmax, nullptr, nullptr, nullptr, nullptr, d_occ, h->stream);
+      LaunchTraceRays(h->dsc, count, d_org, d_dir, d_tmax, nullptr, nullptr, nullptr, nullptr, d_occ, 0, h->stream);
       Check(hipGetLastError(), "launch");
       Check(hipStreamSynchronize(h->stream), "sync");
       Check(hipMemcpy(occluded, d_occ, n * 4, hipMemcpyDeviceToHost), "D2H");

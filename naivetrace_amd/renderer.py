@@ -117,14 +117,15 @@ class GpuRenderer:
         self._check(self._lib.ttgpu_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
-    def trace_closest(self, org, direction, tmax):
+    def trace_closest(self, org, direction, tmax, bruteforce=False):
         org = np.ascontiguousarray(org, dtype=np.float32)
         direction = np.ascontiguousarray(direction, dtype=np.float32)
         tmax = np.ascontiguousarray(tmax, dtype=np.float32)
         n = len(org)
         t = np.zeros(n, np.float32); u = np.zeros(n, np.float32); v = np.zeros(n, np.float32)
         prim = np.zeros(n, np.int32)
-        self._check(self._lib.ttgpu_trace_closest(self._h, n, _f32p(org), _f32p(direction), _f32p(tmax), _f32p(t),
+        fn = self._lib.ttgpu_trace_closest_bruteforce if bruteforce else self._lib.ttgpu_trace_closest
+        self._check(fn(self._h, n, _f32p(org), _f32p(direction), _f32p(tmax), _f32p(t),
                                                   prim.ctypes.data_as(C.POINTER(C.c_int32)), _f32p(u), _f32p(v)))
         return t, prim, u, v
 

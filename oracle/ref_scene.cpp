@@ -212,6 +212,20 @@ static bool TriWatertight(V3 org, V3 dir, V3 A, V3 B, V3 C, double tnear, double
   const double inv = 1. / det;
   t = T * inv;
   if (!(t > tnear && t < tfar)) return false;
+  // Conservative lower bound on t (PBRT-v3, Triangle::Intersect): a hit closer than the rounding
+  // error of t is rejected, so rays leaving a surface do not re-hit it at t ~ 0.
+  {
+    const double g = std::numeric_limits<double>::epsilon();
+    auto gam = [g](int n) { return n * g / (1 - n * g); };
+    const double maxZt = std::max({std::abs(Az), std::abs(Bz), std::abs(Cz)});
+    const double maxXt = std::max({std::abs(Ax), std::abs(Bx), std::abs(Cx)});
+    const double maxYt = std::max({std::abs(Ay), std::abs(By), std::abs(Cy)});
+    const double deltaZ = gam(3) * maxZt, deltaX = gam(5) * (maxXt + maxZt), deltaY = gam(5) * (maxYt + maxZt);
+    const double deltaE = 2 * (gam(2) * maxXt * maxYt + deltaY * maxXt + deltaX * maxYt);
+    const double maxE = std::max({std::abs(U), std::abs(V), std::abs(W)});
+    const double deltaT = 3 * (gam(3) * maxE * maxZt + deltaE * maxZt + deltaZ * maxE) * std::abs(inv);
+    if (t <= deltaT) return false;
+  }
   // Barycentrics relative to vertices (A,B,C): p = b0*A + b1*B + b2*C.
   b0 = U * inv;
   b1 = V * inv;

@@ -1,0 +1,71 @@
+"""The C-ABI library loads (no GPU needed), exports every entry point that
+include/ttgpu.h declares, and the ctypes mirror has the C layout."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from naivetrace_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ttgpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ttgpu_[a-z_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_all_symbols():
+    lib = C.CDLL(abi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(abi.EXPORTS) == names
+
+
+def test_abi_version_without_device():
+    lib = abi.load_library()
+    assert lib.ttgpu_abi_version() == 1
+
+
+STRUCTS = ["ttgpu_shader", "ttgpu_medium", "ttgpu_material", "ttgpu_area_emitter", "ttgpu_env_light",
+           "ttgpu_point_light", "ttgpu_camera", "ttgpu_scene", "ttgpu_params", "ttgpu_stats"]
+MIRRORS = [abi.Shader, abi.Medium, abi.Material, abi.AreaEmitter, abi.EnvLight, abi.PointLight, abi.Camera,
+           abi.SceneDesc, abi.Params, abi.Stats]
+
+
+def _c_layout():
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for s, m in zip(STRUCTS, MIRRORS):
+        lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
+        for fname, _ in m._fields_:
+            lines.append(f'printf("{s}.{fname} %zu\\n", offsetof({s}, {fname}));')
+    lines.append("return 0;}")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run(["gcc", "-std=c99", "-o", exe, src], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    return dict(l.rsplit(" ", 1) for l in out.strip().splitlines())
+
+
+def test_ctypes_mirror_matches_c_layout():
+    lay = _c_layout()
+    for s, m in zip(STRUCTS, MIRRORS):
+        assert int(lay[s]) == C.sizeof(m), s
+        for fname, _ in m._fields_:
+            assert int(lay[f"{s}.{fname}"]) == getattr(m, fname).offset, f"{s}.{fname}"
+
+
+def test_create_rejects_null_arguments():
+    lib = abi.load_library()
+    h = C.c_void_p()
+    assert lib.ttgpu_create(None, None, C.byref(h)) == 1
+    assert b"null" in lib.ttgpu_create_error()

@@ -30,6 +30,24 @@ def test_device_info(gpu_lib):
     assert "gfx950" in info
 
 
+def test_bvh_matches_gpu_bruteforce(gpu_lib):
+    """BVH traversal against testing every primitive, both on the device."""
+    s = configs.cornell_box(16, 16)
+    p = abi.default_params(16, 16)
+    rng = np.random.default_rng(4)
+    org, d = _random_rays(20000, rng, [-0.95, 0.05, -0.95], [0.95, 1.95, 0.95])
+    tmax = np.full(len(org), 1e30)
+    with GpuRenderer(s, p) as r:
+        t, prim, _, _ = r.trace_closest(org, d, tmax)
+        tb, pb, _, _ = r.trace_closest(org, d, tmax, bruteforce=True)
+    print("bvh hit rate", np.mean(prim >= 0), "brute-force hit rate", np.mean(pb >= 0), flush=True)
+    assert np.array_equal(prim >= 0, pb >= 0)
+    # Same triangle test on both sides; where a ray crosses a shared edge the two orders may keep
+    # either neighbour, whose distances differ in the last bit.
+    np.testing.assert_allclose(t, tb, rtol=3e-7, atol=0)
+    assert np.mean(prim == pb) > 0.999
+
+
 def test_closest_hit_matches_oracle(gpu_lib, oracle):
     s = configs.cornell_box(16, 16)
     p = abi.default_params(16, 16)
@@ -40,11 +58,13 @@ def test_closest_hit_matches_oracle(gpu_lib, oracle):
         t, prim, u, v = r.trace_closest(org, d, tmax)
     t_ref, prim_ref = oracle.trace_closest(s, org, d, tmax)
     hit = prim_ref >= 0
-    assert hit.mean() > 0.99  # closed room
+    assert hit.mean() > 0.8  # the box is open at the front (z = +1)
+    assert np.array_equal(prim >= 0, hit)
     same = prim == prim_ref
-    # Primitive ids agree except for rays grazing a shared edge.
-    assert same.mean() > 0.999, f"{(~same).sum()} mismatches"
-    np.testing.assert_allclose(t[same & hit], t_ref[same & hit], rtol=2e-5, atol=2e-6)
+    # Primitive ids agree except where two surfaces are hit at the same distance (box bottoms are
+    # coplanar with the floor, quads share diagonals); distances agree everywhere (FP32 vs FP64).
+    assert same.mean() > 0.97, f"{(~same).sum()} mismatches"
+    np.testing.assert_allclose(t[hit], t_ref[hit], rtol=2e-5, atol=2e-6)
     assert np.all((u >= -1e-6) & (v >= -1e-6) & (u + v <= 1 + 1e-5))
 
 

@@ -1,0 +1,351 @@
+"""The oracle pinned by golden vectors, known-answer tests, closed forms and
+distribution tests (CPU only).
+
+The oracle is the FP64 C++ restatement of the pt2 path (oracle/); these tests
+are what makes it trustworthy as the GPU's parity checker:
+* PCG32 against O'Neill's published demo output;
+* spectral tables against the fixture lifted from the reference, and the
+  reference's own Spectral.* tests (src/tests.cxx:255-322);
+* the reference's Quadratic error-bound and bisection/tower-sampling KATs
+  (src/tests.cxx:94-122, 908-935);
+* every shader and phase-function sampler against its own pdf with the
+  reference's chi-square + per-bin sigma + albedo-integral methodology
+  (src/tests_sampling.cxx:540-770), plus closed forms (Lambert, normalisation);
+* BVH closest hit against brute force;
+* renders against closed forms: furnace box L/(1-rho), plane under a dome
+  rho*L, and the documented pt2 MIS quirk (biased high in REFERENCE mode).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from naivetrace_amd import abi, configs
+from naivetrace_amd.scene import Scene, flat, spectrum_to_rgb
+import stats_util as S
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---- RNG -----------------------------------------------------------------------
+def test_pcg32_known_answer(oracle):
+    # pcg32_srandom(42, 54) then 6 draws: published output of the PCG reference demo.
+    out = oracle.pcg32(42, 54, 6)
+    assert [int(x) for x in out] == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+
+
+def test_pcg32_double_uses_32_bits(oracle):
+    u = oracle.pcg32(7, 3, 1000).astype(np.float64)
+    d = oracle.pcg32_double(7, 3, 1000)
+    np.testing.assert_array_equal(d, u / 2 ** 32)
+    assert d.min() >= 0 and d.max() < 1
+
+
+# ---- spectral ------------------------------------------------------------------
+def test_spectral_tables_match_fixture(oracle):
+    fx = json.load(open(os.path.join(GOLDEN, "spectral_tables.json")))
+    c, smits, to_rgb, sun = oracle.tables()
+    np.testing.assert_array_equal(c, fx["bin_centers"])
+    np.testing.assert_array_equal(smits, fx["smits"])
+    np.testing.assert_array_equal(to_rgb, fx["spec_to_rgb"])
+    np.testing.assert_array_equal(sun, fx["sun"])
+    assert abs(sum(fx["sun"]) - 1.0) < 1e-12
+    np.testing.assert_allclose(c, 375 + 10 * np.arange(36))
+
+
+TRIALS = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1), (1, .5, .3), (.5, .8, .9)]
+
+
+@pytest.mark.parametrize("rgb", TRIALS)
+def test_rgb_spectrum_round_trip(oracle, rgb):
+    # Spectral.RGBConversion (src/tests.cxx:255-284)
+    s = oracle.rgb_to_spectrum(rgb)
+    assert np.all(s >= -1e-9) and np.all(s <= 1.01)
+    back = oracle.spectrum_to_rgb(s)
+    np.testing.assert_allclose(back, rgb, atol=1.3e-2)
+    # The package's own conversion (used to build scenes) is the same function.
+    from naivetrace_amd.scene import rgb_to_spectrum
+    np.testing.assert_allclose(rgb_to_spectrum(rgb), s, atol=1e-12)
+
+
+def test_rgb_conversion_linearity(oracle):
+    # Spectral.RGBConversionLinearity (src/tests.cxx:287-305)
+    e = np.eye(36)
+    for a, b in [(0, 2), (0, 35), (2, 35)]:
+        ab = oracle.spectrum_to_rgb(e[a] + e[b])
+        np.testing.assert_allclose(ab, oracle.spectrum_to_rgb(e[a]) + oracle.spectrum_to_rgb(e[b]), atol=1e-3)
+
+
+def test_rgb_conversion_selection():
+    # Spectral.RGBConversionSelection (src/tests.cxx:307-322) with 4-bin selections covering all 36 bins.
+    rgb = np.array([0.8, 0.2, 0.6])
+    from naivetrace_amd.scene import rgb_to_spectrum, _TO_RGB
+    spec = rgb_to_spectrum(rgb)
+    acc = np.zeros(3)
+    for start in range(0, 36, 4):
+        idx = np.arange(start, start + 4)
+        acc += _TO_RGB[:, idx] @ spec[idx]
+    np.testing.assert_allclose(acc, rgb, atol=1e-2)
+
+
+# ---- numerics KATs ---------------------------------------------------------------
+def test_quadratic_error_bounds(oracle):
+    # TestMath.Quadratic (src/tests.cxx:94-122): golden values.
+    ok, (t0, t1, e0, e1) = oracle.quadratic_err(1, 100, 1, 0, 0, 0)
+    assert ok
+    assert abs(t0 + 99.99) < 1e-4 and abs(t1 + 0.010001) < 1e-7
+    assert abs(e0 - 3.57604003511369e-5) < 1e-10 and abs(e1 - 3.57675622453257e-9) < 1e-15
+    _, (_, _, e0, e1) = oracle.quadratic_err(1, 100, 1, 0.1, 0, 0)
+    assert abs(e0 - 10.0000362396240) < 1e-4 and abs(e1 - 1.03616784485894e-7) < 1e-10
+    _, (_, _, e0, e1) = oracle.quadratic_err(1, 100, 1, 0, 1, 0)
+    assert abs(e0 - 1.00013589859009) < 1e-7 and abs(e1 - 0.000100033619673923) < 1e-10
+    _, (_, _, e0, e1) = oracle.quadratic_err(1, 100, 1, 0, 0, 0.1)
+    assert abs(e0 - 0.00103596039116383) < 1e-9 and abs(e1 - 0.00100020365789533) < 1e-9
+
+
+def test_bisection_search(oracle):
+    # Bisection.Search (src/tests.cxx:908-919)
+    vals = [1., 3., 9., 10.]
+    assert [oracle.bisection(vals, r) for r in (0.5, 1.5, 9.5, 11)] == [0, 1, 3, 4]
+
+
+# ---- BVH --------------------------------------------------------------------------
+def _moller_trumbore(org, d, v0, v1, v2):
+    e1, e2 = v1 - v0, v2 - v0
+    p = np.cross(d[:, None, :], e2[None])
+    det = np.einsum("tk,rtk->rt", e1, p)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1 / det
+        s = org[:, None, :] - v0[None]
+        u = np.einsum("rtk,rtk->rt", s, p) * inv
+        q = np.cross(s, e1[None])
+        v = np.einsum("rk,rtk->rt", d, q) * inv
+        t = np.einsum("tk,rtk->rt", e2, q) * inv
+    ok = (u >= 0) & (v >= 0) & (u + v <= 1) & (t > 0) & np.isfinite(t)
+    t = np.where(ok, t, np.inf)
+    return t.min(axis=1), np.where(np.isfinite(t.min(axis=1)), t.argmin(axis=1), -1)
+
+
+def test_bvh_closest_hit_matches_brute_force(oracle):
+    s = configs.cornell_box(8, 8)
+    s.to_abi()
+    V, I = s._v.astype(np.float64), s._i
+    tri = V[I]
+    rng = np.random.default_rng(5)
+    org = rng.uniform([-0.9, 0.1, -0.9], [0.9, 1.9, 0.9], (3000, 3))
+    d = rng.normal(size=(3000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    t, prim = oracle.trace_closest(s, org, d, np.full(3000, 1e30))
+    tb, pb = _moller_trumbore(org, d, tri[:, 0], tri[:, 1], tri[:, 2])
+    assert np.all((prim >= 0) == (pb >= 0))
+    # Distances agree everywhere; primitive ids only where the closest surface is unique (the box
+    # bottoms are coplanar with the floor, and shared edges hit two triangles at the same t).
+    np.testing.assert_allclose(t, tb, rtol=1e-9)
+    assert np.mean(prim == pb) > 0.97
+
+
+# ---- shader samplers: chi-square against their own pdf --------------------------------
+def _shader(kind, **kw):
+    s = Scene()
+    s.add_shader(kind, **kw)
+    return s.shaders[0]
+
+
+BINS = [3, 12, 21, 30]
+NG = np.array([0., 0., 1.])
+
+
+def _wi(theta_deg, phi_deg=30.):
+    t, p = np.radians(theta_deg), np.radians(phi_deg)
+    return np.array([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)])
+
+
+SHADER_CASES = [
+    ("diffuse", dict(kind=abi.SHADER_DIFFUSE, reflectance=0.7), 40., 0.0),
+    ("beckmann_rough", dict(kind=abi.SHADER_BECKMANN_METAL, reflectance=0.9, alpha=0.4), 35., 0.0),
+    ("beckmann_smooth", dict(kind=abi.SHADER_BECKMANN_METAL, reflectance=0.9, alpha=0.15), 60., 0.0),
+    ("ggx_rough", dict(kind=abi.SHADER_GGX_METAL, reflectance=0.9, alpha=0.5), 30., 0.0),
+    ("ggx_smooth", dict(kind=abi.SHADER_GGX_METAL, reflectance=0.9, alpha=0.12), 65., 0.0),
+    ("dielectric_mollified", dict(kind=abi.SHADER_DIELECTRIC, ior=1.5), 30., 0.3),
+    ("glossy_dielectric", dict(kind=abi.SHADER_GLOSSY_DIELECTRIC, ior=1.5, alpha=0.3), 40., 0.0),
+    ("dense_dielectric_diffuse_part", dict(kind=abi.SHADER_DENSE_DIELECTRIC, reflectance=0.6, specular=0.1), 50., 0.0),
+]
+
+
+def shader_consistency(sample_fn, pdf_fn, n=400000):
+    """Returns (chi-square p-value, bins within 5 sigma fraction, delta fraction, total probability)."""
+    d, v, p = sample_fn(n)
+    is_delta = p < 0
+    dirs = d[~is_delta]
+    counts = np.bincount(S.cube_bin(dirs, 4), minlength=96).astype(np.float64)
+    probs = S.bin_probabilities(pdf_fn, 4, 32)
+    pval, chi2, dof = S.chi_square_pvalue(counts, probs, n)
+    within = S.bins_within_sigma(counts, probs, n, 5.0, prob_err=0.002)
+    return pval, within.mean(), is_delta.mean(), probs.sum()
+
+
+@pytest.mark.parametrize("name,kw,theta,rough", SHADER_CASES, ids=[c[0] for c in SHADER_CASES])
+def test_oracle_shader_sampler_consistent(oracle, name, kw, theta, rough):
+    sh = _shader(**kw)
+    wi = _wi(theta)
+    smp = lambda n: oracle.sample_shader(sh, wi, NG, NG, BINS, rough, 17, n)
+    pdf = lambda dirs: oracle.eval_shader(sh, wi, NG, NG, BINS, rough, dirs)[1]
+    pval, within, dfrac, total = shader_consistency(smp, pdf)
+    # The non-delta part of the density plus the delta mass integrates to one (CheckTotalProbability).
+    assert abs(total + dfrac - 1.0) < 0.02, (total, dfrac)
+    assert pval > 1e-3, pval
+    assert within > 0.97
+
+
+def test_oracle_lambert_closed_form(oracle):
+    sh = _shader(abi.SHADER_DIFFUSE, reflectance=0.7)
+    wi = _wi(20.)
+    d, v, p = oracle.sample_shader(sh, wi, NG, NG, BINS, 0.0, 3, 100000)
+    np.testing.assert_allclose(p, d[:, 2] / np.pi, rtol=1e-12)
+    # Spectra cross the ABI as float32: reflectance 0.7 is stored as float(0.7).
+    np.testing.assert_allclose(v, 0.7 / np.pi, rtol=1e-7)
+    # Albedo: E[f cos / p] = reflectance.
+    est = (v[:, 0] * d[:, 2] / p).mean()
+    assert abs(est - 0.7) < 1e-7
+
+
+ALBEDO_CASES = [c for c in SHADER_CASES if c[0] != "dielectric_mollified"]
+
+
+@pytest.mark.parametrize("name,kw,theta,rough", ALBEDO_CASES, ids=[c[0] for c in ALBEDO_CASES])
+def test_oracle_albedo_integral(oracle, name, kw, theta, rough):
+    """Monte Carlo albedo from the sampler equals the cubature integral of f*|cos| (CheckIntegral)."""
+    sh = _shader(**kw)
+    wi = _wi(theta)
+    n = 200000
+    d, v, p = oracle.sample_shader(sh, wi, NG, NG, BINS, rough, 23, n)
+    nd = p > 0
+    w = np.where(nd, v[:, 0] * np.abs(d[:, 2]) / np.where(nd, p, 1), 0.0)
+    est, se = w.mean(), w.std() / np.sqrt(n)
+    dirs, wts = S.cell_directions(8, 16)
+    vals, _ = oracle.eval_shader(sh, wi, NG, NG, BINS, rough, dirs.reshape(-1, 3))
+    integral = (vals[:, 0] * np.abs(dirs.reshape(-1, 3)[:, 2]) * wts.reshape(-1)).sum()
+    assert abs(est - integral) < 4 * se + 0.01 * abs(integral) + 1e-4, (est, se, integral)
+    assert est < 1.0 + 4 * se  # energy conservation
+
+
+def test_mollified_dielectric_evaluate_quirk(oracle):
+    """EvaluateBsdfMollified weights the reflection lobe by prob_reflection * F (shader.cxx:574-576)
+    while SampleBsdfMollified uses F * sphere_section_pdf (:517-521); the refraction lobe agrees.
+    Kept as in the reference; this pins the restatement to it."""
+    sh = _shader(abi.SHADER_DIELECTRIC, ior=1.5)
+    wi = _wi(30.)
+    rough = 0.3
+    d, v, p = oracle.sample_shader(sh, wi, NG, NG, BINS, rough, 29, 20000)
+    ve, pe = oracle.eval_shader(sh, wi, NG, NG, BINS, rough, d)
+    refl = d[:, 2] > 0
+    cosd = np.abs(d[:, 2])
+    # Densities agree on both lobes (the cones do not overlap at 30 degrees).
+    np.testing.assert_allclose(pe, p, rtol=1e-9)
+    np.testing.assert_allclose(ve[~refl, 0], v[~refl, 0], rtol=1e-9)
+    sp = 1.0 / (2 * np.pi * rough)
+    F = 0.04215
+    # sampled value F*sp/|cos|, evaluated value prob_refl*F/|cos| with prob_refl = 0.1 (clamped)
+    np.testing.assert_allclose(v[refl, 0] * cosd[refl] / sp, ve[refl, 0] * cosd[refl] / 0.1, rtol=1e-9)
+
+
+def test_oracle_specular_deltas(oracle):
+    """Mirror and regular dielectric produce delta samples with the reference's probabilities."""
+    mirror = _shader(abi.SHADER_MIRROR, reflectance=0.9)
+    wi = _wi(40.)
+    d, v, p = oracle.sample_shader(mirror, wi, NG, NG, BINS, 0.0, 1, 100)
+    assert np.all(p < 0)
+    np.testing.assert_allclose(d, np.tile([-wi[0], -wi[1], wi[2]], (100, 1)), atol=1e-12)
+    np.testing.assert_allclose(v[:, 0] * d[:, 2], 0.9, rtol=1e-7)
+    glass = _shader(abi.SHADER_DIELECTRIC, ior=1.5)
+    d, v, p = oracle.sample_shader(glass, wi, NG, NG, BINS, 0.0, 2, 200000)
+    assert np.all(p < 0)
+    refl = d[:, 2] > 0
+    # Fresnel at 40 deg for n=1.5 is 0.0483 -> reflection probability clamped to 0.1 (shader.cxx:436).
+    assert abs(refl.mean() - 0.1) < 0.003
+    # Weighted throughput: E[v |cos| / p] = R + (1-R) * eta^2 (radiance scaling on transmission).
+    w = v[:, 0] * np.abs(d[:, 2]) / np.abs(p)
+    R = 0.048306
+    assert abs(w.mean() - (R + (1 - R) * (1 / 1.5) ** 2)) < 0.01
+
+
+PHASE_CASES = [(abi.PHASE_UNIFORM, 0.0), (abi.PHASE_HENYEY_GREENSTEIN, 0.7), (abi.PHASE_HENYEY_GREENSTEIN, -0.3),
+               (abi.PHASE_HENYEY_GREENSTEIN, 0.0005), (abi.PHASE_RAYLEIGH, 0.0)]
+
+
+@pytest.mark.parametrize("kind,g", PHASE_CASES)
+def test_oracle_phase_sampler(oracle, kind, g):
+    # PhaseFunctions chi-square tests (src/tests_sampling.cxx:1202-1259).
+    wi = _wi(50.)
+    smp = lambda n: (lambda d, v, p: (d, v[:, None].repeat(4, 1), p))(*oracle.sample_phase(kind, g, wi, 9, n))
+    pdf = lambda dirs: np.array([oracle.eval_phase(kind, g, wi, x) for x in dirs])
+    pval, within, dfrac, total = shader_consistency(smp, pdf, n=200000)
+    assert abs(total - 1.0) < 0.01
+    assert pval > 1e-3
+    d, v, p = oracle.sample_phase(kind, g, wi, 9, 1000)
+    np.testing.assert_allclose(v, p)  # value == pdf: normalised phase functions
+
+
+# ---- renders against closed forms ------------------------------------------------------
+def furnace_scene(rho=0.5, emit=2 * np.pi):
+    s = Scene()
+    sh = s.add_shader(abi.SHADER_DIFFUSE, reflectance=flat(rho))
+    em = s.add_area_emitter(flat(emit))
+    m = s.add_material(shader=sh, emitter=em)
+    s.add_box([-1, -1, -1], [1, 1, 1], m)
+    # Box normals face outwards; the one-sided emitter must face inwards: flip winding.
+    s.indices[0] = s.indices[0][:, ::-1].copy()
+    s.set_camera([0.1, 0.05, -0.2], [0.3, 0.2, 1.0], [0, 1, 0], 60.0)
+    return s
+
+
+def _render_mean(oracle, scene, w, h, spp, depth, mis, mode):
+    p = abi.default_params(w, h, max_depth=depth, seed=5)
+    p.mis_mode = mis
+    p.sample_mode = mode
+    rgb, sq = oracle.render(scene, p, spp)
+    n = w * h * spp
+    mean = rgb.sum(axis=(0, 1)) / n
+    var = sq.sum(axis=(0, 1)) / n - mean ** 2
+    return mean, np.sqrt(var / n)
+
+
+@pytest.mark.parametrize("mode,mis", [(abi.SAMPLE_BSDF, abi.MIS_REFERENCE), (abi.SAMPLE_BOTH, abi.MIS_CONSISTENT),
+                                      (abi.SAMPLE_LIGHTS, abi.MIS_REFERENCE)])
+def test_furnace_closed_form(oracle, mode, mis):
+    """Closed emitting diffuse box: L_inf = L_e / (1 - rho) (depth 60 truncates at rho^59)."""
+    rho = 0.5
+    s = furnace_scene(rho)
+    mean, se = _render_mean(oracle, s, 4, 4, 512, 60, mis, mode)
+    # L_e = 2 pi / (2 pi) = 1 per bin.  "lights" mode never adds emission seen along a path, so the
+    # directly visible emitter term L_e is missing from its estimate (pt2.cxx:627-631).
+    total = 1.0 / (1 - rho) - (1.0 if mode == abi.SAMPLE_LIGHTS else 0.0)
+    expected = spectrum_to_rgb(flat(total))
+    assert np.all(np.abs(mean - expected) < 4 * se + 1e-3 * np.abs(expected)), (mean, se, expected)
+
+
+def test_reference_mis_quirk_overestimates(oracle):
+    """pt2's NEE weight omits the light-selection probability (pt2.cxx:489-501); with 12 emissive
+    triangles the two MIS weights sum to more than one and the furnace estimate is biased high."""
+    rho = 0.5
+    s = furnace_scene(rho)
+    mean, se = _render_mean(oracle, s, 4, 4, 512, 60, abi.MIS_REFERENCE, abi.SAMPLE_BOTH)
+    expected = spectrum_to_rgb(flat(1.0 / (1 - rho)))
+    assert np.all(mean > expected + 6 * se), (mean, se, expected)
+
+
+def test_plane_under_dome_closed_form(oracle):
+    """Diffuse plane under a uniform dome of radiance L, one bounce: outgoing radiance rho * L."""
+    s = Scene()
+    rho, L = 0.6, 1.5
+    m = s.add_material(shader=s.add_shader(abi.SHADER_DIFFUSE, reflectance=flat(rho)))
+    big = 1e4
+    s.add_quad([-big, 0, big], [big, 0, big], [big, 0, -big], [-big, 0, -big], m)
+    s.add_dome([0, 1, 0], flat(L))
+    s.set_camera([0, 1, 0], [0, 0, -0.2], [0, 0, -1], 30.0)
+    for mode, mis in [(abi.SAMPLE_BOTH, abi.MIS_REFERENCE), (abi.SAMPLE_BSDF, abi.MIS_REFERENCE), (abi.SAMPLE_LIGHTS, abi.MIS_REFERENCE)]:
+        mean, se = _render_mean(oracle, s, 4, 4, 2048, 3, mis, mode)
+        expected = spectrum_to_rgb(flat(rho * L))
+        # depth 3: hit plane (node 2), scatter to the sky (node 3): exactly one bounce.
+        assert np.all(np.abs(mean - expected) < 4 * se + 2e-3), (mode, mean, se, expected)

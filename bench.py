@@ -38,7 +38,26 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on the host (0 = skip)")
     ap.add_argument("--cpu-sample-rows", type=int, default=0, help="rows of the frame the CPU sample renders (0 = auto)")
     ap.add_argument("--wavefront", type=int, default=0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run device-to-device copies of this size (PMC byte-counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib, repeats=4):
+    """Plain device-to-device copies of a known size through the HIP runtime (one copy kernel each), so the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run can be scaled to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(repeats):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": repeats}), flush=True)
 
 
 def cpu_baseline(scene, width, height, depth, rows):
@@ -62,6 +81,9 @@ def cpu_baseline(scene, width, height, depth, rows):
 
 def main():
     args = parse()
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is designed, not built)"}))
         sys.exit(2)

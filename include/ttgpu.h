@@ -131,12 +131,16 @@ enum ttgpu_env_kind {
 
 typedef struct ttgpu_env_light {
   int32_t kind;
-  double dir[3];            /* SUN/DIRECTIONAL: direction the light travels (YAML "direction");
-                               DOME/MAP: up direction */
+  double dir[3];            /* SUN/DIRECTIONAL: direction from the scene towards the light
+                               (YAML "direction"; light travels along -dir);
+                               DOME: up direction (light arrives from the hemisphere around dir);
+                               MAP: polar axis of the lat-long map */
   double opening_angle_deg; /* SUN */
   double total_power;       /* SUN */
   float spectrum[TTGPU_NBINS]; /* DOME / DIRECTIONAL */
-  int32_t tex_width, tex_height; /* MAP: row-major RGB float texture, row 0 = top (v=0) */
+  int32_t tex_width, tex_height; /* MAP: row-major RGB float texture; row y holds the directions
+                                    looked at (from the scene towards the map) whose angle to +dir
+                                    lies in [pi*y/h, pi*(y+1)/h]: row 0 is seen looking along dir */
   const float *texture_rgb;
 } ttgpu_env_light;
 
@@ -263,7 +267,8 @@ int ttgpu_set_pass_callback(ttgpu_handle *h, ttgpu_pass_callback cb, void *user)
 int ttgpu_reset_framebuffer(ttgpu_handle *h);
 
 /* Framebuffer: linear RGB sums (double, width*height*3, row-major, row 0 =
- * first image row of the camera) and the samples per pixel accumulated.
+ * first image row of the camera) and the samples accumulated per pixel
+ * (spp: array of width*height counts, may be NULL).
  * rgb_sq (optional, may be NULL): per-pixel sums of squared per-path RGB
  * estimates, for variance estimates. */
 int ttgpu_read_framebuffer(ttgpu_handle *h, double *rgb, double *rgb_sq, uint64_t *spp);

@@ -40,7 +40,63 @@ def parse():
     ap.add_argument("--wavefront", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (PMC byte-counter calibration) and exit")
+    ap.add_argument("--summarize-pmc", default="",
+                    help="print the markdown traffic summary of a directory of rocprofv3 --pmc CSV runs and exit")
     return ap.parse_args()
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_terrain_1080p.md")
+COPY_MIB = 256
+
+
+def _pmc_means(csv_path):
+    import csv
+    import collections
+    acc = collections.defaultdict(list)
+    for row in csv.DictReader(open(csv_path)):
+        name = row["Kernel_Name"]
+        name = name.split("(")[0]
+        acc[name].append(float(row["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def summarize_pmc(d):
+    """Per-dispatch HBM traffic of each kernel from FETCH_SIZE / WRITE_SIZE runs, each counter scaled
+    by its reading for a plain device-to-device copy of COPY_MIB MiB (bytes moved / counter KiB)."""
+    lines = []
+    scale = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        copy = _pmc_means(os.path.join(d, f"copy_{ctr}", f"copy_counter_collection.csv"))
+        kib, _ = copy["__amd_rocclr_copyBuffer"]
+        scale[ctr] = (COPY_MIB << 20) / kib
+    bench = {ctr: _pmc_means(os.path.join(d, f"bench_{ctr}", f"bench_counter_collection.csv")) for ctr in ("FETCH_SIZE", "WRITE_SIZE")}
+    lines.append("| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | HBM read MB | HBM write MB | HBM total MB |")
+    lines.append("|---|---|---|---|---|---|---|")
+    machine = []
+    for k in sorted(bench["FETCH_SIZE"], key=lambda k: -bench["FETCH_SIZE"][k][0]):
+        if not k.startswith("ttgpu::"):
+            continue
+        f, n = bench["FETCH_SIZE"][k]
+        w, _ = bench["WRITE_SIZE"].get(k, (0.0, 0))
+        rb, wb = f * scale["FETCH_SIZE"], w * scale["WRITE_SIZE"]
+        lines.append(f"| {k} | {n} | {f:.1f} | {w:.1f} | {rb / 1e6:.2f} | {wb / 1e6:.2f} | {(rb + wb) / 1e6:.2f} |")
+        machine.append(f"traffic_bytes_per_dispatch {k.split('::')[-1]} {rb + wb:.0f}")
+    head = [f"Calibration: a {COPY_MIB} MiB device-to-device copy reads {(COPY_MIB << 20) / scale['FETCH_SIZE']:.1f} KiB of "
+            f"FETCH_SIZE and writes {(COPY_MIB << 20) / scale['WRITE_SIZE']:.1f} KiB of WRITE_SIZE, so bytes = "
+            f"FETCH_SIZE KiB x {scale['FETCH_SIZE']:.1f} and WRITE_SIZE KiB x {scale['WRITE_SIZE']:.1f}.", ""]
+    return "\n".join(head + lines + [""] + machine)
+
+
+def load_traffic(kernel):
+    """Calibrated HBM bytes per dispatch of `kernel` from the committed PMC summary, or None."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1] == kernel:
+                return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(mib, repeats=4):
@@ -83,6 +139,9 @@ def main():
     args = parse()
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib)
+        return
+    if args.summarize_pmc:
+        print(summarize_pmc(args.summarize_pmc))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is designed, not built)"}))
@@ -132,7 +191,7 @@ def main():
         alg_bytes = SHADE_BYTES_PER_PATH * rays_closest / max(launches, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": f"k_{dom}",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(f"k_{dom}"), "kernel": f"k_{dom}",
                 "avg_launch_ms": round(avg_ms, 4),
                 "bytes_per_unit": {"trace": TRACE_BYTES_PER_RAY, "shade": SHADE_BYTES_PER_PATH, "shadow": SHADOW_BYTES_PER_RAY}[dom]}
     ktimes = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in times.items()}

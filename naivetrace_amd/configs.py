@@ -235,3 +235,70 @@ def atmosphere(width=1024, height=512, sun_elevation_deg=5.0, mie_g=0.76):
     s.set_camera([0.0, 0.002, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], 180.0, kind=abi.CAMERA_FISHEYE)
     s.width, s.height = width, height
     return s
+
+
+# ---------------------------------------------------------------------------
+# Media scenes with closed-form answers (transmittance through a pure absorber)
+# ---------------------------------------------------------------------------
+def absorber_slab(width=16, height=16, seed=3, thickness=0.5, dome=1.0, fov_deg=2.0):
+    """Camera looking along -z through an invisible slab |z| < thickness/2 filled with a purely
+    absorbing chromatic medium, at a uniform dome light behind it.  Every camera path escapes
+    or is absorbed, so each pixel's expected radiance is sum_b M[:, b] * dome * exp(-sigma_a[b] * d)
+    with d the chord of the pixel's ray (d = thickness / |dir_z|)."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    sig_a = rng.uniform(0.2, 2.0, NBINS)
+    med = s.add_homogeneous_medium(np.zeros(NBINS), sig_a)
+    boundary = s.add_material(medium=med)
+    h = 0.5 * thickness
+    s.add_box([-50.0, -50.0, -h], [50.0, 50.0, h], boundary)
+    s.add_dome([0.0, 0.0, -1.0], flat(dome))
+    s.set_camera([0.0, 0.0, 10.0], [0.0, 0.0, 0.0], [0.0, 1.0, 0.0], fov_deg)
+    s.width, s.height = width, height
+    s.absorber_sigma_a = sig_a
+    return s
+
+
+def point_lit_plane_behind_absorber(width=16, height=16, seed=4, thickness=0.4, power=50.0, albedo=0.5):
+    """Lambertian plane z = 0 lit by a point light at (0, 0, 2) through a purely absorbing slab
+    0.8 < z < 0.8 + thickness (an invisible volume boundary).  The camera sits below the slab at
+    z = 0.5 looking straight down, so its rays never cross the slab; only the NEE shadow ray does.
+    Direct light at the plane's centre:  albedo/pi * power/(4 pi r^2) * exp(-sigma_a * thickness)."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    sig_a = rng.uniform(0.5, 3.0, NBINS)
+    med = s.add_homogeneous_medium(np.zeros(NBINS), sig_a)
+    boundary = s.add_material(medium=med)
+    s.add_box([-20.0, -20.0, 0.8], [20.0, 20.0, 0.8 + thickness], boundary)
+    diffuse = s.add_material(shader=s.add_shader(abi.SHADER_DIFFUSE, reflectance=flat(albedo)))
+    s.add_quad([-20, -20, 0], [20, -20, 0], [20, 20, 0], [-20, 20, 0], diffuse)
+    s.add_point_light([0.0, 0.0, 2.0], flat(power))
+    s.set_camera([0.0, 0.0, 0.5], [0.0, 0.0, 0.0], [0.0, 1.0, 0.0], 1.0)
+    s.width, s.height = width, height
+    s.absorber_sigma_a = sig_a
+    return s
+
+
+def fog_box(width=32, height=32, seed=5):
+    """Camera inside an invisible box of scattering fog (HG g=0.5) over a diffuse floor, with a
+    glass sphere holding a denser, higher-priority medium, an area light and a point light.
+    Exercises the medium tracker (initial medium at the camera, priorities, nested media),
+    volume boundaries, phase-function NEE and scattering."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    fog = s.add_homogeneous_medium(rng.uniform(0.2, 0.5, NBINS), rng.uniform(0.02, 0.1, NBINS),
+                                   phase=abi.PHASE_HENYEY_GREENSTEIN, g=0.5, priority=0)
+    ink = s.add_homogeneous_medium(rng.uniform(1.0, 3.0, NBINS), rng.uniform(0.1, 0.5, NBINS),
+                                   phase=abi.PHASE_UNIFORM, priority=1)
+    s.add_box([-3.0, -0.5, -3.0], [3.0, 3.0, 3.0], s.add_material(medium=fog))
+    floor = s.add_material(shader=s.add_shader(abi.SHADER_DIFFUSE, reflectance=flat(0.7)))
+    s.add_quad([-2.5, 0, 2.5], [2.5, 0, 2.5], [2.5, 0, -2.5], [-2.5, 0, -2.5], floor)
+    glass = s.add_shader(abi.SHADER_DIELECTRIC, ior=1.33)
+    s.add_sphere([0.0, 0.8, 0.0], 0.6, s.add_material(shader=glass, medium=ink))
+    em = s.add_area_emitter(flat(30.0))
+    m_light = s.add_material(shader=s.add_shader(abi.SHADER_DIFFUSE, reflectance=flat(0.0)), emitter=em)
+    s.add_quad([-0.5, 2.7, 0.5], [0.5, 2.7, 0.5], [0.5, 2.7, -0.5], [-0.5, 2.7, -0.5], m_light)
+    s.add_point_light([1.2, 2.0, 0.8], flat(40.0))
+    s.set_camera([0.0, 1.2, 2.5], [0.0, 0.8, 0.0], [0.0, 1.0, 0.0], 50.0)
+    s.width, s.height = width, height
+    return s

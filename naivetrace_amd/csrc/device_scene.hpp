@@ -134,14 +134,14 @@ struct PathQueues {
   uint2 *tracker;       // packed medium tracker: current | 4 slots
   // shadow rays
   float4 *sh_org;       // xyz, tmax
-  float4 *sh_dir;       // xyz
-  float4 *sh_contrib;   // rgb if unoccluded
+  float4 *sh_dir;       // xyz; media scenes: w = light pdf, negative for a delta light
+  float4 *sh_contrib;   // rgb if unoccluded (media scenes: per-wavelength value before transmission and MIS)
   // queues
   uint32_t *queue_a, *queue_b, *shadow_queue;
   uint32_t *counters;   // [0] queue_a count, [1] queue_b count, [2] shadow count, [3..] stats
-  // media (only when the scene has media): N x N matrices per path, row-major
-  float *w_track;       // 16 per path
-  float *w_track_null;  // 16 per path
+  // media (only when the scene has media, else null)
+  float4 *wt_r, *wt_c;  // path-space tracking weights r (x) c of the null-path formulation
+  uint4 *sh_aux;        // shadow ray: medium tracker (x, y), BSDF/phase pdf bits (z)
   int capacity;
 };
 

@@ -9,6 +9,7 @@
 #include "device_math.hpp"
 #include "shading.hpp"
 #include "trace.hpp"
+#include "media.hpp"
 #include "kernels.hpp"
 
 namespace ttgpu {
@@ -94,6 +95,10 @@ __global__ void __launch_bounds__(256) k_raygen(DeviceScene sc, PathQueues q, ui
     q.last_pdf[i] = 0.f;
     q.roughness[i] = 0.f;
     q.tracker[i] = make_uint2((uint32_t)sc.cam_tracker, (uint32_t)(sc.cam_tracker >> 32));
+    if (q.wt_r) {
+      q.wt_r[i] = f4(1.f);
+      q.wt_c[i] = f4(1.f);
+    }
     q.queue_a[i] = i;
   }
 }
@@ -266,10 +271,14 @@ __device__ __forceinline__ SurfHit make_hit(const DeviceScene &sc, int prim, flo
                   fabsf(v0.z * b0) + fabsf(v1.z * u) + fabsf(v2.z * v)) * g7;
     h.offset = e.x * fabsf(h.ng.x) + e.y * fabsf(h.ng.y) + e.z * fabsf(h.ng.z);
   } else {
-    const float3 c = xyz(pr.a);
-    const float3 p = org + dir * t;
-    h.ng = normalize(p - c);
-    h.pos = c + h.ng * pr.b.x;
+    // FP64 reprojection onto the sphere (Spheres::GetLocalGeometry, src/primitive.cxx:214-223):
+    // the position then carries only its final rounding to FP32, which the offset covers.
+    const double px = (double)org.x + (double)dir.x * t - pr.a.x, py = (double)org.y + (double)dir.y * t - pr.a.y,
+                 pz = (double)org.z + (double)dir.z * t - pr.a.z;
+    const double inv = 1.0 / sqrt(px * px + py * py + pz * pz);
+    const double nx = px * inv, ny = py * inv, nz = pz * inv, r = pr.b.x;
+    h.ng = mk((float)nx, (float)ny, (float)nz);
+    h.pos = mk((float)(pr.a.x + nx * r), (float)(pr.a.y + ny * r), (float)(pr.a.z + nz * r));
     h.ns = h.ng;
     const float g5 = 5.f * kFltEps * 0.5f / (1.f - 5.f * kFltEps * 0.5f);
     h.offset = (fabsf(h.pos.x) * fabsf(h.ng.x) + fabsf(h.pos.y) * fabsf(h.ng.y) + fabsf(h.pos.z) * fabsf(h.ng.z)) * g5;
@@ -294,6 +303,92 @@ __device__ __forceinline__ void finalize_path(const PathQueues &q, double *fb, d
 }
 
 __device__ __forceinline__ float4 add_rgb(float4 rad, float3 c) { return make_float4(rad.x + c.x, rad.y + c.y, rad.z + c.z, rad.w); }
+
+// ---- light selection and sampling for NEE (pt2.cxx:479-540, lightpicker_trivial.cxx:35-51) ----
+struct LightSample {
+  float3 dir;     // from the shading point towards the light
+  float len;      // distance to the light (environment: 10 x scene diagonal)
+  float4 weight;  // light value / (pdf x selection probability), incl. 1/len^2 for positional lights
+  float pdf;      // pdf handed to MisWeight (solid angle; selection probability only in consistent mode)
+  bool delta, valid;
+};
+
+// `fallback_dir` is used when the sampled light point coincides with `here` (SegmentToPoint).
+__device__ __forceinline__ LightSample sample_light(const DeviceScene &sc, float3 here, float3 fallback_dir, uint32_t bins, Pcg &rng) {
+  LightSample ls;
+  ls.dir = mk(0.f, 0.f, 1.f);
+  ls.len = 0.f;
+  ls.weight = f4(0.f);
+  ls.pdf = 0.f;
+  ls.delta = false;
+  const float u_type = rng.uniform();
+  const float u_idx = rng.uniform();
+  int kind = 3;
+  {
+    float r = u_type;
+    while (kind > 0 && r >= sc.type_probs[kind]) { r -= sc.type_probs[kind]; --kind; }
+  }
+  // A type whose light count is zero can only be reached through float rounding of the
+  // selection probabilities; it contributes nothing.
+  ls.valid = (kind == 0 && sc.num_envs > 0) || (kind == 1 && sc.num_area_lights > 0) || (kind == 2 && sc.num_points > 0);
+  if (!ls.valid) return ls;
+  if (kind == 1) {
+    const uint32_t li = min((uint32_t)(u_idx * sc.num_area_lights), (uint32_t)sc.num_area_lights - 1u);
+    const int lp = sc.area_lights[li];
+    const PrimRecord pr = sc.prims[lp];
+    const float r0 = rng.uniform(), r1 = rng.uniform();
+    float3 lpos, lng, lns;
+    if (as_int(pr.b.w) == 0) {
+      const float3 v0 = xyz(pr.a), v1 = xyz(pr.b), v2 = xyz(pr.c);
+      const float s = sqrtf(r0);
+      const float b0 = 1.f - s, b1 = (1.f - r1) * s, b2 = r1 * s;
+      lpos = v0 * b0 + v1 * b1 + v2 * b2;
+      lng = normalize(cross(v1 - v0, v2 - v0));
+      if (sc.prim_normals) {
+        const float4 *nn = sc.prim_normals + 3 * (size_t)lp;
+        lns = normalize(xyz(nn[0]) * b0 + xyz(nn[1]) * b1 + xyz(nn[2]) * b2);
+      } else {
+        lns = lng;
+      }
+    } else {
+      lng = to_uniform_sphere(r0, r1);
+      lns = lng;
+      lpos = xyz(pr.a) + lng * pr.b.x;
+    }
+    float3 d = lpos - here;
+    ls.len = length(d);
+    ls.dir = ls.len > 0.f ? d * (1.f / ls.len) : fallback_dir;
+    if (!(ls.len > 0.f)) ls.len = 2.220446049250313e-16f;
+    const DevEmitter &em = sc.emitters[sc.materials[sc.prim_material[lp]].emitter];
+    float4 Le = (em.kind == TTGPU_AREA_UNIFORM && dot(lng, -ls.dir) > 0.f) ? take(em.radiance, bins) : f4(0.f);
+    const float area_pdf = 1.f / sc.prim_area[lp];
+    const float dfac = fabsf(dot(lns, ls.dir));
+    ls.weight = (dfac / (area_pdf * sc.area_pick_prob * ls.len * ls.len)) * Le;
+    ls.pdf = area_to_solid_angle(ls.len, ls.dir, lng) * area_pdf;
+    if (sc.mis_mode == TTGPU_MIS_CONSISTENT) ls.pdf *= sc.area_pick_prob;
+  } else if (kind == 0) {
+    float3 dout;
+    float4 val;
+    float p;
+    env_sample(sc, bins, rng, dout, val, p, ls.delta);
+    ls.dir = -dout;
+    ls.len = 10.f * sc.bb_diag;
+    ls.weight = (1.f / (p * sc.type_probs[0])) * val;
+    ls.pdf = p * (sc.mis_mode == TTGPU_MIS_CONSISTENT ? sc.type_probs[0] : 1.f);
+  } else {
+    const uint32_t li = min((uint32_t)(u_idx * sc.num_points), (uint32_t)sc.num_points - 1u);
+    const DevPointLight &pl = sc.points[li];
+    float3 d = pl.pos - here;
+    ls.len = length(d);
+    ls.dir = ls.len > 0.f ? d * (1.f / ls.len) : fallback_dir;
+    if (!(ls.len > 0.f)) ls.len = 2.220446049250313e-16f;
+    const float prob = sc.type_probs[2] / sc.num_points;
+    ls.weight = (1.f / (prob * ls.len * ls.len)) * take(pl.intensity, bins);
+    ls.pdf = 1.f;
+    ls.delta = true;
+  }
+  return ls;
+}
 
 // ---- shade: emission, NEE, scattering (pt2.cxx:410-443, 479-556, 605-684) --------
 __global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
@@ -355,91 +450,19 @@ __global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, con
         const float3 wi = -dir;
         // AddDirectLighting (pt2.cxx:479-556), vacuum transmission.
         if (use_nee && (sc.num_area_lights + sc.num_envs + sc.num_points) > 0) {
-          const float u_type = rng.uniform();
-          const float u_idx = rng.uniform();
-          int kind = 3;
-          {
-            float r = u_type;
-            while (kind > 0 && r >= sc.type_probs[kind]) { r -= sc.type_probs[kind]; --kind; }
-          }
-          float3 ldir = mk(0.f, 0.f, 1.f);
-          float len = 0.f;
-          float4 lw = f4(0.f);
-          float lpdf = 0.f;
-          bool ldelta = false;
-          // A type whose light count is zero can only be reached through float rounding of the
-          // selection probabilities; it contributes nothing.
-          bool valid = (kind == 0 && sc.num_envs > 0) || (kind == 1 && sc.num_area_lights > 0) || (kind == 2 && sc.num_points > 0);
-          if (!valid) {
-          } else if (kind == 1) {
-            const uint32_t li = min((uint32_t)(u_idx * sc.num_area_lights), (uint32_t)sc.num_area_lights - 1u);
-            const int lp = sc.area_lights[li];
-            const PrimRecord pr = sc.prims[lp];
-            const float r0 = rng.uniform(), r1 = rng.uniform();
-            float3 lpos, lng, lns;
-            if (as_int(pr.b.w) == 0) {
-              const float3 v0 = xyz(pr.a), v1 = xyz(pr.b), v2 = xyz(pr.c);
-              const float s = sqrtf(r0);
-              const float b0 = 1.f - s, b1 = (1.f - r1) * s, b2 = r1 * s;
-              lpos = v0 * b0 + v1 * b1 + v2 * b2;
-              lng = normalize(cross(v1 - v0, v2 - v0));
-              if (sc.prim_normals) {
-                const float4 *nn = sc.prim_normals + 3 * (size_t)lp;
-                lns = normalize(xyz(nn[0]) * b0 + xyz(nn[1]) * b1 + xyz(nn[2]) * b2);
-              } else {
-                lns = lng;
-              }
-            } else {
-              lng = to_uniform_sphere(r0, r1);
-              lns = lng;
-              lpos = xyz(pr.a) + lng * pr.b.x;
-            }
-            float3 d = lpos - h.pos;
-            len = length(d);
-            ldir = len > 0.f ? d * (1.f / len) : h.n;
-            if (!(len > 0.f)) len = 2.220446049250313e-16f;
-            const DevEmitter &em = sc.emitters[sc.materials[sc.prim_material[lp]].emitter];
-            float4 Le = (em.kind == TTGPU_AREA_UNIFORM && dot(lng, -ldir) > 0.f) ? take(em.radiance, bins) : f4(0.f);
-            const float area_pdf = 1.f / sc.prim_area[lp];
-            const float dfac = fabsf(dot(lns, ldir));
-            lw = (dfac / (area_pdf * sc.area_pick_prob * len * len)) * Le;
-            lpdf = area_to_solid_angle(len, ldir, lng) * area_pdf;
-            if (sc.mis_mode == TTGPU_MIS_CONSISTENT) lpdf *= sc.area_pick_prob;
-          } else if (kind == 0) {
-            float3 dout;
-            float4 val;
-            float p;
-            env_sample(sc, bins, rng, dout, val, p, ldelta);
-            ldir = -dout;
-            len = 10.f * sc.bb_diag;
-            lw = (1.f / (p * sc.type_probs[0])) * val;
-            lpdf = p * (sc.mis_mode == TTGPU_MIS_CONSISTENT ? sc.type_probs[0] : 1.f);
-          } else if (kind == 2) {
-            const uint32_t li = min((uint32_t)(u_idx * sc.num_points), (uint32_t)sc.num_points - 1u);
-            const DevPointLight &pl = sc.points[li];
-            float3 d = pl.pos - h.pos;
-            len = length(d);
-            ldir = len > 0.f ? d * (1.f / len) : h.n;
-            if (!(len > 0.f)) len = 2.220446049250313e-16f;
-            const float prob = sc.type_probs[2] / sc.num_points;
-            lw = (1.f / (prob * len * len)) * take(pl.intensity, bins);
-            lpdf = 1.f;
-            ldelta = true;
-          } else {
-            valid = false;
-          }
-          if (valid) {
+          const LightSample ls = sample_light(sc, h.pos, h.n, bins, rng);
+          if (ls.valid) {
             float bsdf_pdf;
-            float4 f = eval_bsdf(shd, h, wi, bins, rough, lambda0, ldir, &bsdf_pdf);
-            const float dfac = fabsf(dot(h.ns, ldir));
+            float4 f = eval_bsdf(shd, h, wi, bins, rough, lambda0, ls.dir, &bsdf_pdf);
+            const float dfac = fabsf(dot(h.ns, ls.dir));
             float w = 1.f;
-            if (!ldelta && mis) w = lpdf / (lpdf + bsdf_pdf);
-            float4 c = (w * dfac) * (thru * f * lw);
+            if (!ls.delta && mis) w = ls.pdf / (ls.pdf + bsdf_pdf);
+            float4 c = (w * dfac) * (thru * f * ls.weight);
             float3 crgb = spectral_to_rgb(c, bins);
             if (crgb.x != 0.f || crgb.y != 0.f || crgb.z != 0.f) {
-              const float3 o = offset_origin(h, ldir);
-              q.sh_org[i] = make_float4(o.x, o.y, o.z, len * 0.9999f);
-              q.sh_dir[i] = make_float4(ldir.x, ldir.y, ldir.z, 0.f);
+              const float3 o = offset_origin(h, ls.dir);
+              q.sh_org[i] = make_float4(o.x, o.y, o.z, ls.len * 0.9999f);
+              q.sh_dir[i] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.f);
               q.sh_contrib[i] = make_float4(crgb.x, crgb.y, crgb.z, 0.f);
               push_shadow = true;
             }
@@ -495,6 +518,207 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DeviceScene sc, PathQueu
     bool occ = traverse<true>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
     float4 rad = q.rad[i];
     if (!occ) rad = add_rgb(rad, xyz(q.sh_contrib[i]));
+    if (q.flags[i] & kFlagFinalizeAfterShadow) finalize_path(q, fb, fb_sq, i, rad);
+    else q.rad[i] = rad;
+  }
+  if (gtid() == 0) atomicAdd(stats + kStatShadow, count);
+}
+
+// ---- media: shade with null-path tracking (pt2.cxx:410-443 with media_integrator.cxx) ------
+// One Step of the reference's path loop for scenes with participating media: tracking along the
+// traced segment (volume boundaries, delta tracking), then emission, NEE and scattering at the
+// surface hit or at the real collision in a medium.
+__global__ void __launch_bounds__(kTraceBlock) k_shade_media(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
+                                                             const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
+                                                             uint32_t *count_out, double *fb, double *fb_sq) {
+  const uint32_t count = *count_in;
+  const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
+  const bool use_emission = sc.sample_mode != TTGPU_SAMPLE_LIGHTS;
+  const bool mis = use_nee && use_emission;
+  const uint32_t n_iter = (count + grid_stride() - 1) / grid_stride();
+  for (uint32_t it = 0; it < n_iter; ++it) {
+    const uint32_t k = it * grid_stride() + gtid();
+    const bool active = k < count;
+    bool push = false, push_shadow = false;
+    uint32_t i = 0;
+    if (active) {
+      i = queue_in[k];
+      const float4 hitv = q.hit[i];
+      const float3 dir = xyz(q.dir[i]);
+      const float4 org4 = q.org[i];
+      const float3 org = xyz(org4);
+      const float lambda0 = org4.w;
+      float4 thru = q.thru[i];
+      float4 rad = q.rad[i];
+      const uint32_t bins = q.bins[i];
+      uint32_t flags = q.flags[i];
+      const float last_pdf = q.last_pdf[i];
+      const bool mis_last = (flags & kFlagHasLastPdf) && !(flags & kFlagLastPdfDelta) && mis;
+      Pcg rng = load_rng(q, i);
+      const uint2 tp = q.tracker[i];
+      Tracker tr{tp.x, tp.y};
+      NullMat wt{q.wt_r[i], q.wt_c[i]};
+      const int prim = as_int(hitv.w);
+      // nullpath::Tracking(scene, ...) up to just before the surface hit.
+      const double tfar = prim >= 0 ? (double)hitv.x : kLargeNumber;
+      NullMat nulls = nm_ones(), track = nm_ones();
+      double tcoll = 0.;
+      float4 vss = f4(0.f);
+      int vmed = kVacuum;
+      const bool vol = track_scene(sc, org, dir, tfar * 0.9999, bins, rng, tr, nulls, track, tcoll, vss, vmed);
+      NullMat wtn = wt;
+      nm_acc(wtn, nulls);
+      nm_acc(wt, track);
+      if (!vol && prim < 0) {
+        // AddEnvEmission (pt2.cxx:661-684)
+        if (use_emission && sc.num_envs > 0) {
+          float penv;
+          const float4 L = env_eval(sc, -dir, bins, &penv);
+          const float4 fac = mis_factor(mis_last, last_pdf, penv * sc.type_probs[0], wt, wtn);
+          rad = add_rgb(rad, spectral_to_rgb(fac * (L * thru), bins));
+        }
+        finalize_path(q, fb, fb_sq, i, rad);
+      } else {
+        SurfHit h;
+        D3 vpos{0., 0., 0.};
+        float3 here;
+        float rough = q.roughness[i];
+        int shader = -1;
+        if (!vol) {
+          h = make_hit(sc, prim, hitv.y, hitv.z, hitv.x, org, dir);
+          const DevMaterial mat = sc.materials[h.material];
+          shader = mat.shader;
+          here = h.pos;
+          // MaybeAddEmission (pt2.cxx:627-658)
+          if (use_emission && mat.emitter >= 0) {
+            const DevEmitter &em = sc.emitters[mat.emitter];
+            if (em.kind == TTGPU_AREA_UNIFORM && dot(h.ng, -dir) > 0.f) {
+              const float4 L = take(em.radiance, bins);
+              const float p_light = sc.area_pick_prob / sc.prim_area[prim] * area_to_solid_angle(hitv.x, dir, h.n);
+              const float4 fac = mis_factor(mis_last, last_pdf, p_light, wt, wtn);
+              rad = add_rgb(rad, spectral_to_rgb(fac * (L * thru), bins));
+            }
+          }
+        } else {
+          vpos = d3_along(d3(org), dir, tcoll);
+          here = f3(vpos);
+        }
+        const float3 wi = -dir;
+        // AddDirectLighting (pt2.cxx:479-556); the transmission estimate is made by k_shadow_media.
+        if (use_nee && (sc.num_area_lights + sc.num_envs + sc.num_points) > 0) {
+          const LightSample ls = sample_light(sc, here, vol ? mk(1.f, 0.f, 0.f) : h.n, bins, rng);
+          if (ls.valid) {
+            float bsdf_pdf;
+            float4 f;
+            Tracker ntr = tr;
+            float3 o;
+            if (!vol) {
+              f = eval_bsdf(sc.shaders[shader], h, wi, bins, rough, lambda0, ls.dir, &bsdf_pdf) * fabsf(dot(h.ns, ls.dir));
+              maybe_going_through(sc, ntr, ls.dir, h);
+              o = offset_origin(h, ls.dir);
+            } else {
+              f = medium_phase_eval(sc, vmed, wi, vpos, bins, ls.dir, &bsdf_pdf);
+              float4 ss, st;
+              medium_coeffs(sc, vmed, vpos, bins, ss, st);
+              f = f * make_float4(ss.x / (st.x + kDblEps), ss.y / (st.y + kDblEps), ss.z / (st.z + kDblEps), ss.w / (st.w + kDblEps));
+              o = here;
+            }
+            const float4 c = thru * f * ls.weight;
+            if (!all_zero(c)) {
+              q.sh_org[i] = make_float4(o.x, o.y, o.z, ls.len * 0.9999f);
+              q.sh_dir[i] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.delta ? -ls.pdf : ls.pdf);
+              q.sh_contrib[i] = c;
+              q.sh_aux[i] = make_uint4(ntr.cur, ntr.slots, (uint32_t)as_int(bsdf_pdf), 0u);
+              push_shadow = true;
+            }
+          }
+        }
+        // MaybeScatter (pt2.cxx:605-622) and PrepareStateForAfterScattering (:583-599)
+        Scatter s;
+        if (!vol) {
+          const DevShader &shd = sc.shaders[shader];
+          rough = fmaxf(my_roughness(shd), rough);
+          s = sample_bsdf(shd, h, wi, bins, rough, lambda0, rng);
+        } else {
+          s = medium_phase_sample(sc, vmed, wi, vpos, bins, rng);
+          float4 ss, st;
+          medium_coeffs(sc, vmed, vpos, bins, ss, st);
+          s.value = s.value * make_float4(ss.x / (st.x + kDblEps), ss.y / (st.y + kDblEps), ss.z / (st.z + kDblEps), ss.w / (st.w + kDblEps));
+        }
+        const uint32_t node = (flags & kFlagNodeMask) + 1u;
+        bool alive = node <= (uint32_t)sc.max_depth;
+        if (alive) {
+          const float d = vol ? 1.f : fabsf(dot(h.ns, s.dir));
+          thru = (d / s.pdf) * (thru * s.value);
+          alive = !all_zero(thru) && isfinite(thru.x + thru.y + thru.z + thru.w);
+        }
+        if (alive) {
+          float3 o;
+          if (!vol) {
+            o = offset_origin(h, s.dir);
+            maybe_going_through(sc, tr, s.dir, h);
+          } else {
+            o = here;
+          }
+          q.org[i] = make_float4(o.x, o.y, o.z, lambda0);
+          q.dir[i] = make_float4(s.dir.x, s.dir.y, s.dir.z, 0.f);
+          q.thru[i] = thru;
+          q.last_pdf[i] = s.pdf;
+          q.roughness[i] = rough;
+          flags = (flags & ~(kFlagNodeMask | kFlagLastPdfDelta)) | node | kFlagHasLastPdf | (s.delta ? kFlagLastPdfDelta : 0u);
+          push = true;
+        } else if (push_shadow) {
+          flags |= kFlagFinalizeAfterShadow;
+        } else {
+          finalize_path(q, fb, fb_sq, i, rad);
+        }
+        q.flags[i] = flags;
+        q.rad[i] = rad;
+        q.tracker[i] = make_uint2(tr.cur, tr.slots);
+        q.wt_r[i] = wt.r;
+        q.wt_c[i] = wt.c;
+        store_rng(q, i, rng);
+      }
+    }
+    const uint32_t slot = wave_append(count_out, push);
+    if (push) queue_out[slot] = i;
+    const uint32_t sslot = wave_append(q.counters + kCountShadow, push_shadow);
+    if (push_shadow) q.shadow_queue[sslot] = i;
+  }
+}
+
+// ---- media: shadow rays with occlusion + null-path transmission + MIS -----------------------
+__global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DeviceScene sc, PathQueues q, double *fb, double *fb_sq, uint32_t *stats) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  const uint32_t count = q.counters[kCountShadow];
+  const bool mis = sc.sample_mode == TTGPU_SAMPLE_BOTH;
+  for (uint32_t k = gtid(); k < count; k += grid_stride()) {
+    const uint32_t i = q.shadow_queue[k];
+    const float4 o = q.sh_org[i];
+    const float4 d4 = q.sh_dir[i];
+    Ray r;
+    r.org = xyz(o);
+    r.dir = xyz(d4);
+    r.tmax = o.w;
+    HitRec h;
+    const bool occ = traverse<true>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
+    float4 rad = q.rad[i];
+    if (!occ) {
+      const uint4 aux = q.sh_aux[i];
+      Tracker tr{aux.x, aux.y};
+      const float bsdf_pdf = as_float((int)aux.z);
+      const uint32_t bins = q.bins[i];
+      Pcg rng = load_rng(q, i);
+      NullMat nulls = nm_ones(), track = nm_ones();
+      transmit_scene(sc, r.org, r.dir, (double)o.w, bins, rng, tr, nulls, track);
+      store_rng(q, i, rng);
+      const NullMat wt{q.wt_r[i], q.wt_c[i]};
+      nm_acc(nulls, wt);
+      nm_acc(track, wt);
+      const bool delta = d4.w < 0.f;
+      const float4 fac = mis_factor(mis && !delta, fabsf(d4.w), bsdf_pdf, nulls, track);
+      rad = add_rgb(rad, spectral_to_rgb(fac * q.sh_contrib[i], bins));
+    }
     if (q.flags[i] & kFlagFinalizeAfterShadow) finalize_path(q, fb, fb_sq, i, rad);
     else q.rad[i] = rad;
   }
@@ -621,11 +845,19 @@ void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
 }
 void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
                  uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
-  hipLaunchKernelGGL(k_shade, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out, count_out,
-                     fb, fb_sq);
+  if (sc.has_media)
+    hipLaunchKernelGGL(k_shade_media, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, queue_in, count_in,
+                       queue_out, count_out, fb, fb_sq);
+  else
+    hipLaunchKernelGGL(k_shade, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out, count_out,
+                       fb, fb_sq);
 }
 void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
-  hipLaunchKernelGGL(k_shadow, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq, q.counters);
+  if (sc.has_media)
+    hipLaunchKernelGGL(k_shadow_media, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq,
+                       q.counters);
+  else
+    hipLaunchKernelGGL(k_shadow, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq, q.counters);
 }
 void LaunchTraceRays(const DeviceScene &sc, int64_t n, const float *org, const float *dir, const float *tmax, float *t, int32_t *prim,
                      float *u, float *v, int32_t *occl, int brute, hipStream_t st) {

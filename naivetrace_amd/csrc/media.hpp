@@ -1,0 +1,461 @@
+// Participating media on the GPU: medium tracker, collision coefficients,
+// spectral null-path tracking and transmission, volume-boundary queries.
+//
+// Semantics follow ToyTrace's null-path integrator
+// (src/media_integrator.cxx:218-508), the medium tracker
+// (src/rendering_util.hxx:18-137), the media (src/shader.cxx:1162-1316) and
+// the exponential atmosphere (src/atmosphere.cxx:20-200).
+//
+// The N x N path-space weight matrices of the null-path formulation are
+// products of outer products  (1/f) (x) p  accumulated element-wise, so each is
+// itself an outer product  r (x) c.  A path therefore carries 2 x 4 floats
+// instead of 16.  Every accumulated factor is rescaled by the hero
+// wavelength's pdf (r *= s/f, c *= p/s), which keeps r and c near the
+// magnitude of the matrix entries they represent.  Entries that come out NaN
+// (0 x inf) read as +inf, like the reference's FixNan.
+//
+// Distances along a ray and positions inside the atmosphere are FP64: the
+// atmosphere spans thousands of kilometres and the optical depth of long
+// grazing rays needs the extra range.
+#pragma once
+#include "device_math.hpp"
+#include "shading.hpp"
+#include "trace.hpp"
+
+namespace ttgpu {
+
+constexpr double kLargeNumber = 1.1235582092889474e+307;  // DBL_MAX / 16, the reference's LargeNumber
+constexpr float kFltLarge = 2.1267647e+37f;               // FLT_MAX / 16
+constexpr int kMaxTrackingIterations = 10000;              // media_integrator.cxx:353
+constexpr int kVolStack = 64;                              // volume-BVH traversal stack (host rejects deeper trees)
+
+// ---- path-space weight matrix as an outer product ----------------------------
+struct NullMat {
+  float4 r, c;
+};
+TT_HD NullMat nm_ones() { return {f4(1.f), f4(1.f)}; }
+TT_HD NullMat nm_inf() { return {f4(__builtin_huge_valf()), f4(1.f)}; }
+TT_HD float4 div4(float s, float4 f) { return make_float4(s / f.x, s / f.y, s / f.z, s / f.w); }
+// Accumulate (1/f) (x) p  (nullpath::AccumulateContributions, media_integrator.cxx:218-238).
+TT_HD void nm_acc(NullMat &w, float4 f, float4 p) {
+  const float s = p.x;
+  if (s > 0.f && s < __builtin_huge_valf()) {
+    w.r = w.r * div4(s, f);
+    w.c = w.c * (p * (1.f / s));
+  } else {
+    w.r = w.r * div4(1.f, f);
+    w.c = w.c * p;
+  }
+}
+TT_HD void nm_acc(NullMat &w, const NullMat &x) {
+  w.r = w.r * x.r;
+  w.c = w.c * x.c;
+}
+TT_HD float nm_fix(float v) { return v != v ? __builtin_huge_valf() : v; }
+TT_HD float nm_entry(const NullMat &m, int i, int j) { return nm_fix(at(m.r, i) * at(m.c, j)); }
+// Mean over each row of the matrix times `scale`, NaN read as +inf (pt2.cxx:452-476 helpers).
+TT_HD float4 nm_row_mean(const NullMat &m, float scale) {
+  float out[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s += nm_fix(nm_entry(m, i, j) * scale);
+    out[i] = 0.25f * s;
+  }
+  return make_float4(out[0], out[1], out[2], out[3]);
+}
+// MisWeight (pt2.cxx:452-476): contribution factor w / den per wavelength.
+TT_HD float4 mis_factor(bool use_mis, float scatter_pdf, float other_pdf, const NullMat &dp, const NullMat &dpo) {
+  if (!use_mis) {
+    const float4 den = nm_row_mean(dp, 1.f);
+    return div4(1.f, den);
+  }
+  const float4 den = nm_row_mean(dp, scatter_pdf) + nm_row_mean(dpo, other_pdf);
+  return div4(scatter_pdf, den);
+}
+
+// ---- medium tracker (rendering_util.hxx:18-137) -----------------------------------
+// Packed as: x = current medium (8 bit), y = four slots of 8 bit (kEmptySlot = free).
+struct Tracker {
+  uint32_t cur, slots;
+};
+TT_HD int tr_slot(const Tracker &t, int k) { return (t.slots >> (8 * k)) & 0xFF; }
+TT_HD void tr_set(Tracker &t, int k, int m) { t.slots = (t.slots & ~(0xFFu << (8 * k))) | ((uint32_t)m << (8 * k)); }
+TT_HD int medium_priority(const DeviceScene &sc, int m) { return m == kVacuum ? -1 : sc.media[m].priority; }
+
+TT_HD void going_through_surface(const DeviceScene &sc, Tracker &t, float3 dir, float3 ng, int material) {
+  const DevMaterial mat = sc.materials[material];
+  if (mat.medium < 0 && mat.outer_medium < 0) return;
+  int to_enter = mat.medium, to_exit = mat.outer_medium;
+  if (dot(dir, ng) > 0.f) { int x = to_enter; to_enter = to_exit; to_exit = x; }
+  if (to_exit >= 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (tr_slot(t, k) == to_exit) { tr_set(t, k, kEmptySlot); break; }
+    if ((int)t.cur == to_exit) {
+      int best = sc.empty_space_medium;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int s = tr_slot(t, k);
+        if (s != kEmptySlot && medium_priority(sc, best) < medium_priority(sc, s)) best = s;
+      }
+      t.cur = (uint32_t)best;
+    }
+  }
+  if (to_enter >= 0) {
+    bool inserted = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (!inserted && tr_slot(t, k) == kEmptySlot) { tr_set(t, k, to_enter); inserted = true; }
+    if (inserted && medium_priority(sc, to_enter) > medium_priority(sc, (int)t.cur)) t.cur = (uint32_t)to_enter;
+  }
+}
+// MaybeGoingThroughSurface (rendering_util.hxx:130-137): only when leaving through the surface.
+TT_HD void maybe_going_through(const DeviceScene &sc, Tracker &t, float3 dir, const SurfHit &h) {
+  if (dot(dir, h.n) < 0.f) going_through_surface(sc, t, dir, h.ng, h.material);
+}
+
+// ---- collision coefficients --------------------------------------------------------
+struct D3 {
+  double x, y, z;
+};
+TT_HD D3 d3(float3 a) { return {(double)a.x, (double)a.y, (double)a.z}; }
+TT_HD D3 d3_along(D3 o, float3 d, double t) { return {o.x + (double)d.x * t, o.y + (double)d.y * t, o.z + (double)d.z * t}; }
+TT_HD float3 f3(D3 a) { return mk((float)a.x, (float)a.y, (float)a.z); }
+
+TT_HD double atmo_altitude(const DevMedium &m, D3 p) {
+  const double x = p.x - m.planet_center[0], y = p.y - m.planet_center[1], z = p.z - m.planet_center[2];
+  return sqrt(x * x + y * y + z * z) - m.planet_radius;
+}
+// ExponentialConstituentDistribution::ComputeCollisionCoefficients (atmosphere.cxx:130-200).
+TT_HD void atmo_coeffs(const DevMedium &m, double alt, uint32_t bins, float4 &ss_ray, float4 &ss_mie, float4 &sa) {
+  alt = alt > m.lower_cutoff ? alt : m.lower_cutoff;
+  const float rr = expf((float)(-alt * m.rayleigh_inv_h)), rm = expf((float)(-alt * m.mie_inv_h));
+  ss_ray = take(m.rayleigh_sigma_s, bins) * rr;
+  ss_mie = take(m.mie_sigma_s, bins) * rm;
+  sa = take(m.mie_sigma_a, bins) * rm;
+}
+TT_HD void medium_coeffs(const DeviceScene &sc, int mid, D3 pos, uint32_t bins, float4 &ss, float4 &st) {
+  const DevMedium &m = sc.media[mid];
+  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    float4 a, b, c;
+    atmo_coeffs(m, atmo_altitude(m, pos), bins, a, b, c);
+    ss = a + b;
+    st = ss + c;
+    return;
+  }
+  ss = take(m.sigma_s, bins);
+  st = take(m.sigma_t, bins);
+}
+// Majorant over a segment; the atmosphere's is taken at the segment's lowest point
+// (SphereGeometry::ComputeLowestPointAlong, atmosphere.hxx:95-123).
+TT_HD float4 medium_majorant(const DeviceScene &sc, int mid, D3 org, float3 dir, double len, uint32_t bins) {
+  const DevMedium &m = sc.media[mid];
+  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    const double ox = org.x - m.planet_center[0], oy = org.y - m.planet_center[1], oz = org.z - m.planet_center[2];
+    double tl = -(ox * dir.x + oy * dir.y + oz * dir.z);
+    tl = tl > len ? len : (tl < 0. ? 0. : tl);
+    float4 a, b, c;
+    atmo_coeffs(m, atmo_altitude(m, d3_along(org, dir, tl)), bins, a, b, c);
+    return a + b + c;
+  }
+  return take(m.sigma_t, bins);
+}
+
+// Phase function of a medium at a point.  The atmosphere mixes Rayleigh and Henyey-Greenstein
+// by the constituents' scattering coefficients (SimpleCombined, phasefunctions.cxx:199-270).
+TT_HD void atmo_phase_weights(const DevMedium &m, D3 pos, uint32_t bins, float4 w[2], float sel[2]) {
+  float4 sa;
+  atmo_coeffs(m, atmo_altitude(m, pos), bins, w[0], w[1], sa);
+  const float4 norm = w[0] + w[1];
+  float tot = 0.f;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    sel[c] = w[c].x + w[c].y + w[c].z + w[c].w;
+    tot += sel[c];
+  }
+  if (tot > 0.f) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      sel[c] /= tot;
+      w[c] = make_float4(w[c].x / norm.x, w[c].y / norm.y, w[c].z / norm.z, w[c].w / norm.w);
+    }
+  } else {
+    sel[0] = sel[1] = 0.5f;
+  }
+}
+TT_HD Scatter medium_phase_sample(const DeviceScene &sc, int mid, float3 wi, D3 pos, uint32_t bins, Pcg &rng) {
+  const DevMedium &m = sc.media[mid];
+  Scatter s;
+  s.delta = false;
+  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    float4 w[2];
+    float sel[2];
+    atmo_phase_weights(m, pos, bins, w, sel);
+    const int c = rng.uniform() < sel[0] ? 0 : 1;
+    const int kinds[2] = {TTGPU_PHASE_RAYLEIGH, TTGPU_PHASE_HENYEY_GREENSTEIN};
+    const float gs[2] = {0.f, m.mie_g};
+    float val;
+    phase_sample(kinds[c], gs[c], wi, rng, s.dir, val);
+    const float other = phase_eval(kinds[1 - c], gs[1 - c], wi, s.dir);
+    const float p0 = c == 0 ? val : other, p1 = c == 0 ? other : val;
+    s.value = w[0] * p0 + w[1] * p1;
+    s.pdf = sel[0] * p0 + sel[1] * p1;
+    return s;
+  }
+  float val;
+  phase_sample(m.phase_kind, m.phase_g, wi, rng, s.dir, val);
+  s.value = f4(val);
+  s.pdf = val;
+  return s;
+}
+TT_HD float4 medium_phase_eval(const DeviceScene &sc, int mid, float3 wi, D3 pos, uint32_t bins, float3 wo, float *pdf) {
+  const DevMedium &m = sc.media[mid];
+  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    float4 w[2];
+    float sel[2];
+    atmo_phase_weights(m, pos, bins, w, sel);
+    const float p0 = phase_eval(TTGPU_PHASE_RAYLEIGH, 0.f, wi, wo), p1 = phase_eval(TTGPU_PHASE_HENYEY_GREENSTEIN, m.mie_g, wi, wo);
+    *pdf = sel[0] * p0 + sel[1] * p1;
+    return w[0] * p0 + w[1] * p1;
+  }
+  const float v = phase_eval(m.phase_kind, m.phase_g, wi, wo);
+  *pdf = v;
+  return f4(v);
+}
+
+TT_HD float4 exp_neg(double seg, float4 maj) {
+  return make_float4(expf((float)(-seg * maj.x)), expf((float)(-seg * maj.y)), expf((float)(-seg * maj.z)),
+                     expf((float)(-seg * maj.w)));
+}
+
+// nullpath::Tracking(medium, ...) (media_integrator.cxx:240-290): hero-wavelength free-flight
+// sampling against the majorant; null collisions continue, the first real one is returned.
+TT_HD bool track_medium(const DeviceScene &sc, int mid, D3 org, float3 dir, double len, uint32_t bins, Pcg &rng, NullMat &w_nulls,
+                        NullMat &w_track, double &t_out, float4 &sigma_s_out) {
+  const float4 maj = medium_majorant(sc, mid, org, dir, len, bins);
+  const double mfp = 1.0 / (double)maj.x;
+  double t = 0.;
+  for (int it = 1; it < kMaxTrackingIterations; ++it) {
+    const double v = (double)(-logf(1.f - rng.uniform())) * mfp;
+    const double tlocal = v < kLargeNumber ? v : kLargeNumber;
+    const double seg = tlocal < len - t ? tlocal : len - t;
+    const float4 tr = exp_neg(seg, maj);
+    t += tlocal;
+    if (t >= len) {
+      nm_acc(w_track, tr, tr);
+      nm_acc(w_nulls, tr, tr);
+      t_out = t;
+      return false;
+    }
+    float4 ss, st;
+    medium_coeffs(sc, mid, d3_along(org, dir, t), bins, ss, st);
+    const float4 sn = maj - st;
+    const float prob_n = (float)((double)sn.x * mfp);
+    if (rng.uniform() < prob_n) {
+      nm_acc(w_track, tr * sn, tr * sn);
+      nm_acc(w_nulls, tr * sn, tr * maj);
+    } else {
+      nm_acc(w_track, tr * st, tr * st);
+      nm_acc(w_nulls, tr * st, tr);
+      t_out = t;
+      sigma_s_out = ss;
+      return true;
+    }
+  }
+  t_out = len;
+  return false;
+}
+
+// nullpath::Transmission(medium, ...) (media_integrator.cxx:292-320): ratio tracking through
+// null collisions only.
+TT_HD void transmit_medium(const DeviceScene &sc, int mid, D3 org, float3 dir, double len, uint32_t bins, Pcg &rng, NullMat &w_nulls,
+                           NullMat &w_track) {
+  const float4 maj = medium_majorant(sc, mid, org, dir, len, bins);
+  const double mfp = 1.0 / (double)maj.x;
+  double t = 0.;
+  for (int it = 1; it < kMaxTrackingIterations; ++it) {
+    const double v = (double)(-logf(1.f - rng.uniform())) * mfp;
+    const double tlocal = v < kLargeNumber ? v : kLargeNumber;
+    const double seg = tlocal < len - t ? tlocal : len - t;
+    const float4 tr = exp_neg(seg, maj);
+    t += tlocal;
+    if (t >= len) {
+      nm_acc(w_track, tr, tr);
+      nm_acc(w_nulls, tr, tr);
+      return;
+    }
+    float4 ss, st;
+    medium_coeffs(sc, mid, d3_along(org, dir, t), bins, ss, st);
+    const float4 sn = maj - st;
+    nm_acc(w_track, tr * sn, tr * sn);
+    nm_acc(w_nulls, tr * sn, tr * maj);
+    bool all_large = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) all_large &= nm_entry(w_nulls, k, k) > kFltLarge;
+    if (all_large) {
+      w_nulls = nm_inf();
+      return;
+    }
+  }
+}
+
+// ---- volume boundaries ---------------------------------------------------------------
+// All boundary crossings at the smallest distance in (t_prev, t_end), at most 4, one per
+// material (the ordered all-hits query with (t, material) de-duplication that the segment
+// iteration of nullpath::Tracking walks, media_integrator.cxx:330-376).
+struct BoundaryHits {
+  double t;
+  int n;
+  int prim[4];
+};
+
+TT_HD bool sphere_roots(float3 org, float3 dir, float4 cr, double &t0, double &t1) {
+  const double ox = (double)org.x - cr.x, oy = (double)org.y - cr.y, oz = (double)org.z - cr.z;
+  const double dx = dir.x, dy = dir.y, dz = dir.z, r = cr.w;
+  const double a = dx * dx + dy * dy + dz * dz;
+  const double b = ox * dx + oy * dy + oz * dz;
+  const double qx = ox - dx * (b / a), qy = oy - dy * (b / a), qz = oz - dz * (b / a);
+  const double disc = a * (r * r - (qx * qx + qy * qy + qz * qz));
+  if (disc < 0.) return false;
+  const double sd = sqrt(disc);
+  const double q = b > 0. ? -b - sd : -b + sd;
+  const double c = ox * ox + oy * oy + oz * oz - r * r;
+  t0 = q != 0. ? c / q : 0.;
+  t1 = q / a;
+  if (t0 > t1) { double x = t0; t0 = t1; t1 = x; }
+  return true;
+}
+
+TT_HD void boundary_consider(const DeviceScene &sc, BoundaryHits &bh, int prim, double t) {
+  if (t < bh.t) {
+    bh.t = t;
+    bh.n = 1;
+    bh.prim[0] = prim;
+    return;
+  }
+  if (t == bh.t && bh.n < 4) {
+    const int m = sc.vol_material[prim];
+    for (int k = 0; k < bh.n; ++k)
+      if (sc.vol_material[bh.prim[k]] == m) return;
+    bh.prim[bh.n++] = prim;
+  }
+}
+
+TT_HD void next_boundaries(const DeviceScene &sc, float3 org, float3 dir, double t_prev, double t_end, BoundaryHits &bh) {
+  bh.t = t_end;
+  bh.n = 0;
+  if (sc.num_vol_prims == 0) return;
+  Ray r;
+  r.org = org;
+  r.dir = dir;
+  r.tmax = (float)t_end;
+  const RayPre p = precompute(r);
+  int stack[kVolStack];
+  int sp = 0;
+  int node = 0;
+  for (;;) {
+    const BvhNode n = sc.vol_nodes[node];
+    float tl, trr;
+    box2(n, p, __builtin_huge_valf(), tl, trr);
+    const int children[2] = {n.ch.x, n.ch.y};
+    const int counts[2] = {n.ch.z, n.ch.w};
+    const float tc[2] = {tl, trr};
+    int next = -1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (!((double)tc[c] <= bh.t)) continue;
+      if (children[c] < 0) {
+        const int first = ~children[c];
+        for (int k = 0; k < counts[c]; ++k) {
+          const int pi = first + k;
+          const PrimRecord pr = sc.vol_prims[pi];
+          if (as_int(pr.b.w) == 0) {
+            float t, u, v;
+            if (intersect_triangle(p, org, xyz(pr.a), xyz(pr.b), xyz(pr.c), __builtin_huge_valf(), t, u, v) && (double)t > t_prev &&
+                (double)t <= bh.t && (double)t < t_end)
+              boundary_consider(sc, bh, pi, (double)t);
+          } else {
+            double t0, t1;
+            if (sphere_roots(org, dir, make_float4(pr.a.x, pr.a.y, pr.a.z, pr.b.x), t0, t1)) {
+              if (t0 > t_prev && t0 > 0. && t0 < t_end) boundary_consider(sc, bh, pi, t0);
+              if (t1 > t_prev && t1 > 0. && t1 < t_end) boundary_consider(sc, bh, pi, t1);
+            }
+          }
+        }
+      } else if (next < 0) {
+        next = children[c];
+      } else if (sp < kVolStack) {
+        stack[sp++] = children[c];
+      }
+    }
+    if (next < 0) {
+      if (sp == 0) break;
+      next = stack[--sp];
+    }
+    node = next;
+  }
+}
+
+// Geometric normal of a volume-boundary primitive at distance t along the ray (unoriented).
+TT_HD float3 boundary_normal(const DeviceScene &sc, int prim, float3 org, float3 dir, double t) {
+  const PrimRecord pr = sc.vol_prims[prim];
+  if (as_int(pr.b.w) == 0) return normalize(cross(xyz(pr.b) - xyz(pr.a), xyz(pr.c) - xyz(pr.a)));
+  const D3 p = d3_along(d3(org), dir, t);
+  const double x = p.x - pr.a.x, y = p.y - pr.a.y, z = p.z - pr.a.z;
+  const double inv = 1.0 / sqrt(x * x + y * y + z * z);
+  return mk((float)(x * inv), (float)(y * inv), (float)(z * inv));
+}
+
+TT_HD void cross_boundaries(const DeviceScene &sc, Tracker &tr, const BoundaryHits &bh, float3 org, float3 dir) {
+  for (int k = 0; k < bh.n; ++k)
+    going_through_surface(sc, tr, dir, boundary_normal(sc, bh.prim[k], org, dir, bh.t), sc.vol_material[bh.prim[k]]);
+}
+
+// nullpath::Tracking(scene, ...) (media_integrator.cxx:330-376) along [0, seg_end): returns true
+// on a real collision at distance t_coll with scattering coefficient ss.
+TT_HD bool track_scene(const DeviceScene &sc, float3 org, float3 dir, double seg_end, uint32_t bins, Pcg &rng, Tracker &tr,
+                       NullMat &nulls, NullMat &track, double &t_coll, float4 &ss, int &coll_medium) {
+  double tnear = 0.;
+  for (int guard = 0; guard < 1 << 16; ++guard) {
+    BoundaryHits bh;
+    next_boundaries(sc, org, dir, tnear, seg_end, bh);
+    const double sfar = bh.n ? bh.t : seg_end;
+    if ((int)tr.cur != kVacuum) {
+      double t;
+      if (track_medium(sc, (int)tr.cur, d3_along(d3(org), dir, tnear), dir, sfar - tnear, bins, rng, nulls, track, t, ss)) {
+        t_coll = tnear + t;
+        coll_medium = (int)tr.cur;
+        return true;
+      }
+    }
+    if (!bh.n) break;
+    cross_boundaries(sc, tr, bh, org, dir);
+    tnear = bh.t;
+  }
+  return false;
+}
+
+// nullpath::Transmission(scene, ...) (media_integrator.cxx:378-408) along [0, length) after the
+// caller's occlusion test.
+TT_HD void transmit_scene(const DeviceScene &sc, float3 org, float3 dir, double length, uint32_t bins, Pcg &rng, Tracker &tr,
+                          NullMat &nulls, NullMat &track) {
+  double tnear = 0.;
+  for (int guard = 0; guard < 1 << 16; ++guard) {
+    BoundaryHits bh;
+    next_boundaries(sc, org, dir, tnear, length, bh);
+    const double sfar = bh.n ? bh.t : length;
+    if ((int)tr.cur != kVacuum) {
+      NullMat s_n = nm_ones(), s_t = nm_ones();
+      transmit_medium(sc, (int)tr.cur, d3_along(d3(org), dir, tnear), dir, sfar - tnear, bins, rng, s_n, s_t);
+      nm_acc(nulls, s_n);
+      nm_acc(track, s_t);
+    }
+    if (!bh.n) break;
+    cross_boundaries(sc, tr, bh, org, dir);
+    tnear = bh.t;
+  }
+}
+
+}  // namespace ttgpu

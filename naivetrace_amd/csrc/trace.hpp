@@ -97,21 +97,27 @@ TT_HD bool intersect_triangle(const RayPre &p, float3 org, float3 v0, float3 v1,
   return true;
 }
 
-// Sphere: nearest root in (tmin_eps, tmax), numerically stable form.
+// Sphere: nearest root in (eps, tmax), numerically stable form, in FP64 so that spheres of
+// planetary size (thousands of scene units) keep their hit distances and inside/outside
+// decisions exact to well below the self-intersection offset.  The lower bound eps is
+// 1e-9 of the sphere's extent and absorbs the residual root of a ray leaving the surface.
 TT_HD bool intersect_sphere(float3 org, float3 dir, float4 cr, float tmax, float &t) {
-  float3 oc = org - mk(cr.x, cr.y, cr.z);
-  float b = dot(oc, dir);
-  float3 qc = oc - dir * b;
-  float disc = cr.w * cr.w - dot(qc, qc);
-  if (disc < 0.f) return false;
-  float sd = sqrtf(disc);
-  float q = b > 0.f ? -b - sd : -b + sd;
-  float c = dot(oc, oc) - cr.w * cr.w;
-  float t0 = c / q, t1 = q;
-  if (t0 > t1) { float x = t0; t0 = t1; t1 = x; }
-  float eps = 1e-6f * (fabsf(cr.x) + fabsf(cr.y) + fabsf(cr.z) + cr.w);
-  if (t0 > eps && t0 < tmax) { t = t0; return true; }
-  if (t1 > eps && t1 < tmax) { t = t1; return true; }
+  const double ox = (double)org.x - cr.x, oy = (double)org.y - cr.y, oz = (double)org.z - cr.z;
+  const double dx = dir.x, dy = dir.y, dz = dir.z, r = cr.w;
+  const double a = dx * dx + dy * dy + dz * dz;
+  const double b = ox * dx + oy * dy + oz * dz;
+  const double s = b / a;
+  const double qx = ox - dx * s, qy = oy - dy * s, qz = oz - dz * s;
+  const double disc = a * (r * r - (qx * qx + qy * qy + qz * qz));
+  if (disc < 0.) return false;
+  const double sd = sqrt(disc);
+  const double q = b > 0. ? -b - sd : -b + sd;
+  const double c = ox * ox + oy * oy + oz * oz - r * r;
+  double t0 = q != 0. ? c / q : 0., t1 = q / a;
+  if (t0 > t1) { double x = t0; t0 = t1; t1 = x; }
+  const double eps = 1e-9 * ((double)fabsf(cr.x) + fabsf(cr.y) + fabsf(cr.z) + r);
+  if (t0 > eps && t0 < (double)tmax) { t = (float)t0; return true; }
+  if (t1 > eps && t1 < (double)tmax) { t = (float)t1; return true; }
   return false;
 }
 

@@ -193,10 +193,17 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
     if (e.kind == TTGPU_ENV_MAP && (e.tex_width <= 0 || e.tex_height <= 0 || !e.texture_rgb)) bad("env map needs a texture");
   }
   if (s.num_media > kMaxMedia) bad("too many media");
-  if (s.num_media > 0 || s.empty_space_medium >= 0) {
-    bool any_medium = s.empty_space_medium >= 0;
-    for (int i = 0; i < s.num_materials; ++i) any_medium |= s.materials[i].medium >= 0 || s.materials[i].outer_medium >= 0;
-    if (any_medium) bad("participating media are handled by the media kernels, which this build does not enable yet");
+  if (s.num_media > 0 && !s.media) bad("media table missing");
+  if (s.empty_space_medium >= s.num_media || s.empty_space_medium < -1) bad("empty_space_medium out of range");
+  for (int i = 0; i < s.num_media; ++i) {
+    const auto &m = s.media[i];
+    if (m.kind < TTGPU_MEDIUM_VACUUM || m.kind > TTGPU_MEDIUM_ATMOSPHERE) bad("unknown medium kind");
+    if (m.kind == TTGPU_MEDIUM_HOMOGENEOUS && (m.phase_kind < TTGPU_PHASE_UNIFORM || m.phase_kind > TTGPU_PHASE_RAYLEIGH))
+      bad("unknown phase function kind");
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE && !(m.rayleigh_scale_height > 0. && m.mie_scale_height > 0. && m.planet_radius > 0.))
+      bad("atmosphere needs positive scale heights and planet radius");
+    for (int b = 0; b < kNBins; ++b)
+      if (!(m.sigma_s[b] >= 0.f && m.sigma_a[b] >= 0.f)) bad("medium coefficients must be non-negative");
   }
   if (s.camera.kind != TTGPU_CAMERA_PINHOLE && s.camera.kind != TTGPU_CAMERA_FISHEYE) bad("unknown camera kind");
 }
@@ -311,6 +318,114 @@ void BuildGeometry(ttgpu_handle *h, const ttgpu_scene &s, const std::vector<int6
   }
 }
 
+// MediumTracker::initializePosition (src/rendering_util.cxx:13-37) at the camera, on the host in
+// FP64: a probe ray from outside the scene along -x to the camera position, crossing every
+// volume boundary and then every surface in order.  Returns the packed tracker
+// (current medium in bits 0-7, four slots in bits 32-63).
+struct HostTracker {
+  int cur;
+  int slots[4];
+};
+void HostGoingThrough(const ttgpu_scene &s, HostTracker &t, V3d dir, V3d n, int material, int empty) {
+  const auto &mat = s.materials[material];
+  if (mat.medium < 0 && mat.outer_medium < 0) return;
+  auto prio = [&](int m) { return m == kVacuum ? -1 : s.media[m].priority; };
+  int to_enter = mat.medium, to_exit = mat.outer_medium;
+  float d = (float)dir.x * (float)n.x + (float)dir.y * (float)n.y + (float)dir.z * (float)n.z;
+  if (d > 0.f) std::swap(to_enter, to_exit);
+  if (to_exit >= 0) {
+    for (int &sl : t.slots)
+      if (sl == to_exit) { sl = kEmptySlot; break; }
+    if (to_exit == t.cur) {
+      int best = empty;
+      for (int sl : t.slots)
+        if (sl != kEmptySlot && prio(best) < prio(sl)) best = sl;
+      t.cur = best;
+    }
+  }
+  if (to_enter >= 0) {
+    bool inserted = false;
+    for (int &sl : t.slots)
+      if (sl == kEmptySlot) { sl = to_enter; inserted = true; break; }
+    if (inserted && prio(to_enter) > prio(t.cur)) t.cur = to_enter;
+  }
+}
+uint64_t PackTracker(const HostTracker &t) {
+  uint64_t v = (uint64_t)(uint32_t)t.cur;
+  for (int k = 0; k < 4; ++k) v |= (uint64_t)(uint32_t)t.slots[k] << (32 + 8 * k);
+  return v;
+}
+uint64_t CameraTracker(const ttgpu_scene &s, int empty) {
+  HostTracker t{empty, {kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot}};
+  const int64_t nprim = s.num_triangles + s.num_spheres;
+  if (nprim == 0) return PackTracker(t);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  auto grow = [&](const double p[3], double r) {
+    for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], p[a] - r); hi[a] = std::max(hi[a], p[a] + r); }
+  };
+  for (int64_t p = 0; p < s.num_triangles; ++p)
+    for (int j = 0; j < 3; ++j) {
+      const float *v = s.vertices + 3 * s.indices[3 * p + j];
+      double q[3] = {v[0], v[1], v[2]};
+      grow(q, 0.);
+    }
+  for (int64_t p = 0; p < s.num_spheres; ++p) {
+    const float *sp = s.spheres + 4 * p;
+    double q[3] = {sp[0], sp[1], sp[2]};
+    grow(q, sp[3]);
+  }
+  const V3d pos{s.camera.pos[0], s.camera.pos[1], s.camera.pos[2]};
+  const double pc[3] = {pos.x, pos.y, pos.z};
+  for (int a = 0; a < 3; ++a)
+    if (pc[a] < lo[a] || pc[a] > hi[a]) return PackTracker(t);
+  const double dist = 2. * std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+  const V3d org = pos + V3d{dist, 0., 0.}, dir{-1., 0., 0.};
+  struct BHit { double t; int material; V3d n; };
+  for (int pass = 0; pass < 2; ++pass) {  // volume boundaries first, then surfaces
+    std::vector<BHit> hits;
+    auto add = [&](double tt, int material, V3d n) {
+      if (!(tt > 0. && tt < dist)) return;
+      for (const auto &b : hits)
+        if (b.t == tt && b.material == material) return;
+      hits.push_back({tt, material, n});
+    };
+    for (int64_t p = 0; p < nprim; ++p) {
+      const int m = p < s.num_triangles ? s.tri_material[p] : s.sphere_material[p - s.num_triangles];
+      const bool surface = s.materials[m].shader >= 0;
+      if (surface != (pass == 1)) continue;
+      if (p < s.num_triangles) {
+        const uint32_t *ix = s.indices + 3 * p;
+        V3d A{s.vertices[3 * ix[0]], s.vertices[3 * ix[0] + 1], s.vertices[3 * ix[0] + 2]};
+        V3d B{s.vertices[3 * ix[1]], s.vertices[3 * ix[1] + 1], s.vertices[3 * ix[1] + 2]};
+        V3d C{s.vertices[3 * ix[2]], s.vertices[3 * ix[2] + 1], s.vertices[3 * ix[2] + 2]};
+        const V3d e1 = B - A, e2 = C - A, pv = Cross(dir, e2);
+        const double det = Dot(e1, pv);
+        if (det == 0.) continue;
+        const double inv = 1. / det;
+        const V3d tv = org - A;
+        const double u = Dot(tv, pv) * inv;
+        if (u < 0. || u > 1.) continue;
+        const V3d qv = Cross(tv, e1);
+        const double v = Dot(dir, qv) * inv;
+        if (v < 0. || u + v > 1.) continue;
+        add(Dot(e2, qv) * inv, m, Normalized(Cross(e1, e2)));
+      } else {
+        const float *sp = s.spheres + 4 * (p - s.num_triangles);
+        const V3d c{sp[0], sp[1], sp[2]};
+        const V3d oc = org - c;
+        const double b = Dot(oc, dir), cc = Dot(oc, oc) - (double)sp[3] * sp[3];
+        const double disc = b * b - cc;
+        if (disc < 0.) continue;
+        const double sd = std::sqrt(disc);
+        for (double tt : {-b - sd, -b + sd}) add(tt, m, Normalized(org + dir * tt - c));
+      }
+    }
+    std::stable_sort(hits.begin(), hits.end(), [](const BHit &a, const BHit &b) { return a.t < b.t; });
+    for (const auto &b : hits) HostGoingThrough(s, t, dir, b.n, b.material, empty);
+  }
+  return PackTracker(t);
+}
+
 void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
   std::vector<DevShader> shaders(std::max(1, s.num_shaders));
   for (int i = 0; i < s.num_shaders; ++i) {
@@ -400,10 +515,40 @@ void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
   for (int k = 0; k < 4; ++k) h->dsc.type_probs[k] = tot > 0 ? (float)(counts[k] / tot) : 0.f;
   h->dsc.area_pick_prob = h->dsc.num_area_lights ? (float)(counts[1] / tot / h->dsc.num_area_lights) : 0.f;
   h->dsc.bb_diag = (float)bb_diag;
-  h->dsc.empty_space_medium = kVacuum;
-  h->dsc.has_media = 0;
-  h->dsc.cam_tracker = ((uint64_t)kEmptySlot << 32) | ((uint64_t)kEmptySlot << 40) | ((uint64_t)kEmptySlot << 48) |
-                       ((uint64_t)kEmptySlot << 56) | (uint64_t)kVacuum;
+  // Media (src/shader.cxx:1162-1316, src/atmosphere.cxx:130-200).
+  std::vector<DevMedium> media(std::max(1, s.num_media));
+  for (int i = 0; i < s.num_media; ++i) {
+    const auto &a = s.media[i];
+    DevMedium &d = media[i];
+    std::memset(&d, 0, sizeof(d));
+    d.kind = a.kind;
+    d.priority = a.priority;
+    d.phase_kind = a.phase_kind;
+    d.phase_g = a.phase_g;
+    if (a.kind == TTGPU_MEDIUM_HOMOGENEOUS) {
+      for (int b = 0; b < kNBins; ++b) {
+        d.sigma_s[b] = a.sigma_s[b];
+        d.sigma_t[b] = (float)((double)a.sigma_s[b] + a.sigma_a[b]);
+      }
+    } else if (a.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+      for (int k = 0; k < 3; ++k) d.planet_center[k] = a.planet_center[k];
+      d.planet_radius = a.planet_radius;
+      d.rayleigh_inv_h = 1.0 / a.rayleigh_scale_height;
+      d.mie_inv_h = 1.0 / a.mie_scale_height;
+      d.lower_cutoff = std::max(-a.rayleigh_scale_height, -a.mie_scale_height);
+      d.mie_g = a.mie_g;
+      std::memcpy(d.rayleigh_sigma_s, a.rayleigh_sigma_s, sizeof(d.rayleigh_sigma_s));
+      std::memcpy(d.mie_sigma_s, a.mie_sigma_s, sizeof(d.mie_sigma_s));
+      std::memcpy(d.mie_sigma_a, a.mie_sigma_a, sizeof(d.mie_sigma_a));
+    }
+  }
+  h->dsc.media = h->Upload(media.data(), media.size());
+  h->dsc.num_media = s.num_media;
+  h->dsc.empty_space_medium = s.empty_space_medium >= 0 ? s.empty_space_medium : kVacuum;
+  bool any_medium = s.empty_space_medium >= 0;
+  for (int i = 0; i < s.num_materials; ++i) any_medium |= s.materials[i].medium >= 0 || s.materials[i].outer_medium >= 0;
+  h->dsc.has_media = any_medium ? 1 : 0;
+  h->dsc.cam_tracker = CameraTracker(s, h->dsc.empty_space_medium);
   // Camera (src/camera.hxx:43-93, 165-174).
   const auto &c = s.camera;
   V3d up = Normalized({c.up[0], c.up[1], c.up[2]}), dir = Normalized({c.dir[0], c.dir[1], c.dir[2]});
@@ -439,9 +584,15 @@ void AllocateQueues(ttgpu_handle *h) {
   q.queue_a = h->Alloc<uint32_t>(N); q.queue_b = h->Alloc<uint32_t>(N); q.shadow_queue = h->Alloc<uint32_t>(N);
   q.counters = h->Alloc<uint32_t>(kNumCounters);
   Check(hipMemset(q.counters, 0, kNumCounters * sizeof(uint32_t)), "hipMemset counters");
-  q.w_track = nullptr;
-  q.w_track_null = nullptr;
-  h->stats.state_bytes = (size_t)N * (16 * 8 + 4 * 6 + 8 * 3 + 12);
+  if (h->dsc.has_media) {
+    q.wt_r = h->Alloc<float4>(N);
+    q.wt_c = h->Alloc<float4>(N);
+    q.sh_aux = h->Alloc<uint4>(N);
+  } else {
+    q.wt_r = q.wt_c = nullptr;
+    q.sh_aux = nullptr;
+  }
+  h->stats.state_bytes = (size_t)N * (16 * 8 + 4 * 6 + 8 * 3 + 12 + (h->dsc.has_media ? 48 : 0));
   const size_t npix = (size_t)h->npix;
   h->fb = h->Alloc<double>(3 * npix);
   h->fb_sq = h->Alloc<double>(3 * npix);

@@ -79,6 +79,19 @@ template <int NW> inline Spec<NW> RowMean(const Mat<NW> &m, double scale) {
   return r;
 }
 
+// ---- exponential atmosphere geometry (atmosphere.hxx:95-123, atmosphere.cxx:84-91) ----
+inline double AtmoAltitude(const ttgpu_medium &m, V3 p) {
+  return Length(p - V3{m.planet_center[0], m.planet_center[1], m.planet_center[2]}) - m.planet_radius;
+}
+inline double AtmoLowerCutoffOf(const ttgpu_medium &m) { return std::max(-m.rayleigh_scale_height, -m.mie_scale_height); }
+// Point of the segment org + t dir, t in [0, len], closest to the planet centre.
+inline V3 AtmoLowestPoint(const ttgpu_medium &m, V3 org, V3 dir, double len) {
+  V3 c{m.planet_center[0], m.planet_center[1], m.planet_center[2]};
+  double tl = -Dot(org - c, dir);
+  if (tl > len) tl = len; else if (tl < 0.) tl = 0.;
+  return org + dir * tl;
+}
+
 struct ThroughputAndPdfsTag {};
 
 template <int NW> struct Engine {
@@ -364,10 +377,8 @@ template <int NW> struct Engine {
     if (Dot(dir, si.normal) < 0.) GoingThroughSurface(t, dir, si.geometry_normal, si.material);
   }
 
-  double Altitude(const ttgpu_medium &m, V3 p) const {
-    return Length(p - V3{m.planet_center[0], m.planet_center[1], m.planet_center[2]}) - m.planet_radius;
-  }
-  double AtmoLowerCutoff(const ttgpu_medium &m) const { return std::max(-m.rayleigh_scale_height, -m.mie_scale_height); }
+  double Altitude(const ttgpu_medium &m, V3 p) const { return AtmoAltitude(m, p); }
+  double AtmoLowerCutoff(const ttgpu_medium &m) const { return AtmoLowerCutoffOf(m); }
   void AtmoCoeffs(const ttgpu_medium &m, double alt, const PathContext<NW> &ctx, Spec<NW> &ss_ray, Spec<NW> &ss_mie, Spec<NW> &sa) const {
     alt = alt > AtmoLowerCutoff(m) ? alt : AtmoLowerCutoff(m);
     double rr = std::exp(-alt / m.rayleigh_scale_height), rm = std::exp(-alt / m.mie_scale_height);
@@ -401,10 +412,7 @@ template <int NW> struct Engine {
     if (mid == VACUUM) return SpecConst<NW>(0.);
     const auto &m = sc.media[mid];
     if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
-      V3 c{m.planet_center[0], m.planet_center[1], m.planet_center[2]};
-      double tl = -Dot(org - c, dir);
-      if (tl > len) tl = len; else if (tl < 0.) tl = 0.;
-      double alt = std::max(AtmoLowerCutoff(m), Altitude(m, org + dir * tl));
+      double alt = std::max(AtmoLowerCutoff(m), Altitude(m, AtmoLowestPoint(m, org, dir, len)));
       Spec<NW> a, b, cc;
       AtmoCoeffs(m, alt, ctx, a, b, cc);
       return a + b + cc;
@@ -1107,5 +1115,20 @@ extern "C" int ttref_quadratic_err(float a, float b, float c, float ea, float eb
   if (!QuadraticErr<float>(a, b, c, ea, eb, ec, t0, t1, e0, e1)) return 0;
   out[0] = t0; out[1] = t1; out[2] = e0; out[3] = e1;
   return 1;
+}
+extern "C" double ttref_atmo_altitude(const ttgpu_medium *m, const double pos[3]) { return AtmoAltitude(*m, V3{pos[0], pos[1], pos[2]}); }
+extern "C" void ttref_atmo_lowest_point(const ttgpu_medium *m, const double org[3], const double dir[3], double len, double out[3]) {
+  V3 p = AtmoLowestPoint(*m, V3{org[0], org[1], org[2]}, V3{dir[0], dir[1], dir[2]}, len);
+  out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+// Scattering and absorption coefficients of all 36 bins at an altitude.
+extern "C" void ttref_atmo_coeffs(const ttgpu_medium *m, double altitude, double sigma_s[36], double sigma_a[36]) {
+  const double cut = AtmoLowerCutoffOf(*m);
+  const double alt = altitude > cut ? altitude : cut;
+  const double rr = std::exp(-alt / m->rayleigh_scale_height), rm = std::exp(-alt / m->mie_scale_height);
+  for (int b = 0; b < 36; ++b) {
+    sigma_s[b] = m->rayleigh_sigma_s[b] * rr + m->mie_sigma_s[b] * rm;
+    sigma_a[b] = m->mie_sigma_a[b] * rm;
+  }
 }
 extern "C" long ttref_bisection(const double *vals, long n, double r) { return BisectionSearch(vals, n, r); }

@@ -29,6 +29,9 @@ void ttref_spectrum_to_rgb(const double spec[36], double out[3]);
 void ttref_tables(double *centers, double *smits, double *to_rgb, double *sun);
 int ttref_quadratic_err(float a, float b, float c, float ea, float eb, float ec, float *out);
 long ttref_bisection(const double *vals, long n, double r);
+double ttref_atmo_altitude(const ttgpu_medium *m, const double pos[3]);
+void ttref_atmo_lowest_point(const ttgpu_medium *m, const double org[3], const double dir[3], double len, double out[3]);
+void ttref_atmo_coeffs(const ttgpu_medium *m, double altitude, double sigma_s[36], double sigma_a[36]);
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,11 @@ FIXTURES = {
     "cornell64": (lambda w, h: configs.cornell_box(w, h), 64, 64, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
     "cornell32_consistent": (lambda w, h: configs.cornell_box(w, h), 32, 32, 8192, 8, abi.MIS_CONSISTENT, abi.SAMPLE_BOTH),
     "cornell32_bsdf": (lambda w, h: configs.cornell_box(w, h), 32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BSDF),
+    # participating media: glass sphere with a chromatic HG medium, fog box with nested media and
+    # the camera inside, exponential atmosphere seen through an upward fisheye
+    "medium_sphere32": (lambda w, h: configs.medium_sphere(w, h), 32, 32, 8192, 16, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+    "fog_box32": (lambda w, h: configs.fog_box(w, h), 32, 32, 8192, 10, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+    "atmosphere32": (lambda w, h: configs.atmosphere(w, h), 32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
 }
 
 
@@ -41,7 +46,7 @@ def make(name):
     stderr = np.sqrt(var / spp)
     out = os.path.join(HERE, f"{name}_oracle.npz")
     np.savez_compressed(out, mean=mean, stderr=stderr, spp=spp, width=w, height=h, max_depth=depth, mis_mode=mis,
-                        sample_mode=mode, seed=7)
+                        sample_mode=mode, seed=7, scene=name)
     print(f"{name}: {time.time() - t:.1f}s -> {out}")
 
 

@@ -45,6 +45,10 @@ def lib():
         L.ttref_quadratic_err.argtypes = [C.c_float] * 6 + [P(C.c_float)]
         L.ttref_bisection.argtypes = [f64p, C.c_long, C.c_double]
         L.ttref_bisection.restype = C.c_long
+        L.ttref_atmo_altitude.argtypes = [P(abi.Medium), f64p]
+        L.ttref_atmo_altitude.restype = C.c_double
+        L.ttref_atmo_lowest_point.argtypes = [P(abi.Medium), f64p, f64p, C.c_double, f64p]
+        L.ttref_atmo_coeffs.argtypes = [P(abi.Medium), C.c_double, f64p, f64p]
         _lib = L
     return _lib
 
@@ -151,3 +155,23 @@ def quadratic_err(a, b, c, ea, eb, ec):
 def bisection(vals, r):
     v = np.ascontiguousarray(vals, dtype=np.float64)
     return lib().ttref_bisection(_p(v), len(v), r)
+
+
+def atmo_altitude(medium, pos):
+    p = np.asarray(pos, dtype=np.float64)
+    return lib().ttref_atmo_altitude(C.byref(medium), _p(p))
+
+
+def atmo_lowest_point(medium, org, direction, length):
+    o = np.asarray(org, dtype=np.float64)
+    d = np.asarray(direction, dtype=np.float64)
+    out = np.zeros(3)
+    lib().ttref_atmo_lowest_point(C.byref(medium), _p(o), _p(d), length, _p(out))
+    return out
+
+
+def atmo_coeffs(medium, altitude):
+    ss = np.zeros(36)
+    sa = np.zeros(36)
+    lib().ttref_atmo_coeffs(C.byref(medium), altitude, _p(ss), _p(sa))
+    return ss, sa

@@ -122,8 +122,12 @@ def render_parity(name, mis=abi.MIS_REFERENCE, mode=abi.SAMPLE_BOTH, builder=con
     assert frac3 < 0.015
     # No systematic bias: the mean z-score over all pixel channels is ~N(0, 1/sqrt(n)).
     assert abs(z.mean()) < 5.0 / np.sqrt(z.size) + 0.02
-    # Image-average radiance agrees to within 0.5 %.
-    np.testing.assert_allclose(mean.mean(axis=(0, 1)), f["mean"].mean(axis=(0, 1)), rtol=5e-3)
+    # Image-average radiance agrees within 4 standard errors of the difference (+0.1 % for FP32).
+    npix = mean.shape[0] * mean.shape[1]
+    img, img_ref = mean.mean(axis=(0, 1)), f["mean"].mean(axis=(0, 1))
+    img_se = np.sqrt((var / spp).sum(axis=(0, 1)) + (f["stderr"] ** 2).sum(axis=(0, 1))) / npix
+    print(f"{name}: image mean {img} oracle {img_ref} +- {img_se}", flush=True)
+    assert np.all(np.abs(img - img_ref) < 4 * img_se + 1e-3 * np.abs(img_ref))
 
 
 def test_cornell_parity_reference_mis(gpu_lib):

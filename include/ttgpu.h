@@ -217,10 +217,13 @@ typedef struct ttgpu_params {
   int32_t linear_output;        /* --linear-out for GenerateImage */
   int32_t num_threads;          /* CPU threads (oracle / host side) */
   /* guiding (ptg) */
-  int32_t guide_kind;           /* ttgpu_guide_kind */
-  int32_t guide_max_spp;        /* training schedule, --guide-max-spp */
-  float guide_mix_bsdf;         /* one-sample MIS: probability to sample the BSDF (0.5) */
-  int32_t reserved[8];
+  int32_t guide_kind;           /* ttgpu_guide_kind (TTGPU_ALGO_PTG: VMF_TREE when NONE) */
+  int32_t guide_max_spp;        /* --guide-max-spp: training sweeps of 2, 4, .. up to this many spp (512) */
+  float guide_mix_bsdf;         /* ignored: the probability of keeping the BSDF sample is the
+                                   shader's (0.5; glossy dielectric with alpha > 0.05: 0.1),
+                                   src/shader.cxx:224-227, 937-941 */
+  int32_t guide_subdiv_factor;  /* --guide-subdiv-factor: min records per kd-tree leaf (100) */
+  int32_t reserved[7];
 } ttgpu_params;
 
 typedef struct ttgpu_stats {
@@ -313,6 +316,42 @@ int ttgpu_eval_shader(const ttgpu_shader *shader, const float wi[3], const float
 /* Phase function sampling (src/phasefunctions.cxx): wi = reverse incident dir. */
 int ttgpu_sample_phase(int32_t phase_kind, float g, const float wi[3], uint64_t seed,
                        int64_t count, float *out_dir, float *out_value, float *out_pdf);
+
+/* ---- guiding (TTGPU_ALGO_PTG) ------------------------------------------------
+ * ttgpu_run on a guided handle first trains: one 1-spp pass, then sweeps of
+ * 2, 4, .. up to guide_max_spp 1-spp passes, each pass recording radiance and
+ * refitting the recorder, each sweep splitting kd-tree cells and baking the
+ * mixtures the renderer samples (PathTracingAlgo2::Run of the guided
+ * integrator, src/renderingalgorithms_pathtracing_guided.cxx:1110-1211).  Then
+ * it renders with the final guide on the usual doubling schedule.  The
+ * framebuffer is cleared before each sweep and before the final render.
+ * Recorder state: kd-tree cells, training rounds (1-spp passes), records of the
+ * last training pass and records dropped for lack of buffer space. */
+int ttgpu_guide_info(ttgpu_handle *h, int32_t *cells, int32_t *rounds, int64_t *records_last_pass,
+                     int64_t *records_dropped);
+
+/* 8-lobe von Mises-Fisher mixture (vmf_fitting::VonMisesFischerMixture<8>,
+ * src/distribution_mixture_models.hxx:17-29). */
+typedef struct ttgpu_vmf8 {
+  float mu[8][3];
+  float kappa[8];
+  float w[8];
+} ttgpu_vmf8;
+/* Building blocks of the recorder, for tests (host code of the library):
+ * InitializeForUnitSphere<8> (distribution_mixture_models.cxx:167-183);
+ * incremental::Fit over n weighted directions with the recorder's parameters for
+ * a cell of max_num_samples records (prior strength max(1, 0.01 max_num_samples),
+ * M-step every 10x that; path_guiding.cxx:62-77, distribution_mixture_models.cxx:247-343),
+ * continuing from *m; kdtree::Builder (path_guiding_tree.hxx:300-438): leaf index
+ * of every point. */
+int ttgpu_guide_init_vmf(ttgpu_vmf8 *out);
+int ttgpu_guide_fit_vmf(ttgpu_vmf8 *m, const float *dirs, const float *weights, int64_t n, int64_t max_num_samples);
+int ttgpu_guide_build_kdtree(const double *pts, int64_t n, int32_t min_points, int32_t max_depth,
+                             int32_t *leaf_of_point, int32_t *num_leafs);
+/* GPU sampling and pdf of a mixture, as the guided kernels use them
+ * (vmf_fitting::Sample / Pdf, distribution_mixture_models.cxx:88-150). */
+int ttgpu_guide_sample_vmf(const ttgpu_vmf8 *m, uint64_t seed, int64_t count, float *out_dir, float *out_pdf);
+int ttgpu_guide_pdf_vmf(const ttgpu_vmf8 *m, int64_t count, const float *dirs, float *out_pdf);
 
 /* Device / build information string ("gfx950, 256 CUs, ..."). */
 const char *ttgpu_device_info(void);

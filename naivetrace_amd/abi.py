@@ -91,7 +91,7 @@ class Params(C.Structure):
                 ("sample_mode", C.c_int32), ("mis_mode", C.c_int32), ("num_wavelengths", C.c_int32),
                 ("seed", C.c_uint64), ("wavefront_paths", C.c_int64), ("linear_output", C.c_int32),
                 ("num_threads", C.c_int32), ("guide_kind", C.c_int32), ("guide_max_spp", C.c_int32),
-                ("guide_mix_bsdf", C.c_float), ("reserved", C.c_int32 * 8)]
+                ("guide_mix_bsdf", C.c_float), ("guide_subdiv_factor", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class Stats(C.Structure):
@@ -99,6 +99,10 @@ class Stats(C.Structure):
                 ("total_spp", C.c_uint64), ("render_ms", C.c_double), ("bvh_build_ms", C.c_double),
                 ("upload_ms", C.c_double), ("bvh_nodes", C.c_uint64), ("scene_bytes", C.c_uint64),
                 ("state_bytes", C.c_uint64), ("passes", C.c_int32), ("wavefront_paths", C.c_int32)]
+
+
+class Vmf8(C.Structure):
+    _fields_ = [("mu", (C.c_float * 3) * 8), ("kappa", C.c_float * 8), ("w", C.c_float * 8)]
 
 
 PASS_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32)
@@ -111,7 +115,8 @@ EXPORTS = [
     "ttgpu_kernel_time", "ttgpu_enable_kernel_timing", "ttgpu_trace_closest", "ttgpu_trace_occluded",
     "ttgpu_trace_closest_bruteforce",
     "ttgpu_sample_shader", "ttgpu_eval_shader", "ttgpu_sample_phase", "ttgpu_device_info",
-    "ttgpu_abi_version",
+    "ttgpu_abi_version", "ttgpu_guide_info", "ttgpu_guide_init_vmf", "ttgpu_guide_fit_vmf",
+    "ttgpu_guide_build_kdtree", "ttgpu_guide_sample_vmf", "ttgpu_guide_pdf_vmf",
 ]
 
 _lib = None
@@ -147,6 +152,12 @@ def _declare(lib):
         "ttgpu_sample_phase": (C.c_int, [C.c_int32, C.c_float, f32p, C.c_uint64, C.c_int64, f32p, f32p, f32p]),
         "ttgpu_device_info": (C.c_char_p, []),
         "ttgpu_abi_version": (C.c_int, []),
+        "ttgpu_guide_info": (C.c_int, [h, i32p, i32p, i64p, i64p]),
+        "ttgpu_guide_init_vmf": (C.c_int, [P(Vmf8)]),
+        "ttgpu_guide_fit_vmf": (C.c_int, [P(Vmf8), f32p, f32p, C.c_int64, C.c_int64]),
+        "ttgpu_guide_build_kdtree": (C.c_int, [f64p, C.c_int64, C.c_int32, C.c_int32, i32p, i32p]),
+        "ttgpu_guide_sample_vmf": (C.c_int, [P(Vmf8), C.c_uint64, C.c_int64, f32p, f32p]),
+        "ttgpu_guide_pdf_vmf": (C.c_int, [P(Vmf8), C.c_int64, f32p, f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -185,4 +196,5 @@ def default_params(width, height, spp=0, max_depth=8, seed=1, algo=ALGO_PT2):
     p.guide_kind = GUIDE_NONE
     p.guide_max_spp = 512
     p.guide_mix_bsdf = 0.5
+    p.guide_subdiv_factor = 100
     return p

@@ -85,6 +85,22 @@ struct DevCamera {
   int width, height;
 };
 
+// Guiding kd-tree node: a child >= 0 is a branch, < 0 the leaf (cell) ~child.
+struct GuideNode {
+  float split;
+  int axis;
+  int left, right;
+};
+
+// Baked 8-lobe vMF mixture of one cell: lobe = (mean direction, concentration),
+// wn = weight x k / (2 pi (1 - exp(-2k))), cdf = cumulative lobe weights.
+constexpr int kGuideLobes = 8;
+struct GuideCell {
+  float4 lobe[kGuideLobes];
+  float wn[kGuideLobes];
+  float cdf[kGuideLobes];
+};
+
 struct DeviceScene {
   const BvhNode *nodes;
   const PrimRecord *prims;         // leaf order
@@ -115,6 +131,11 @@ struct DeviceScene {
   uint64_t cam_tracker;            // packed medium tracker at the camera
   DevCamera cam;
   int sample_mode, mis_mode, max_depth;
+  // guiding (guided integrator only)
+  const GuideNode *guide_nodes;
+  const GuideCell *guide_cells;
+  int guide_root;                  // root handle (< 0: the tree is a single cell ~root)
+  int guide_record;                // 1: training pass, write radiance records
 };
 
 // Path state, structure of arrays over the wavefront capacity.
@@ -139,6 +160,15 @@ struct PathQueues {
   // queues
   uint32_t *queue_a, *queue_b, *shadow_queue;
   uint32_t *counters;   // [0] queue_a count, [1] queue_b count, [2] shadow count, [3..] stats
+  // guided integrator: per-vertex state, depth-major ([vertex * capacity + path]), and records
+  float4 *gv_pos_pdf;   // scatter position, mixture pdf of the sampled direction (<= 0: no record)
+  float4 *gv_dir_acc;   // sampled direction, accumulated incident radiance (mean over wavelengths)
+  float4 *gv_thru;      // path throughput after the scatter at the vertex
+  float4 *sh_nee;       // shadow ray: NEE radiance towards the previous vertex (per wavelength)
+  float4 *rec_pos_w;    // records: position, weight = radiance / pdf
+  float4 *rec_dir;      // records: direction
+  uint32_t rec_capacity;
+  int max_vertices;
   // media (only when the scene has media, else null)
   float4 *wt_r, *wt_c;  // path-space tracking weights r (x) c of the null-path formulation
   uint4 *sh_aux;        // shadow ray: medium tracker (x, y), BSDF/phase pdf bits (z)
@@ -150,8 +180,13 @@ enum FlagBits : uint32_t {
   kFlagHasLastPdf = 1u << 8,
   kFlagLastPdfDelta = 1u << 9,
   kFlagFinalizeAfterShadow = 1u << 10,
+  kFlagVertexShift = 16,            // guided integrator: number of stored vertices in bits 16-23
+  kFlagVertexMask = 0xFFu << 16,
 };
 
-enum Counter { kCountA = 0, kCountB = 1, kCountShadow = 2, kStatClosest = 4, kStatShadow = 5, kStatPaths = 6, kNumCounters = 8 };
+enum Counter {
+  kCountA = 0, kCountB = 1, kCountShadow = 2, kCountRecords = 3, kStatClosest = 4, kStatShadow = 5, kStatPaths = 6,
+  kStatRecordsDropped = 7, kNumCounters = 8
+};
 
 }  // namespace ttgpu

@@ -1,0 +1,158 @@
+// Path-guiding recorder of the guided integrator: an adaptive kd-tree over
+// space whose leaves (cells) each hold an 8-lobe von Mises-Fisher mixture of
+// the incident radiance, fitted by weighted online MAP-EM to the radiance
+// records the renderer produces.
+//
+// Follows ToyTrace's MoVMF radiance distribution and recorder:
+//   src/distribution_mixture_models.cxx:88-388 (pdf, sampling, incremental EM),
+//   src/path_guiding.cxx:45-77, 320-812 (records, stochastic filtering, rounds,
+//   initial tree build, adaptive splitting), src/path_guiding_tree.hxx:50-438
+//   (kd-tree, adaptor, builder).
+// Runs on the host between passes (the reference's recorder is host code too);
+// the GPU samples and evaluates the baked mixtures (guide_vmf.hpp).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <random>
+#include <utility>
+#include <vector>
+
+namespace ttgpu {
+namespace guide {
+
+constexpr int kLobes = 8;
+constexpr int kMaxTreeDepth = 20;  // path_guiding_tree.hxx:20
+constexpr float kKappaMin = 1.e-4f, kKappaMax = 40.f;  // distribution_mixture_models.hxx:13-15
+constexpr float kEps = 1.e-38f;
+
+struct Vmf {
+  float mu[kLobes][3];
+  float kappa[kLobes];
+  float w[kLobes];
+};
+
+// InitializeForUnitSphere<8> (distribution_mixture_models.cxx:167-183): fixed lobe directions,
+// concentration 2, equal weights.
+void InitUnitSphere(Vmf &m);
+float ComponentPdf(const float mu[3], float k, const float x[3]);
+float Pdf(const Vmf &m, const float x[3]);
+// Sample(mixture, rs) (distribution_mixture_models.cxx:116-150).
+void Sample(const Vmf &m, double r0, double r1, double r2, float out[3]);
+float MeanCosineToConc(float r);
+
+// Sufficient statistics of the incremental EM (incremental::Data), as running means.
+struct FitData {
+  double pos[kLobes][3] = {};
+  double resp[kLobes] = {};
+  int64_t count = 0;        // samples since the last M-step
+  double weight_mean = 0.;  // mean record weight over all samples (never reset)
+  int64_t weight_count = 0;
+};
+
+struct FitParams {
+  float prior_nu = 100.f, prior_tau = 100.f, prior_alpha = 100.f;
+  int m_step_every = 10;
+  const Vmf *prior = nullptr;
+};
+
+// One weighted sample of incremental::Fit (distribution_mixture_models.cxx:247-343).
+void FitSample(Vmf &m, FitData &d, const FitParams &p, const float x[3], float weight);
+void MaximizationStep(Vmf &m, const FitData &d, const FitParams &p);
+
+// Online mean and covariance of 3-d points (Accumulators::OnlineCovariance).
+struct OnlineCov3 {
+  int64_t n = 0;
+  double mean[3] = {0, 0, 0};
+  double c[3][3] = {};
+  void Add(const double p[3]);
+  void Cov(double out[3][3]) const;
+  void Var(double out[3]) const;
+};
+
+// Radiance record (IncidentRadiance, path_guiding.hxx:49-55).
+struct Record {
+  double pos[3];
+  float dir[3];
+  float weight;
+  bool original;
+};
+
+// kd-tree over space; leaves are cells.  A child handle >= 0 is a branch index, < 0 is leaf ~idx.
+struct KdTree {
+  struct Node {
+    int axis;
+    double pos;
+    int left, right;
+  };
+  std::vector<Node> nodes;
+  int root = ~0;
+  int num_leafs = 1;
+  int Lookup(const double p[3]) const;
+};
+
+// kdtree::Builder (path_guiding_tree.hxx:300-438): median split along the largest extent while
+// both halves keep >= min_points; `order` receives the point indices in leaf order and
+// `leaf_ranges` the [begin, end) range of each leaf.
+KdTree BuildTree(const std::vector<std::array<double, 3>> &pts, int min_points, int max_depth, std::vector<int> &order,
+                 std::vector<std::pair<int, int>> &leaf_ranges);
+// kdtree::TreeAdaptor (path_guiding_tree.hxx:160-238): every leaf for which `decide` returns an
+// axis >= 0 is split at the returned position (unless at max depth).  `mapping[old_leaf]` receives
+// the new leaf indices (second = -1 when not split).
+KdTree AdaptTree(const KdTree &t, const std::function<std::pair<int, double>(int)> &decide,
+                 std::vector<std::pair<int, int>> &mapping);
+
+struct Box3 {
+  double lo[3], hi[3];
+};
+void LeafBoxes(const KdTree &t, const Box3 &root_box, std::vector<Box3> &out);
+
+struct CellData {
+  // current estimate (sampled by the renderer)
+  Vmf current;
+  double cov_frame[3][3] = {};  // eigenvectors scaled by the standard deviations
+  // learned
+  Vmf learned;
+  FitData fit;
+  OnlineCov3 leaf_stats;
+  int64_t last_num_samples = 0, max_num_samples = 0;
+  CellData() {
+    InitUnitSphere(current);
+    InitUnitSphere(learned);
+  }
+};
+
+// Recorder (PathGuiding, path_guiding.cxx:320-812).
+class Recorder {
+ public:
+  Recorder(int subdiv_factor, uint64_t seed, int threads);
+  // One 1-spp pass worth of records (BeginRound .. FinalizeRound).
+  void FinalizeRound(std::vector<Record> &records);
+  // After each doubling sweep: split cells and bake the learned mixtures (PrepareAdaptedStructures).
+  void PrepareAdaptedStructures();
+  const KdTree &tree() const { return tree_; }
+  const std::vector<CellData> &cells() const { return cells_; }
+  int round() const { return round_; }
+
+ private:
+  void AdaptInitial(std::vector<Record> &records);
+  void FitTheSamples(std::vector<Record> &records);
+  void AdaptIncremental();
+  FitParams ParamsOf(const CellData &c) const;
+  void ParallelFor(int n, const std::function<void(int, int)> &fn);
+
+  KdTree tree_;
+  std::vector<CellData> cells_;
+  Vmf prior_;
+  int subdiv_factor_;
+  int round_ = 0;
+  int64_t previous_max_samples_per_cell_ = 0, previous_total_samples_ = 0;
+  uint64_t seed_;
+  int threads_;
+};
+
+// Symmetric 3x3 eigen decomposition (cyclic Jacobi): columns of `vec` are eigenvectors.
+void SymEig3(const double a[3][3], double val[3], double vec[3][3]);
+
+}  // namespace guide
+}  // namespace ttgpu

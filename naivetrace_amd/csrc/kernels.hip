@@ -10,6 +10,7 @@
 #include "shading.hpp"
 #include "trace.hpp"
 #include "media.hpp"
+#include "guide_vmf.hpp"
 #include "kernels.hpp"
 
 namespace ttgpu {
@@ -725,6 +726,208 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DeviceScene sc, Pa
   if (gtid() == 0) atomicAdd(stats + kStatShadow, count);
 }
 
+// ---- guided integrator (ptg): BSDF/vMF one-sample mixture, power-heuristic MIS, records ------
+// Radiance records follow PropagateIlluminationToParents / AddToTrainingData
+// (src/renderingalgorithms_pathtracing_guided.cxx:1968-2034): the record of vertex j holds the
+// radiance arriving from the direction sampled at j, i.e. the emission and NEE of vertex j+1 plus
+// everything found later, carried back with throughput ratios clamped at 10; weight = radiance / pdf.
+__device__ __forceinline__ float mean4(float4 v) { return 0.25f * (v.x + v.y + v.z + v.w); }
+__device__ __forceinline__ float clamp_ratio(float a, float b) { return b > 0.f ? fminf(a / b, 10.f) : 0.f; }
+
+// Radiance leaving vertex v towards vertex v-1: `direct` is recorded at v-1 as it is, `weighted`
+// (emission with its MIS weight, plus NEE) is carried to the vertices before v-1.
+__device__ __forceinline__ void propagate_radiance(const PathQueues &q, uint32_t i, int v, float4 direct, float4 weighted,
+                                                   float4 thru_prev) {
+  if (v <= 0) return;
+  const size_t cap = (size_t)q.capacity;
+  q.gv_dir_acc[(size_t)(v - 1) * cap + i].w += mean4(direct);
+  for (int j = 0; j < v - 1; ++j) {
+    const float4 tj = q.gv_thru[(size_t)j * cap + i];
+    const float4 r = make_float4(clamp_ratio(thru_prev.x, tj.x), clamp_ratio(thru_prev.y, tj.y), clamp_ratio(thru_prev.z, tj.z),
+                                 clamp_ratio(thru_prev.w, tj.w));
+    q.gv_dir_acc[(size_t)j * cap + i].w += mean4(r * weighted);
+  }
+}
+
+__device__ __forceinline__ void emit_records(const PathQueues &q, uint32_t i, int nv) {
+  const size_t cap = (size_t)q.capacity;
+  for (int j = 0; j < nv; ++j) {
+    const float4 pp = q.gv_pos_pdf[(size_t)j * cap + i];
+    if (!(pp.w > 0.f)) continue;
+    const float4 da = q.gv_dir_acc[(size_t)j * cap + i];
+    const uint32_t slot = atomicAdd(q.counters + kCountRecords, 1u);
+    if (slot < q.rec_capacity) {
+      q.rec_pos_w[slot] = make_float4(pp.x, pp.y, pp.z, da.w / pp.w);
+      q.rec_dir[slot] = make_float4(da.x, da.y, da.z, 0.f);
+    } else {
+      atomicAdd(q.counters + kStatRecordsDropped, 1u);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
+                                                      const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
+                                                      uint32_t *count_out, double *fb, double *fb_sq) {
+  const uint32_t count = *count_in;
+  const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
+  const bool use_emission = sc.sample_mode != TTGPU_SAMPLE_LIGHTS;
+  const bool mis = use_nee && use_emission;
+  const bool rec = sc.guide_record != 0;
+  const size_t cap = (size_t)q.capacity;
+  const uint32_t n_iter = (count + grid_stride() - 1) / grid_stride();
+  for (uint32_t it = 0; it < n_iter; ++it) {
+    const uint32_t k = it * grid_stride() + gtid();
+    const bool active = k < count;
+    bool push = false, push_shadow = false;
+    uint32_t i = 0;
+    if (active) {
+      i = queue_in[k];
+      const float4 hitv = q.hit[i];
+      const float3 dir = xyz(q.dir[i]);
+      const float4 org4 = q.org[i];
+      const float lambda0 = org4.w;
+      float4 thru = q.thru[i];
+      float4 rad = q.rad[i];
+      const uint32_t bins = q.bins[i];
+      uint32_t flags = q.flags[i];
+      const int nv = (int)((flags & kFlagVertexMask) >> kFlagVertexShift);
+      const float last_pdf = q.last_pdf[i];
+      const bool mis_last = (flags & kFlagHasLastPdf) && !(flags & kFlagLastPdfDelta) && mis;
+      Pcg rng = load_rng(q, i);
+      const int prim = as_int(hitv.w);
+      if (prim < 0) {
+        // AddEnvEmission with the power heuristic
+        if (use_emission && sc.num_envs > 0) {
+          float penv;
+          const float4 L = env_eval(sc, -dir, bins, &penv);
+          const float w = mis_last ? power_heuristic(last_pdf, penv * sc.type_probs[0]) : 1.f;
+          rad = add_rgb(rad, spectral_to_rgb(w * (L * thru), bins));
+          if (rec) propagate_radiance(q, i, nv, L, w * L, thru);
+        }
+        if (rec) emit_records(q, i, nv);
+        finalize_path(q, fb, fb_sq, i, rad);
+      } else {
+        const SurfHit h = make_hit(sc, prim, hitv.y, hitv.z, hitv.x, xyz(org4), dir);
+        const DevMaterial mat = sc.materials[h.material];
+        const GuideCell &cell = sc.guide_cells[guide_lookup(sc, h.pos)];
+        if (use_emission && mat.emitter >= 0) {
+          const DevEmitter &em = sc.emitters[mat.emitter];
+          if (em.kind == TTGPU_AREA_UNIFORM && dot(h.ng, -dir) > 0.f) {
+            const float4 L = take(em.radiance, bins);
+            const float p_light = sc.area_pick_prob / sc.prim_area[prim] * area_to_solid_angle(hitv.x, dir, h.n);
+            const float w = mis_last ? power_heuristic(last_pdf, p_light) : 1.f;
+            rad = add_rgb(rad, spectral_to_rgb(w * (L * thru), bins));
+            if (rec) propagate_radiance(q, i, nv, L, w * L, thru);
+          }
+        }
+        const DevShader &shd = sc.shaders[mat.shader];
+        float rough = q.roughness[i];
+        const float3 wi = -dir;
+        // NEE with the selection probability in the light pdf and the power heuristic against the
+        // BSDF/guide mixture.
+        if (use_nee && (sc.num_area_lights + sc.num_envs + sc.num_points) > 0) {
+          const LightSample ls = sample_light(sc, h.pos, h.n, bins, rng);
+          if (ls.valid) {
+            float bsdf_pdf;
+            const float4 f = eval_bsdf(shd, h, wi, bins, rough, lambda0, ls.dir, &bsdf_pdf);
+            const float dfac = fabsf(dot(h.ns, ls.dir));
+            // The guided integrator always runs with TTGPU_MIS_CONSISTENT, so ls.pdf includes the
+            // light-selection probability.
+            float w = 1.f;
+            if (!ls.delta && mis) w = power_heuristic(ls.pdf, guided_pdf(shd, cell, ls.dir, bsdf_pdf));
+            const float4 nee = (w * dfac) * (f * ls.weight);
+            const float3 crgb = spectral_to_rgb(thru * nee, bins);
+            if (crgb.x != 0.f || crgb.y != 0.f || crgb.z != 0.f) {
+              const float3 o = offset_origin(h, ls.dir);
+              q.sh_org[i] = make_float4(o.x, o.y, o.z, ls.len * 0.9999f);
+              q.sh_dir[i] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, as_float(nv));
+              q.sh_contrib[i] = make_float4(crgb.x, crgb.y, crgb.z, 0.f);
+              if (rec) q.sh_nee[i] = nee;
+              push_shadow = true;
+            }
+          }
+        }
+        rough = fmaxf(my_roughness(shd), rough);
+        const Scatter s = sample_guided(shd, cell, h, wi, bins, rough, lambda0, rng);
+        const uint32_t node = (flags & kFlagNodeMask) + 1u;
+        bool alive = node <= (uint32_t)sc.max_depth;
+        if (alive) {
+          thru = (fabsf(dot(h.ns, s.dir)) / s.pdf) * (thru * s.value);
+          alive = !all_zero(thru) && isfinite(thru.x + thru.y + thru.z + thru.w);
+        }
+        int nv_new = nv;
+        if (rec && nv < q.max_vertices) {
+          // Vertex nv: where the path scattered, into which direction, with which pdf.
+          q.gv_pos_pdf[(size_t)nv * cap + i] = make_float4(h.pos.x, h.pos.y, h.pos.z, (alive && !s.delta) ? s.pdf : -1.f);
+          q.gv_dir_acc[(size_t)nv * cap + i] = make_float4(s.dir.x, s.dir.y, s.dir.z, 0.f);
+          q.gv_thru[(size_t)nv * cap + i] = alive ? thru : f4(0.f);
+          nv_new = nv + 1;
+        }
+        flags = (flags & ~kFlagVertexMask) | ((uint32_t)nv_new << kFlagVertexShift);
+        if (alive) {
+          const float3 o = offset_origin(h, s.dir);
+          q.org[i] = make_float4(o.x, o.y, o.z, lambda0);
+          q.dir[i] = make_float4(s.dir.x, s.dir.y, s.dir.z, 0.f);
+          q.thru[i] = thru;
+          q.last_pdf[i] = s.pdf;
+          q.roughness[i] = rough;
+          flags = (flags & ~(kFlagNodeMask | kFlagLastPdfDelta)) | node | kFlagHasLastPdf | (s.delta ? kFlagLastPdfDelta : 0u);
+          push = true;
+        } else if (push_shadow) {
+          flags |= kFlagFinalizeAfterShadow;
+        } else {
+          if (rec) emit_records(q, i, nv_new);
+          finalize_path(q, fb, fb_sq, i, rad);
+        }
+        q.flags[i] = flags;
+        q.rad[i] = rad;
+        store_rng(q, i, rng);
+      }
+    }
+    const uint32_t slot = wave_append(count_out, push);
+    if (push) queue_out[slot] = i;
+    const uint32_t sslot = wave_append(q.counters + kCountShadow, push_shadow);
+    if (push_shadow) q.shadow_queue[sslot] = i;
+  }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_shadow_guided(DeviceScene sc, PathQueues q, double *fb, double *fb_sq, uint32_t *stats) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  const uint32_t count = q.counters[kCountShadow];
+  const bool rec = sc.guide_record != 0;
+  const size_t cap = (size_t)q.capacity;
+  for (uint32_t k = gtid(); k < count; k += grid_stride()) {
+    const uint32_t i = q.shadow_queue[k];
+    const float4 o = q.sh_org[i];
+    const float4 d4 = q.sh_dir[i];
+    Ray r;
+    r.org = xyz(o);
+    r.dir = xyz(d4);
+    r.tmax = o.w;
+    HitRec h;
+    const bool occ = traverse<true>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, h);
+    float4 rad = q.rad[i];
+    const uint32_t flags = q.flags[i];
+    if (!occ) {
+      rad = add_rgb(rad, xyz(q.sh_contrib[i]));
+      if (rec) {
+        const int v = as_int(d4.w);
+        if (v > 0) {
+          const float4 nee = q.sh_nee[i];
+          propagate_radiance(q, i, v, nee, nee, q.gv_thru[(size_t)(v - 1) * cap + i]);
+        }
+      }
+    }
+    if (flags & kFlagFinalizeAfterShadow) {
+      if (rec) emit_records(q, i, (int)((flags & kFlagVertexMask) >> kFlagVertexShift));
+      finalize_path(q, fb, fb_sq, i, rad);
+    } else {
+      q.rad[i] = rad;
+    }
+  }
+  if (gtid() == 0) atomicAdd(stats + kStatShadow, count);
+}
+
 // ---- debug / test kernels -------------------------------------------------------------
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DeviceScene sc, int64_t n, const float *org, const float *dir, const float *tmax,
                                                             float *t, int32_t *prim, float *u, float *v, int32_t *occl, int brute) {
@@ -810,6 +1013,20 @@ __global__ void k_sample_phase(int kind, float g, float3 wi, uint64_t seed, int6
   }
 }
 
+__global__ void k_guide_sample(GuideCell c, uint64_t seed, int64_t n, float *out_dir, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) {
+    Pcg rng;
+    rng.seed(seed, (uint64_t)i);
+    const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+    const float3 d = guide_sample(c, u0, u1, u2);
+    out_dir[3 * i] = d.x; out_dir[3 * i + 1] = d.y; out_dir[3 * i + 2] = d.z;
+    out_pdf[i] = guide_pdf(c, d);
+  }
+}
+__global__ void k_guide_pdf(GuideCell c, int64_t n, const float *dirs, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) out_pdf[i] = guide_pdf(c, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
 // ---- host-side launchers ------------------------------------------------------------------
 static int g_trace_blocks = 0;
 
@@ -845,7 +1062,10 @@ void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
 }
 void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
                  uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
-  if (sc.has_media)
+  if (sc.guide_cells)
+    hipLaunchKernelGGL(k_shade_guided, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out,
+                       count_out, fb, fb_sq);
+  else if (sc.has_media)
     hipLaunchKernelGGL(k_shade_media, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, queue_in, count_in,
                        queue_out, count_out, fb, fb_sq);
   else
@@ -853,7 +1073,10 @@ void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
                        fb, fb_sq);
 }
 void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
-  if (sc.has_media)
+  if (sc.guide_cells)
+    hipLaunchKernelGGL(k_shadow_guided, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq,
+                       q.counters);
+  else if (sc.has_media)
     hipLaunchKernelGGL(k_shadow_media, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq,
                        q.counters);
   else
@@ -871,6 +1094,12 @@ void LaunchSampleShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uin
 void LaunchEvalShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, int64_t n, const float *dirs,
                       float *v, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_eval_shader, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, s, wi, ng, ns, bins, rough, n, dirs, v, p);
+}
+void LaunchGuideSample(const GuideCell &c, uint64_t seed, int64_t n, float *d, float *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_guide_sample, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, c, seed, n, d, p);
+}
+void LaunchGuidePdf(const GuideCell &c, int64_t n, const float *dirs, float *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_guide_pdf, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, c, n, dirs, p);
 }
 void LaunchSamplePhase(int kind, float g, float3 wi, uint64_t seed, int64_t n, float *d, float *v, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_sample_phase, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, kind, g, wi, seed, n, d, v, p);

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -18,6 +19,7 @@
 #include "../../include/ttgpu.h"
 #include "bvh_build.hpp"
 #include "device_scene.hpp"
+#include "guiding.hpp"
 #include "kernels.hpp"
 
 #include "spectral_tables.inc"
@@ -68,7 +70,8 @@ struct ttgpu_handle {
   std::vector<void *> allocs;
   double *fb = nullptr, *fb_sq = nullptr;
   std::vector<uint64_t> pixel_spp;
-  uint64_t total_spp = 0;
+  uint64_t total_spp = 0;    // all samples per pixel rendered (the RNG sample index base)
+  uint64_t sched_total = 0;  // samples per pixel of the current schedule (capped by max spp)
   int sched_spp = 1;
   std::atomic<bool> stop{false}, interrupt{false};
   ttgpu_pass_callback cb = nullptr;
@@ -88,6 +91,11 @@ struct ttgpu_handle {
   // host copies needed later
   std::vector<float> host_area;
   int num_triangles = 0;
+  // guided integrator
+  bool guided = false, trained = false;
+  std::unique_ptr<guide::Recorder> recorder;
+  void *d_guide_nodes = nullptr, *d_guide_cells = nullptr;
+  int64_t records_last_pass = 0, records_dropped = 0;
 
   template <class T> T *Alloc(size_t n) {
     void *p = nullptr;
@@ -134,10 +142,12 @@ struct ttgpu_handle {
     event_next = 0;
   }
   ~ttgpu_handle() {
-    if (stream) hipStreamSynchronize(stream);
-    for (auto e : event_pool) hipEventDestroy(e);
-    for (void *p : allocs) hipFree(p);
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : event_pool) (void)hipEventDestroy(e);
+    for (void *p : allocs) (void)hipFree(p);
+    if (d_guide_nodes) (void)hipFree(d_guide_nodes);
+    if (d_guide_cells) (void)hipFree(d_guide_cells);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -166,6 +176,9 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (p.max_ray_depth < 2 || p.max_ray_depth > 255) bad("max_ray_depth must be in [2, 255]");
   if (p.num_wavelengths != 0 && p.num_wavelengths != 4) bad("the GPU path traces 4 wavelengths per path");
   if (p.sample_mode < 0 || p.sample_mode > 2) bad("bad sample_mode");
+  if (p.algo != TTGPU_ALGO_PT2 && p.algo != TTGPU_ALGO_PTG) bad("unknown algorithm");
+  if (p.algo == TTGPU_ALGO_PTG && p.guide_kind == TTGPU_GUIDE_LEARNED)
+    bad("learned guiding is not available in this build; use TTGPU_GUIDE_VMF_TREE");
   if (s.num_triangles < 0 || s.num_spheres < 0 || s.num_vertices < 0) bad("negative counts");
   if (s.num_triangles > 0 && (!s.vertices || !s.indices || !s.tri_material)) bad("triangles need vertices, indices, materials");
   if (s.num_spheres > 0 && (!s.spheres || !s.sphere_material)) bad("spheres need data and materials");
@@ -195,6 +208,11 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (s.num_media > kMaxMedia) bad("too many media");
   if (s.num_media > 0 && !s.media) bad("media table missing");
   if (s.empty_space_medium >= s.num_media || s.empty_space_medium < -1) bad("empty_space_medium out of range");
+  if (p.algo == TTGPU_ALGO_PTG) {
+    bool any_medium = s.empty_space_medium >= 0;
+    for (int i = 0; i < s.num_materials; ++i) any_medium |= s.materials[i].medium >= 0 || s.materials[i].outer_medium >= 0;
+    if (any_medium) bad("the guided integrator guides surface scattering only; scenes with media use pt2");
+  }
   for (int i = 0; i < s.num_media; ++i) {
     const auto &m = s.media[i];
     if (m.kind < TTGPU_MEDIUM_VACUUM || m.kind > TTGPU_MEDIUM_ATMOSPHERE) bad("unknown medium kind");
@@ -593,6 +611,22 @@ void AllocateQueues(ttgpu_handle *h) {
     q.sh_aux = nullptr;
   }
   h->stats.state_bytes = (size_t)N * (16 * 8 + 4 * 6 + 8 * 3 + 12 + (h->dsc.has_media ? 48 : 0));
+  q.gv_pos_pdf = q.gv_dir_acc = q.gv_thru = q.sh_nee = q.rec_pos_w = q.rec_dir = nullptr;
+  q.rec_capacity = 0;
+  q.max_vertices = 0;
+  if (h->guided) {
+    // Per-vertex state of the guided integrator: a path scatters at most max_depth - 1 times.
+    const int nvtx = std::max(1, h->params.max_ray_depth - 1);
+    q.max_vertices = nvtx;
+    q.gv_pos_pdf = h->Alloc<float4>((size_t)N * nvtx);
+    q.gv_dir_acc = h->Alloc<float4>((size_t)N * nvtx);
+    q.gv_thru = h->Alloc<float4>((size_t)N * nvtx);
+    q.sh_nee = h->Alloc<float4>(N);
+    q.rec_capacity = (uint32_t)std::min<uint64_t>((uint64_t)h->npix * nvtx, (uint64_t)1 << 25);
+    q.rec_pos_w = h->Alloc<float4>(q.rec_capacity);
+    q.rec_dir = h->Alloc<float4>(q.rec_capacity);
+    h->stats.state_bytes += (size_t)N * (48 * nvtx + 16) + (size_t)q.rec_capacity * 32;
+  }
   const size_t npix = (size_t)h->npix;
   h->fb = h->Alloc<double>(3 * npix);
   h->fb_sq = h->Alloc<double>(3 * npix);
@@ -600,6 +634,64 @@ void AllocateQueues(ttgpu_handle *h) {
   LaunchFill(h->fb_sq, 3 * npix, 0.0, h->stream);
   Check(hipGetLastError(), "fill framebuffer");
   Check(hipStreamSynchronize(h->stream), "sync");
+}
+
+// Bakes the recorder's kd-tree and current mixtures into the device layout (GuideNode, GuideCell).
+void UploadGuide(ttgpu_handle *h) {
+  const guide::KdTree &t = h->recorder->tree();
+  const auto &cells = h->recorder->cells();
+  std::vector<GuideNode> nodes(std::max<size_t>(1, t.nodes.size()));
+  for (size_t k = 0; k < t.nodes.size(); ++k) {
+    const auto &n = t.nodes[k];
+    nodes[k] = {(float)n.pos, n.axis, n.left, n.right};
+  }
+  std::vector<GuideCell> dc(std::max<size_t>(1, cells.size()));
+  for (size_t c = 0; c < cells.size(); ++c) {
+    const guide::Vmf &m = cells[c].current;
+    double acc = 0.;
+    for (int k = 0; k < kGuideLobes; ++k) {
+      const double kap = m.kappa[k];
+      dc[c].lobe[k] = make_float4(m.mu[k][0], m.mu[k][1], m.mu[k][2], m.kappa[k]);
+      dc[c].wn[k] = (float)(m.w[k] * kap / (2. * kPiD * -std::expm1(-2. * kap)));
+      acc += m.w[k];
+      dc[c].cdf[k] = (float)acc;
+    }
+    dc[c].cdf[kGuideLobes - 1] = 2.f;  // every u in [0, 1) selects a lobe
+  }
+  if (h->d_guide_nodes) Check(hipFree(h->d_guide_nodes), "hipFree");
+  if (h->d_guide_cells) Check(hipFree(h->d_guide_cells), "hipFree");
+  h->d_guide_nodes = h->d_guide_cells = nullptr;
+  Check(hipMalloc(&h->d_guide_nodes, nodes.size() * sizeof(GuideNode)), "hipMalloc guide");
+  Check(hipMalloc(&h->d_guide_cells, dc.size() * sizeof(GuideCell)), "hipMalloc guide");
+  Check(hipMemcpy(h->d_guide_nodes, nodes.data(), nodes.size() * sizeof(GuideNode), hipMemcpyHostToDevice), "upload guide");
+  Check(hipMemcpy(h->d_guide_cells, dc.data(), dc.size() * sizeof(GuideCell), hipMemcpyHostToDevice), "upload guide");
+  h->dsc.guide_nodes = (const GuideNode *)h->d_guide_nodes;
+  h->dsc.guide_cells = (const GuideCell *)h->d_guide_cells;
+  h->dsc.guide_root = t.root;
+}
+
+// Downloads the records of the pass just rendered and hands them to the recorder.
+void CollectRecords(ttgpu_handle *h) {
+  uint32_t counters[kNumCounters];
+  Check(hipMemcpy(counters, h->q.counters, sizeof(counters), hipMemcpyDeviceToHost), "read counters");
+  const uint32_t n = std::min(counters[kCountRecords], h->q.rec_capacity);
+  h->records_last_pass = n;
+  h->records_dropped += counters[kStatRecordsDropped];
+  std::vector<float4> pw(n), dr(n);
+  if (n) {
+    Check(hipMemcpy(pw.data(), h->q.rec_pos_w, n * sizeof(float4), hipMemcpyDeviceToHost), "records");
+    Check(hipMemcpy(dr.data(), h->q.rec_dir, n * sizeof(float4), hipMemcpyDeviceToHost), "records");
+  }
+  std::vector<guide::Record> recs(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    guide::Record &r = recs[k];
+    r.pos[0] = pw[k].x; r.pos[1] = pw[k].y; r.pos[2] = pw[k].z;
+    r.weight = std::isfinite(pw[k].w) ? pw[k].w : 0.f;
+    const float len = std::sqrt(dr[k].x * dr[k].x + dr[k].y * dr[k].y + dr[k].z * dr[k].z);
+    r.dir[0] = dr[k].x / len; r.dir[1] = dr[k].y / len; r.dir[2] = dr[k].z / len;
+    r.original = true;
+  }
+  h->recorder->FinalizeRound(recs);
 }
 
 int Fail(ttgpu_handle *h, int code, const std::string &msg) {
@@ -620,6 +712,14 @@ int Fail(ttgpu_handle *h, int code, const std::string &msg) {
     return Fail(h, TTGPU_ERR_RUNTIME, e.what());                          \
   }
 
+void ResetFramebuffer(ttgpu_handle *h) {
+  LaunchFill(h->fb, 3 * (size_t)h->npix, 0.0, h->stream);
+  LaunchFill(h->fb_sq, 3 * (size_t)h->npix, 0.0, h->stream);
+  Check(hipGetLastError(), "fill");
+  Check(hipStreamSynchronize(h->stream), "sync");
+  std::fill(h->pixel_spp.begin(), h->pixel_spp.end(), 0);
+}
+
 // Renders `spp` samples per pixel; returns false if a stop cut the pass short.
 bool RenderPass(ttgpu_handle *h, int spp) {
   const uint64_t npix = (uint64_t)h->npix;
@@ -631,6 +731,10 @@ bool RenderPass(ttgpu_handle *h, int spp) {
   hipEvent_t ev0 = h->NextEvent(), ev1 = h->NextEvent();
   Check(hipEventRecord(ev0, h->stream), "hipEventRecord");
   bool complete = true;
+  if (h->dsc.guide_record) {
+    LaunchResetCounter(q.counters, kCountRecords, h->stream);
+    LaunchResetCounter(q.counters, kStatRecordsDropped, h->stream);
+  }
   for (uint64_t base = 0; base < total; base += cap) {
     if (h->stop.load()) { complete = false; break; }
     if (h->interrupt.exchange(false)) {
@@ -673,7 +777,10 @@ bool RenderPass(ttgpu_handle *h, int spp) {
   Check(hipMemcpy(counters, q.counters, sizeof(counters), hipMemcpyDeviceToHost), "read counters");
   h->stats.rays_closest = counters[kStatClosest];
   h->stats.rays_shadow = counters[kStatShadow];
-  if (complete) h->total_spp += spp;
+  if (complete) {
+    h->total_spp += spp;
+    h->sched_total += spp;
+  }
   h->stats.total_spp = h->total_spp;
   h->stats.passes += 1;
   return complete;
@@ -745,11 +852,23 @@ int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_han
     double diag = nprim ? std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]))
                         : 1.0;
     BuildTables(h, s, diag);
+    h->guided = params->algo == TTGPU_ALGO_PTG;
+    if (h->guided) {
+      // The guided integrator's MIS (power heuristic) always includes the light-selection
+      // probability in the NEE pdf.
+      h->params.mis_mode = TTGPU_MIS_CONSISTENT;
+      h->dsc.mis_mode = TTGPU_MIS_CONSISTENT;
+    }
     int64_t cap = params->wavefront_paths > 0 ? params->wavefront_paths : (int64_t)1 << 22;
     cap = std::min<int64_t>(cap, (int64_t)1 << 26);
     h->capacity = (uint32_t)cap;
     h->stats.wavefront_paths = (int32_t)cap;
     AllocateQueues(h);
+    if (h->guided) {
+      const int subdiv = params->guide_subdiv_factor > 0 ? params->guide_subdiv_factor : 100;
+      h->recorder = std::make_unique<guide::Recorder>(subdiv, params->seed, params->num_threads);
+      UploadGuide(h);
+    }
     auto t1 = std::chrono::steady_clock::now();
     h->stats.upload_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() - h->stats.bvh_build_ms;
     *out = h;
@@ -804,14 +923,42 @@ int ttgpu_run(ttgpu_handle *h, int32_t max_passes) {
     constexpr int kMaxSppPerIteration = 1024;
     const int max_spp = h->params.max_samples_per_pixel;
     int passes = 0;
+    if (h->guided && !h->trained) {
+      // Training (guided PathTracingAlgo2::Run, src/renderingalgorithms_pathtracing_guided.cxx:1110-1160):
+      // an initial 1-spp pass builds the kd-tree, then sweeps of 2, 4, .. 1-spp passes refit the
+      // cells after every pass and split cells / bake mixtures after every sweep.
+      h->dsc.guide_record = 1;
+      if (!h->stop.load()) {
+        RenderPass(h, 1);
+        CollectRecords(h);
+        UploadGuide(h);
+        if (h->cb) h->cb(h->cb_user, 1);
+      }
+      for (int num = 2; !h->stop.load() && num <= h->params.guide_max_spp; num *= 2) {
+        ResetFramebuffer(h);
+        for (int k = 0; k < num && !h->stop.load(); ++k) {
+          RenderPass(h, 1);
+          CollectRecords(h);
+        }
+        if (h->cb) h->cb(h->cb_user, 1);
+        h->recorder->PrepareAdaptedStructures();
+        UploadGuide(h);
+      }
+      h->dsc.guide_record = 0;
+      if (h->stop.load()) return TTGPU_ERR_INTERRUPTED;
+      ResetFramebuffer(h);
+      h->trained = true;
+      h->sched_total = 0;
+      h->sched_spp = 1;
+    }
     // SamplesPerPixelSchedule (src/renderingalgorithms_simplebase.hxx:34-62).
-    if (max_spp > 0 && (int64_t)h->total_spp + h->sched_spp > max_spp) h->sched_spp = max_spp - (int)h->total_spp;
+    if (max_spp > 0 && (int64_t)h->sched_total + h->sched_spp > max_spp) h->sched_spp = max_spp - (int)h->sched_total;
     while (!h->stop.load() && h->sched_spp > 0 && (max_passes <= 0 || passes < max_passes)) {
       RenderPass(h, h->sched_spp);
       ++passes;
       if (h->cb) h->cb(h->cb_user, 1);
       if (h->sched_spp < kMaxSppPerIteration) h->sched_spp *= 2;
-      if (max_spp > 0 && (int64_t)h->total_spp + h->sched_spp > max_spp) h->sched_spp = max_spp - (int)h->total_spp;
+      if (max_spp > 0 && (int64_t)h->sched_total + h->sched_spp > max_spp) h->sched_spp = max_spp - (int)h->sched_total;
     }
     return h->stop.load() ? TTGPU_ERR_INTERRUPTED : TTGPU_OK;
   })
@@ -820,11 +967,7 @@ int ttgpu_run(ttgpu_handle *h, int32_t max_passes) {
 int ttgpu_reset_framebuffer(ttgpu_handle *h) {
   if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
   TTGPU_GUARD(h, {
-    LaunchFill(h->fb, 3 * (size_t)h->npix, 0.0, h->stream);
-    LaunchFill(h->fb_sq, 3 * (size_t)h->npix, 0.0, h->stream);
-    Check(hipGetLastError(), "fill");
-    Check(hipStreamSynchronize(h->stream), "sync");
-    std::fill(h->pixel_spp.begin(), h->pixel_spp.end(), 0);
+    ResetFramebuffer(h);
     return TTGPU_OK;
   })
 }
@@ -911,7 +1054,7 @@ static int TraceClosestImpl(ttgpu_handle *h, int64_t count, const float *org, co
     } else {
       for (size_t i = 0; i < n; ++i) { t[i] = INFINITY; prim[i] = -1; u[i] = v[i] = 0.f; }
     }
-    hipFree(d_org); hipFree(d_dir); hipFree(d_tmax); hipFree(d_t); hipFree(d_u); hipFree(d_v); hipFree(d_prim);
+    (void)hipFree(d_org); (void)hipFree(d_dir); (void)hipFree(d_tmax); (void)hipFree(d_t); (void)hipFree(d_u); (void)hipFree(d_v); (void)hipFree(d_prim);
     return TTGPU_OK;
   })
 }
@@ -946,7 +1089,7 @@ int ttgpu_trace_occluded(ttgpu_handle *h, int64_t count, const float *org, const
     } else {
       for (size_t i = 0; i < n; ++i) occluded[i] = 0;
     }
-    hipFree(d_org); hipFree(d_dir); hipFree(d_tmax); hipFree(d_occ);
+    (void)hipFree(d_org); (void)hipFree(d_dir); (void)hipFree(d_tmax); (void)hipFree(d_occ);
     return TTGPU_OK;
   })
 }
@@ -976,15 +1119,15 @@ static int SamplerCall(int64_t count, size_t out_floats_per, std::function<void(
     Check(hipMemcpy(op, dp, n * 4, hipMemcpyDeviceToHost), "D2H");
   } catch (const std::exception &e) {
     g_create_error = e.what();
-    if (dd) hipFree(dd);
-    if (dv) hipFree(dv);
-    if (dp) hipFree(dp);
+    if (dd) (void)hipFree(dd);
+    if (dv) (void)hipFree(dv);
+    if (dp) (void)hipFree(dp);
     return TTGPU_ERR_DEVICE;
   }
   (void)out_floats_per;
-  if (dd) hipFree(dd);
-  hipFree(dv);
-  hipFree(dp);
+  if (dd) (void)hipFree(dd);
+  (void)hipFree(dv);
+  (void)hipFree(dp);
   return TTGPU_OK;
 }
 
@@ -1014,14 +1157,14 @@ int ttgpu_eval_shader(const ttgpu_shader *shader, const float wi[3], const float
   float *ddirs = nullptr;
   if (hipMalloc(&ddirs, 3 * (size_t)count * 4) != hipSuccess) return TTGPU_ERR_DEVICE;
   if (hipMemcpy(ddirs, dirs, 3 * (size_t)count * 4, hipMemcpyHostToDevice) != hipSuccess) {
-    hipFree(ddirs);
+    (void)hipFree(ddirs);
     return TTGPU_ERR_DEVICE;
   }
   int rc = SamplerCall(count, 5, [&](float *, float *dv, float *dp) {
     LaunchEvalShader(d, make_float3(wi[0], wi[1], wi[2]), make_float3(ng[0], ng[1], ng[2]), make_float3(ns[0], ns[1], ns[2]), pb,
                      roughness, count, ddirs, dv, dp, nullptr);
   }, nullptr, out_value, out_pdf, 0, 4);
-  hipFree(ddirs);
+  (void)hipFree(ddirs);
   return rc;
 }
 
@@ -1033,6 +1176,127 @@ int ttgpu_sample_phase(int32_t phase_kind, float g, const float wi[3], uint64_t 
   return SamplerCall(count, 5, [&](float *dd, float *dv, float *dp) {
     LaunchSamplePhase(phase_kind, g, make_float3(wi[0], wi[1], wi[2]), seed, count, dd, dv, dp, nullptr);
   }, out_dir, out_value, out_pdf, 3, 1);
+}
+
+int ttgpu_guide_info(ttgpu_handle *h, int32_t *cells, int32_t *rounds, int64_t *records_last_pass, int64_t *records_dropped) {
+  if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (!h->guided) return Fail(h, TTGPU_ERR_INVALID_ARGUMENT, "not a guided (TTGPU_ALGO_PTG) handle");
+  if (cells) *cells = (int32_t)h->recorder->cells().size();
+  if (rounds) *rounds = h->recorder->round();
+  if (records_last_pass) *records_last_pass = h->records_last_pass;
+  if (records_dropped) *records_dropped = h->records_dropped;
+  return TTGPU_OK;
+}
+
+static void ToVmf(const ttgpu_vmf8 &a, guide::Vmf &m) {
+  for (int k = 0; k < guide::kLobes; ++k) {
+    for (int c = 0; c < 3; ++c) m.mu[k][c] = a.mu[k][c];
+    m.kappa[k] = a.kappa[k];
+    m.w[k] = a.w[k];
+  }
+}
+static void FromVmf(const guide::Vmf &m, ttgpu_vmf8 &a) {
+  for (int k = 0; k < guide::kLobes; ++k) {
+    for (int c = 0; c < 3; ++c) a.mu[k][c] = m.mu[k][c];
+    a.kappa[k] = m.kappa[k];
+    a.w[k] = m.w[k];
+  }
+}
+
+int ttgpu_guide_init_vmf(ttgpu_vmf8 *out) {
+  if (!out) return TTGPU_ERR_INVALID_ARGUMENT;
+  guide::Vmf m;
+  guide::InitUnitSphere(m);
+  FromVmf(m, *out);
+  return TTGPU_OK;
+}
+
+int ttgpu_guide_fit_vmf(ttgpu_vmf8 *mix, const float *dirs, const float *weights, int64_t n, int64_t max_num_samples) {
+  if (!mix || (n > 0 && (!dirs || !weights)) || n < 0) return TTGPU_ERR_INVALID_ARGUMENT;
+  guide::Vmf m, prior;
+  ToVmf(*mix, m);
+  guide::InitUnitSphere(prior);
+  guide::FitParams p;
+  const double strength = std::max(1., 0.01 * (double)max_num_samples);
+  p.prior_alpha = p.prior_nu = p.prior_tau = (float)strength;
+  p.m_step_every = std::max(1, (int)strength * 10);
+  p.prior = &prior;
+  guide::FitData d;
+  for (int64_t i = 0; i < n; ++i)
+    if (weights[i] > 0.f) guide::FitSample(m, d, p, dirs + 3 * i, weights[i]);
+  FromVmf(m, *mix);
+  return TTGPU_OK;
+}
+
+int ttgpu_guide_build_kdtree(const double *pts, int64_t n, int32_t min_points, int32_t max_depth, int32_t *leaf_of_point,
+                             int32_t *num_leafs) {
+  if ((n > 0 && (!pts || !leaf_of_point)) || n < 0 || min_points < 1 || max_depth < 1 || !num_leafs)
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  std::vector<std::array<double, 3>> p(n);
+  for (int64_t i = 0; i < n; ++i) p[i] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  std::vector<int> order;
+  std::vector<std::pair<int, int>> ranges;
+  guide::KdTree t = guide::BuildTree(p, min_points, max_depth, order, ranges);
+  for (size_t l = 0; l < ranges.size(); ++l)
+    for (int k = ranges[l].first; k < ranges[l].second; ++k) leaf_of_point[order[k]] = (int32_t)l;
+  *num_leafs = t.num_leafs;
+  return TTGPU_OK;
+}
+
+static GuideCell BakeCell(const ttgpu_vmf8 &m) {
+  GuideCell c;
+  double acc = 0.;
+  for (int k = 0; k < kGuideLobes; ++k) {
+    c.lobe[k] = make_float4(m.mu[k][0], m.mu[k][1], m.mu[k][2], m.kappa[k]);
+    c.wn[k] = (float)(m.w[k] * (double)m.kappa[k] / (2. * kPiD * -std::expm1(-2. * (double)m.kappa[k])));
+    acc += m.w[k];
+    c.cdf[k] = (float)acc;
+  }
+  c.cdf[kGuideLobes - 1] = 2.f;
+  return c;
+}
+
+int ttgpu_guide_sample_vmf(const ttgpu_vmf8 *m, uint64_t seed, int64_t count, float *out_dir, float *out_pdf) {
+  if (!m || count < 0 || (count > 0 && (!out_dir || !out_pdf))) return TTGPU_ERR_INVALID_ARGUMENT;
+  const GuideCell c = BakeCell(*m);
+  if (count == 0) return TTGPU_OK;
+  float *dd = nullptr, *dp = nullptr;
+  if (hipMalloc(&dd, count * 3 * sizeof(float)) != hipSuccess) return TTGPU_ERR_DEVICE;
+  if (hipMalloc(&dp, count * sizeof(float)) != hipSuccess) {
+    (void)hipFree(dd);
+    return TTGPU_ERR_DEVICE;
+  }
+  int rc = TTGPU_OK;
+  LaunchGuideSample(c, seed, count, dd, dp, 0);
+  if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess) rc = TTGPU_ERR_DEVICE;
+  if (rc == TTGPU_OK && (hipMemcpy(out_dir, dd, count * 3 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+                         hipMemcpy(out_pdf, dp, count * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess))
+    rc = TTGPU_ERR_DEVICE;
+  (void)hipFree(dd);
+  (void)hipFree(dp);
+  return rc;
+}
+
+int ttgpu_guide_pdf_vmf(const ttgpu_vmf8 *m, int64_t count, const float *dirs, float *out_pdf) {
+  if (!m || count < 0 || (count > 0 && (!dirs || !out_pdf))) return TTGPU_ERR_INVALID_ARGUMENT;
+  const GuideCell c = BakeCell(*m);
+  float *dd = nullptr, *dp = nullptr;
+  if (count == 0) return TTGPU_OK;
+  if (hipMalloc(&dd, count * 3 * sizeof(float)) != hipSuccess) return TTGPU_ERR_DEVICE;
+  if (hipMalloc(&dp, count * sizeof(float)) != hipSuccess) {
+    (void)hipFree(dd);
+    return TTGPU_ERR_DEVICE;
+  }
+  int rc = TTGPU_OK;
+  if (hipMemcpy(dd, dirs, count * 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) rc = TTGPU_ERR_DEVICE;
+  if (rc == TTGPU_OK) {
+    LaunchGuidePdf(c, count, dd, dp, 0);
+    if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess) rc = TTGPU_ERR_DEVICE;
+  }
+  if (rc == TTGPU_OK && hipMemcpy(out_pdf, dp, count * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) rc = TTGPU_ERR_DEVICE;
+  (void)hipFree(dd);
+  (void)hipFree(dp);
+  return rc;
 }
 
 }  // extern "C"

@@ -117,6 +117,13 @@ class GpuRenderer:
         self._check(self._lib.ttgpu_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def guide_info(self):
+        """Guided handles: kd-tree cells, training rounds, records of the last pass, dropped records."""
+        cells, rounds = C.c_int32(), C.c_int32()
+        last, dropped = C.c_int64(), C.c_int64()
+        self._check(self._lib.ttgpu_guide_info(self._h, C.byref(cells), C.byref(rounds), C.byref(last), C.byref(dropped)))
+        return {"cells": cells.value, "rounds": rounds.value, "records_last_pass": last.value, "records_dropped": dropped.value}
+
     def trace_closest(self, org, direction, tmax, bruteforce=False):
         org = np.ascontiguousarray(org, dtype=np.float32)
         direction = np.ascontiguousarray(direction, dtype=np.float32)
@@ -173,6 +180,58 @@ def sample_phase(kind, g, wi, seed, count):
     if rc:
         raise TTGpuError(rc, lib.ttgpu_create_error().decode())
     return d, v, p
+
+
+def vmf_init():
+    """InitializeForUnitSphere<8> of the library's recorder."""
+    lib = abi.load_library()
+    m = abi.Vmf8()
+    lib.ttgpu_guide_init_vmf(C.byref(m))
+    return m
+
+
+def vmf_fit(mixture, dirs, weights, max_num_samples):
+    """Incremental EM of the recorder over weighted directions, continuing from `mixture` (in place)."""
+    lib = abi.load_library()
+    d = np.ascontiguousarray(dirs, dtype=np.float32)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    rc = lib.ttgpu_guide_fit_vmf(C.byref(mixture), _f32p(d), _f32p(w), len(w), int(max_num_samples))
+    if rc:
+        raise TTGpuError(rc, "ttgpu_guide_fit_vmf")
+    return mixture
+
+
+def build_kdtree(points, min_points, max_depth):
+    """Leaf index of every point and the number of leaves of the recorder's kd-tree builder."""
+    lib = abi.load_library()
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    leaf = np.zeros(len(p), np.int32)
+    n = C.c_int32()
+    rc = lib.ttgpu_guide_build_kdtree(p.ctypes.data_as(C.POINTER(C.c_double)), len(p), min_points, max_depth,
+                                      leaf.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n))
+    if rc:
+        raise TTGpuError(rc, "ttgpu_guide_build_kdtree")
+    return leaf, n.value
+
+
+def vmf_sample_gpu(mixture, seed, count):
+    lib = abi.load_library()
+    d = np.zeros((count, 3), np.float32)
+    p = np.zeros(count, np.float32)
+    rc = lib.ttgpu_guide_sample_vmf(C.byref(mixture), seed, count, _f32p(d), _f32p(p))
+    if rc:
+        raise TTGpuError(rc, "ttgpu_guide_sample_vmf")
+    return d, p
+
+
+def vmf_pdf_gpu(mixture, dirs):
+    lib = abi.load_library()
+    d = np.ascontiguousarray(dirs, dtype=np.float32)
+    p = np.zeros(len(d), np.float32)
+    rc = lib.ttgpu_guide_pdf_vmf(C.byref(mixture), len(d), _f32p(d), _f32p(p))
+    if rc:
+        raise TTGpuError(rc, "ttgpu_guide_pdf_vmf")
+    return p
 
 
 class RenderingAlgo:

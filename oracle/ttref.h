@@ -32,6 +32,12 @@ long ttref_bisection(const double *vals, long n, double r);
 double ttref_atmo_altitude(const ttgpu_medium *m, const double pos[3]);
 void ttref_atmo_lowest_point(const ttgpu_medium *m, const double org[3], const double dir[3], double len, double out[3]);
 void ttref_atmo_coeffs(const ttgpu_medium *m, double altitude, double sigma_s[36], double sigma_a[36]);
+/* guiding building blocks (ref_guiding.cpp) */
+void ttref_vmf_init(ttgpu_vmf8 *out);
+void ttref_vmf_pdf(const ttgpu_vmf8 *m, int64_t n, const double *dirs, double *out);
+void ttref_vmf_sample(const ttgpu_vmf8 *m, int64_t n, const double *u, double *out);
+void ttref_vmf_fit(ttgpu_vmf8 *m, const double *dirs, const double *weights, int64_t n, int64_t max_num_samples);
+int ttref_kdtree_build(const double *pts, int64_t n, int32_t min_points, int32_t max_depth, int32_t *leaf_of_point);
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,12 @@ def lib():
         L.ttref_atmo_altitude.restype = C.c_double
         L.ttref_atmo_lowest_point.argtypes = [P(abi.Medium), f64p, f64p, C.c_double, f64p]
         L.ttref_atmo_coeffs.argtypes = [P(abi.Medium), C.c_double, f64p, f64p]
+        L.ttref_vmf_init.argtypes = [P(abi.Vmf8)]
+        L.ttref_vmf_pdf.argtypes = [P(abi.Vmf8), C.c_int64, f64p, f64p]
+        L.ttref_vmf_sample.argtypes = [P(abi.Vmf8), C.c_int64, f64p, f64p]
+        L.ttref_vmf_fit.argtypes = [P(abi.Vmf8), f64p, f64p, C.c_int64, C.c_int64]
+        L.ttref_kdtree_build.argtypes = [f64p, C.c_int64, C.c_int32, C.c_int32, i32p]
+        L.ttref_kdtree_build.restype = C.c_int
         _lib = L
     return _lib
 
@@ -175,3 +181,37 @@ def atmo_coeffs(medium, altitude):
     sa = np.zeros(36)
     lib().ttref_atmo_coeffs(C.byref(medium), altitude, _p(ss), _p(sa))
     return ss, sa
+
+
+def vmf_init():
+    m = abi.Vmf8()
+    lib().ttref_vmf_init(C.byref(m))
+    return m
+
+
+def vmf_pdf(m, dirs):
+    d = np.ascontiguousarray(dirs, dtype=np.float64)
+    out = np.zeros(len(d))
+    lib().ttref_vmf_pdf(C.byref(m), len(d), _p(d), _p(out))
+    return out
+
+
+def vmf_sample(m, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros((len(u), 3))
+    lib().ttref_vmf_sample(C.byref(m), len(u), _p(u), _p(out))
+    return out
+
+
+def vmf_fit(m, dirs, weights, max_num_samples):
+    d = np.ascontiguousarray(dirs, dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    lib().ttref_vmf_fit(C.byref(m), _p(d), _p(w), len(w), int(max_num_samples))
+    return m
+
+
+def kdtree_build(points, min_points, max_depth):
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    leaf = np.zeros(len(p), np.int32)
+    n = lib().ttref_kdtree_build(_p(p), len(p), min_points, max_depth, _p(leaf, C.c_int32))
+    return leaf, n

@@ -353,6 +353,45 @@ int ttgpu_guide_build_kdtree(const double *pts, int64_t n, int32_t min_points, i
 int ttgpu_guide_sample_vmf(const ttgpu_vmf8 *m, uint64_t seed, int64_t count, float *out_dir, float *out_pdf);
 int ttgpu_guide_pdf_vmf(const ttgpu_vmf8 *m, int64_t count, const float *dirs, float *out_pdf);
 
+/* ---- learned guide (guide_kind TTGPU_GUIDE_LEARNED) ---------------------------
+ * A multiresolution hash-grid encoding of the position (levels x 2 features,
+ * 2^log2_table entries per level, resolutions base_res .. max_res geometric),
+ * the shading normal and the outgoing direction (38 inputs) feed an MLP
+ * 38 -> 64 -> 64 -> 64 -> 80 (ReLU) whose outputs are a 16-lobe vMF mixture:
+ * per lobe (logit, m.xyz, s) -> weight softmax(logit), mean m/|m|,
+ * concentration min(softplus(s) + 0.05, 100).  bf16 MFMA with fp32
+ * accumulation; trained online with Adam on the renderer's radiance records
+ * by minimising -(radiance / pdf) log q(direction).  No counterpart in the
+ * reference (new guiding method).
+ * Positions map to [0,1]^3 through lo' = bb_lo - 1e-3 ext, scale 1 / (1.002 ext).
+ * Parameter vector (fp32): grid [levels][2^log2_table][2] | W1T[64][64] | W2T[64][64] |
+ * W3T[64][64] | W4T[96][64] | b1[64] | b2[64] | b3[64] | b4[96]; WkT[out][in],
+ * input columns 38..63 and output rows 80..95 stay zero. */
+typedef struct ttgpu_nn_config {
+  int32_t levels;        /* 16 */
+  int32_t log2_table;    /* 19 */
+  float base_res;        /* 16 */
+  float max_res;         /* 1024 */
+  int32_t batch;         /* records per Adam step (65536) */
+  float lr;              /* Adam learning rate (1e-2) */
+  float bb_lo[3], bb_hi[3];
+  uint64_t seed;
+} ttgpu_nn_config;
+int64_t ttgpu_nn_param_count(const ttgpu_nn_config *cfg);
+/* Initial parameters as the renderer draws them from cfg->seed. */
+int ttgpu_nn_init_params(const ttgpu_nn_config *cfg, float *params);
+/* Raw outputs (n x 96, entries 80..95 zero) for n records: pos, nrm, wo are n x 3. */
+int ttgpu_nn_forward(const ttgpu_nn_config *cfg, const float *params, int64_t n, const float *pos, const float *nrm,
+                     const float *wo, float *out);
+/* Gradient of the mean over the n records of -weight log q(wi) with respect to every parameter,
+ * and the mean loss.  n <= cfg->batch. */
+int ttgpu_nn_gradient(const ttgpu_nn_config *cfg, const float *params, int64_t n, const float *pos, const float *nrm,
+                      const float *wo, const float *wi, const float *weight, float *grad, float *loss);
+/* `passes` training passes over the records (as the renderer trains after each pass);
+ * params in/out; loss_per_pass (optional) receives each pass's mean loss. */
+int ttgpu_nn_train(const ttgpu_nn_config *cfg, float *params, int64_t n, const float *pos, const float *nrm, const float *wo,
+                   const float *wi, const float *weight, int32_t passes, float *loss_per_pass);
+
 /* Device / build information string ("gfx950, 256 CUs, ..."). */
 const char *ttgpu_device_info(void);
 int ttgpu_abi_version(void);

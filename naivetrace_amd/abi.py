@@ -105,6 +105,22 @@ class Vmf8(C.Structure):
     _fields_ = [("mu", (C.c_float * 3) * 8), ("kappa", C.c_float * 8), ("w", C.c_float * 8)]
 
 
+class NnConfig(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("log2_table", C.c_int32), ("base_res", C.c_float), ("max_res", C.c_float),
+                ("batch", C.c_int32), ("lr", C.c_float), ("bb_lo", C.c_float * 3), ("bb_hi", C.c_float * 3),
+                ("seed", C.c_uint64)]
+
+
+def nn_config(levels=16, log2_table=19, base_res=16.0, max_res=1024.0, batch=65536, lr=1e-2, bb_lo=(0, 0, 0),
+              bb_hi=(1, 1, 1), seed=1):
+    c = NnConfig()
+    c.levels, c.log2_table, c.base_res, c.max_res, c.batch, c.lr = levels, log2_table, base_res, max_res, batch, lr
+    c.bb_lo[:] = [float(x) for x in bb_lo]
+    c.bb_hi[:] = [float(x) for x in bb_hi]
+    c.seed = seed
+    return c
+
+
 PASS_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32)
 
 # Every exported symbol of libttgpu.so, as declared in include/ttgpu.h.
@@ -116,7 +132,8 @@ EXPORTS = [
     "ttgpu_trace_closest_bruteforce",
     "ttgpu_sample_shader", "ttgpu_eval_shader", "ttgpu_sample_phase", "ttgpu_device_info",
     "ttgpu_abi_version", "ttgpu_guide_info", "ttgpu_guide_init_vmf", "ttgpu_guide_fit_vmf",
-    "ttgpu_guide_build_kdtree", "ttgpu_guide_sample_vmf", "ttgpu_guide_pdf_vmf",
+    "ttgpu_guide_build_kdtree", "ttgpu_guide_sample_vmf", "ttgpu_guide_pdf_vmf", "ttgpu_nn_param_count",
+    "ttgpu_nn_init_params", "ttgpu_nn_forward", "ttgpu_nn_gradient", "ttgpu_nn_train",
 ]
 
 _lib = None
@@ -158,6 +175,11 @@ def _declare(lib):
         "ttgpu_guide_build_kdtree": (C.c_int, [f64p, C.c_int64, C.c_int32, C.c_int32, i32p, i32p]),
         "ttgpu_guide_sample_vmf": (C.c_int, [P(Vmf8), C.c_uint64, C.c_int64, f32p, f32p]),
         "ttgpu_guide_pdf_vmf": (C.c_int, [P(Vmf8), C.c_int64, f32p, f32p]),
+        "ttgpu_nn_param_count": (C.c_int64, [P(NnConfig)]),
+        "ttgpu_nn_init_params": (C.c_int, [P(NnConfig), f32p]),
+        "ttgpu_nn_forward": (C.c_int, [P(NnConfig), f32p, C.c_int64, f32p, f32p, f32p, f32p]),
+        "ttgpu_nn_gradient": (C.c_int, [P(NnConfig), f32p, C.c_int64, f32p, f32p, f32p, f32p, f32p, f32p, f32p]),
+        "ttgpu_nn_train": (C.c_int, [P(NnConfig), f32p, C.c_int64, f32p, f32p, f32p, f32p, f32p, C.c_int32, f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -166,7 +166,11 @@ struct PathQueues {
   float4 *gv_thru;      // path throughput after the scatter at the vertex
   float4 *sh_nee;       // shadow ray: NEE radiance towards the previous vertex (per wavelength)
   float4 *rec_pos_w;    // records: position, weight = radiance / pdf
-  float4 *rec_dir;      // records: direction
+  float4 *rec_dir;      // records: direction, weight
+  // learned guide only: per-vertex shading normal and outgoing direction, the same in the records,
+  // and the per-path 16-lobe mixture from batched inference (24 float4 per path)
+  float4 *gv_nrm, *gv_wo, *rec_nrm, *rec_wo;
+  float4 *gmix;
   uint32_t rec_capacity;
   int max_vertices;
   // media (only when the scene has media, else null)

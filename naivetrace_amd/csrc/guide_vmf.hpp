@@ -22,20 +22,29 @@ TT_HD int guide_lookup(const DeviceScene &sc, float3 p) {
   return ~h;
 }
 
-TT_HD float guide_pdf(const GuideCell &c, float3 x) {
+// Mixtures: GuideCell (8 lobes, kd-tree cells) or the learned guide's per-path 16-lobe mixture;
+// both hold lobe[N] = (mean, concentration), wn[N] = weight x normalisation, cdf[N].
+template <class M>
+TT_HD constexpr int lobes_of() {
+  return (int)(sizeof(M::wn) / sizeof(float));
+}
+
+template <class M>
+TT_HD float guide_pdf(const M &c, float3 x) {
   float p = 0.f;
 #pragma unroll
-  for (int k = 0; k < kGuideLobes; ++k) {
+  for (int k = 0; k < lobes_of<M>(); ++k) {
     const float4 l = c.lobe[k];
     p += c.wn[k] * expf(l.w * (l.x * x.x + l.y * x.y + l.z * x.z - 1.f));
   }
   return p;
 }
 
-TT_HD float3 guide_sample(const GuideCell &c, float u0, float u1, float u2) {
-  int idx = kGuideLobes - 1;
+template <class M>
+TT_HD float3 guide_sample(const M &c, float u0, float u1, float u2) {
+  int idx = lobes_of<M>() - 1;
 #pragma unroll
-  for (int k = kGuideLobes - 1; k >= 0; --k)
+  for (int k = lobes_of<M>() - 1; k >= 0; --k)
     if (u0 < c.cdf[k]) idx = k;
   const float4 l = c.lobe[idx];
   const float k = l.w;
@@ -57,7 +66,8 @@ TT_HD float guide_prob_bsdf(const DevShader &s) {
 // One-sample mixture of BSDF and guide: the BSDF is sampled first and a delta lobe is returned as
 // it is; otherwise the BSDF sample is kept with probability alpha or replaced by a guide sample.
 // The returned pdf is (1 - alpha) p_guide + alpha p_bsdf.
-TT_HD Scatter sample_guided(const DevShader &shd, const GuideCell &cell, const SurfHit &h, float3 wi, uint32_t bins, float rough,
+template <class M>
+TT_HD Scatter sample_guided(const DevShader &shd, const M &cell, const SurfHit &h, float3 wi, uint32_t bins, float rough,
                             float lambda0, Pcg &rng) {
   Scatter s = sample_bsdf(shd, h, wi, bins, rough, lambda0, rng);
   if (s.delta) return s;
@@ -75,7 +85,8 @@ TT_HD Scatter sample_guided(const DevShader &shd, const GuideCell &cell, const S
 }
 
 // Mixture pdf of a direction (EvaluateWithBinaryMixtureDensity), given the BSDF's own pdf.
-TT_HD float guided_pdf(const DevShader &shd, const GuideCell &cell, float3 dir, float bsdf_pdf) {
+template <class M>
+TT_HD float guided_pdf(const DevShader &shd, const M &cell, float3 dir, float bsdf_pdf) {
   const float alpha = guide_prob_bsdf(shd);
   return (1.f - alpha) * guide_pdf(cell, dir) + alpha * bsdf_pdf;
 }

@@ -11,6 +11,7 @@
 #include "trace.hpp"
 #include "media.hpp"
 #include "guide_vmf.hpp"
+#include "nn_device.hpp"
 #include "kernels.hpp"
 
 namespace ttgpu {
@@ -757,14 +758,20 @@ __device__ __forceinline__ void emit_records(const PathQueues &q, uint32_t i, in
     const float4 da = q.gv_dir_acc[(size_t)j * cap + i];
     const uint32_t slot = atomicAdd(q.counters + kCountRecords, 1u);
     if (slot < q.rec_capacity) {
-      q.rec_pos_w[slot] = make_float4(pp.x, pp.y, pp.z, da.w / pp.w);
-      q.rec_dir[slot] = make_float4(da.x, da.y, da.z, 0.f);
+      const float w = da.w / pp.w;
+      q.rec_pos_w[slot] = make_float4(pp.x, pp.y, pp.z, w);
+      q.rec_dir[slot] = make_float4(da.x, da.y, da.z, w);
+      if (q.rec_nrm) {
+        q.rec_nrm[slot] = q.gv_nrm[(size_t)j * cap + i];
+        q.rec_wo[slot] = q.gv_wo[(size_t)j * cap + i];
+      }
     } else {
       atomicAdd(q.counters + kStatRecordsDropped, 1u);
     }
   }
 }
 
+template <bool kLearned>
 __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
                                                       const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                       uint32_t *count_out, double *fb, double *fb_sq) {
@@ -809,7 +816,11 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
       } else {
         const SurfHit h = make_hit(sc, prim, hitv.y, hitv.z, hitv.x, xyz(org4), dir);
         const DevMaterial mat = sc.materials[h.material];
-        const GuideCell &cell = sc.guide_cells[guide_lookup(sc, h.pos)];
+        // The guide of this vertex: the kd-tree cell's mixture, or the learned per-path mixture.
+        const GuideCell *cellp = nullptr;
+        const nn::Mix16 *mixp = nullptr;
+        if constexpr (kLearned) mixp = reinterpret_cast<const nn::Mix16 *>(q.gmix) + i;
+        else cellp = sc.guide_cells + guide_lookup(sc, h.pos);
         if (use_emission && mat.emitter >= 0) {
           const DevEmitter &em = sc.emitters[mat.emitter];
           if (em.kind == TTGPU_AREA_UNIFORM && dot(h.ng, -dir) > 0.f) {
@@ -834,7 +845,12 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
             // The guided integrator always runs with TTGPU_MIS_CONSISTENT, so ls.pdf includes the
             // light-selection probability.
             float w = 1.f;
-            if (!ls.delta && mis) w = power_heuristic(ls.pdf, guided_pdf(shd, cell, ls.dir, bsdf_pdf));
+            if (!ls.delta && mis) {
+              float gp;
+              if constexpr (kLearned) gp = guided_pdf(shd, *mixp, ls.dir, bsdf_pdf);
+              else gp = guided_pdf(shd, *cellp, ls.dir, bsdf_pdf);
+              w = power_heuristic(ls.pdf, gp);
+            }
             const float4 nee = (w * dfac) * (f * ls.weight);
             const float3 crgb = spectral_to_rgb(thru * nee, bins);
             if (crgb.x != 0.f || crgb.y != 0.f || crgb.z != 0.f) {
@@ -848,7 +864,9 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
           }
         }
         rough = fmaxf(my_roughness(shd), rough);
-        const Scatter s = sample_guided(shd, cell, h, wi, bins, rough, lambda0, rng);
+        Scatter s;
+        if constexpr (kLearned) s = sample_guided(shd, *mixp, h, wi, bins, rough, lambda0, rng);
+        else s = sample_guided(shd, *cellp, h, wi, bins, rough, lambda0, rng);
         const uint32_t node = (flags & kFlagNodeMask) + 1u;
         bool alive = node <= (uint32_t)sc.max_depth;
         if (alive) {
@@ -861,6 +879,10 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
           q.gv_pos_pdf[(size_t)nv * cap + i] = make_float4(h.pos.x, h.pos.y, h.pos.z, (alive && !s.delta) ? s.pdf : -1.f);
           q.gv_dir_acc[(size_t)nv * cap + i] = make_float4(s.dir.x, s.dir.y, s.dir.z, 0.f);
           q.gv_thru[(size_t)nv * cap + i] = alive ? thru : f4(0.f);
+          if constexpr (kLearned) {
+            q.gv_nrm[(size_t)nv * cap + i] = make_float4(h.ns.x, h.ns.y, h.ns.z, 0.f);
+            q.gv_wo[(size_t)nv * cap + i] = make_float4(wi.x, wi.y, wi.z, 0.f);
+          }
           nv_new = nv + 1;
         }
         flags = (flags & ~kFlagVertexMask) | ((uint32_t)nv_new << kFlagVertexShift);
@@ -926,6 +948,42 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_guided(DeviceScene sc, P
     }
   }
   if (gtid() == 0) atomicAdd(stats + kStatShadow, count);
+}
+
+// ---- learned guide: batched inference over the paths about to be shaded -------------------
+// One 64-thread workgroup per tile of the queue: encode (hit position, shading normal, outgoing
+// direction), MLP on MFMA, 16-lobe mixture per path into q.gmix.
+__global__ void __launch_bounds__(nn::kTile) k_guide_infer(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue,
+                                                          const uint32_t *__restrict__ count_ptr, nn::NetView net) {
+  __shared__ __attribute__((aligned(16))) __bf16 x[nn::kTile * nn::kPitch], h0[nn::kTile * nn::kPitch], h1[nn::kTile * nn::kPitch];
+  __shared__ __attribute__((aligned(16))) float o[nn::kTile * nn::kPitchOut32];
+  const uint32_t count = *count_ptr;
+  const int l = threadIdx.x;
+  for (uint32_t tile = blockIdx.x; tile * nn::kTile < count; tile += gridDim.x) {
+    const uint32_t k = tile * nn::kTile + l;
+    uint32_t i = 0;
+    bool hit = false;
+    if (k < count) {
+      i = queue[k];
+      const float4 hv = q.hit[i];
+      const int prim = as_int(hv.w);
+      if (prim >= 0) {
+        const float3 dir = xyz(q.dir[i]);
+        const SurfHit h = make_hit(sc, prim, hv.y, hv.z, hv.x, xyz(q.org[i]), dir);
+        nn::encode_row(net, h.pos, h.ns, -dir, x + l * nn::kPitch);
+        hit = true;
+      }
+    }
+    if (!hit)
+      for (int f = 0; f < nn::kIn; ++f) x[l * nn::kPitch + f] = (__bf16)0.f;
+    nn::forward_tile(net, x, h0, h1, o, nullptr, 0, 0);
+    if (hit) {
+      nn::Mix16 m;
+      nn::make_mixture(o + l * nn::kPitchOut32, m);
+      *(reinterpret_cast<nn::Mix16 *>(q.gmix) + i) = m;
+    }
+    __syncthreads();
+  }
 }
 
 // ---- debug / test kernels -------------------------------------------------------------
@@ -1063,8 +1121,11 @@ void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
 void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
                  uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
   if (sc.guide_cells)
-    hipLaunchKernelGGL(k_shade_guided, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out,
-                       count_out, fb, fb_sq);
+    hipLaunchKernelGGL(k_shade_guided<false>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
+                       queue_out, count_out, fb, fb_sq);
+  else if (q.gmix)
+    hipLaunchKernelGGL(k_shade_guided<true>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
+                       queue_out, count_out, fb, fb_sq);
   else if (sc.has_media)
     hipLaunchKernelGGL(k_shade_media, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, queue_in, count_in,
                        queue_out, count_out, fb, fb_sq);
@@ -1073,7 +1134,7 @@ void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
                        fb, fb_sq);
 }
 void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
-  if (sc.guide_cells)
+  if (sc.guide_cells || q.gmix)
     hipLaunchKernelGGL(k_shadow_guided, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, fb, fb_sq,
                        q.counters);
   else if (sc.has_media)
@@ -1094,6 +1155,10 @@ void LaunchSampleShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uin
 void LaunchEvalShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, int64_t n, const float *dirs,
                       float *v, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_eval_shader, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, s, wi, ng, ns, bins, rough, n, dirs, v, p);
+}
+void LaunchGuideInfer(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue, const uint32_t *count, uint32_t max_count,
+                      const nn::NetView &net, hipStream_t st) {
+  hipLaunchKernelGGL(k_guide_infer, dim3(Blocks(max_count, nn::kTile, 16384)), dim3(nn::kTile), 0, st, sc, q, queue, count, net);
 }
 void LaunchGuideSample(const GuideCell &c, uint64_t seed, int64_t n, float *d, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_guide_sample, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, c, seed, n, d, p);

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "device_scene.hpp"
+#include "nn.hpp"
 
 namespace ttgpu {
 
@@ -22,6 +23,8 @@ void LaunchSampleShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uin
                         float *d, float *v, float *p, hipStream_t st);
 void LaunchEvalShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint32_t bins, float rough, int64_t n, const float *dirs,
                       float *v, float *p, hipStream_t st);
+void LaunchGuideInfer(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue, const uint32_t *count, uint32_t max_count,
+                      const nn::NetView &net, hipStream_t st);
 void LaunchGuideSample(const GuideCell &c, uint64_t seed, int64_t n, float *d, float *p, hipStream_t st);
 void LaunchGuidePdf(const GuideCell &c, int64_t n, const float *dirs, float *p, hipStream_t st);
 void LaunchSamplePhase(int kind, float g, float3 wi, uint64_t seed, int64_t n, float *d, float *v, float *p, hipStream_t st);

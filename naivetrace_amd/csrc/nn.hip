@@ -1,0 +1,393 @@
+// Training of the learned guide on the GPU: batched forward pass, gradient of
+// the mixture's negative log-likelihood, backward pass through the dense layers
+// and the hash grid, split-K weight gradients, Adam.  All dense algebra runs on
+// bf16 MFMA with fp32 accumulation (nn_device.hpp); the hash-grid gradient is
+// scattered with fp32 atomics; parameters keep an fp32 master copy.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nn_device.hpp"
+#include "nn_host.hpp"
+
+namespace ttgpu {
+namespace nn {
+
+namespace {
+__device__ __forceinline__ float3 xyz3(float4 a) { return make_float3(a.x, a.y, a.z); }
+
+__global__ void __launch_bounds__(kTile) k_forward(NetView net, int64_t n, int64_t n_pad, const float4 *__restrict__ pos,
+                                                  const float4 *__restrict__ nrm, const float4 *__restrict__ wo,
+                                                  const uint32_t *__restrict__ index, float *__restrict__ out, __bf16 *acts) {
+  __shared__ __attribute__((aligned(16))) __bf16 x[kTile * kPitch], h0[kTile * kPitch], h1[kTile * kPitch];
+  __shared__ __attribute__((aligned(16))) float o[kTile * kPitchOut32];
+  const int l = threadIdx.x;
+  for (int64_t tile = blockIdx.x; tile * kTile < n_pad; tile += gridDim.x) {
+    const int64_t base = tile * kTile, r = base + l;
+    if (r < n) {
+      const uint32_t id = index ? index[r] : (uint32_t)r;
+      encode_row(net, xyz3(pos[id]), xyz3(nrm[id]), xyz3(wo[id]), x + l * kPitch);
+    } else {
+      for (int f = 0; f < kIn; ++f) x[l * kPitch + f] = (__bf16)0.f;
+    }
+    forward_tile(net, x, h0, h1, o, acts, n_pad, base);
+    if (out)
+      for (int e = l; e < kTile * kOut; e += kTile) {
+        const int rec = e / kOut, col = e % kOut;
+        if (base + rec < n) out[(base + rec) * kOut + col] = o[rec * kPitchOut32 + col];
+      }
+    __syncthreads();
+  }
+}
+
+// Per record: gradient of -w log q(wi) at the raw outputs, scaled by `scale` (1 / (batch x mean
+// weight of the records)), written feature-major in bf16.
+__global__ void k_loss(int64_t n, int64_t n_pad, const float *__restrict__ out, const float4 *__restrict__ wi_w,
+                       const uint32_t *__restrict__ index, const double *__restrict__ weight_sum, int64_t weight_count,
+                       float batch_inv, __bf16 *__restrict__ dyT, float *loss_sum) {
+  const double mean_w = weight_sum ? *weight_sum / (double)weight_count : 1.0;
+  const float scale = mean_w > 0. ? (float)(batch_inv / mean_w) : 0.f;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_pad; r += (int64_t)gridDim.x * blockDim.x) {
+    float g[kOut];
+    float L = 0.f;
+    if (r < n) {
+      const uint32_t id = index ? index[r] : (uint32_t)r;
+      float raw[kOutUsed];
+      for (int k = 0; k < kOutUsed; ++k) raw[k] = out[r * kOut + k];
+      const float4 d = wi_w[id];
+      L = mixture_nll_grad(raw, make_float3(d.x, d.y, d.z), d.w * scale, g);
+    } else {
+      for (int k = 0; k < kOut; ++k) g[k] = 0.f;
+    }
+    for (int k = 0; k < kOut; ++k) dyT[(int64_t)k * n_pad + r] = (__bf16)g[k];
+    if (loss_sum && L != 0.f) atomicAdd(loss_sum, L);
+  }
+}
+
+__global__ void __launch_bounds__(kTile) k_backward(NetView net, int64_t n, int64_t n_pad, const __bf16 *__restrict__ dyT,
+                                                   const __bf16 *__restrict__ acts, __bf16 *__restrict__ dacts,
+                                                   const float4 *__restrict__ pos, const uint32_t *__restrict__ index,
+                                                   float *__restrict__ grad) {
+  __shared__ __attribute__((aligned(16))) __bf16 d4[kTile * kPitchOut], g0[kTile * kPitch], g1[kTile * kPitch];
+  __shared__ float dx[kTile * 33];
+  const int l = threadIdx.x, lr = l & 15, lo = (l >> 4) * 4;
+  const int64_t T = (int64_t)1 << net.L.log2_table;
+  for (int64_t tile = blockIdx.x; tile * kTile < n_pad; tile += gridDim.x) {
+    const int64_t base = tile * kTile;
+    for (int e = l; e < kTile * kOut; e += kTile) {
+      const int col = e / kTile, rec = e % kTile;
+      d4[rec * kPitchOut + col] = dyT[(int64_t)col * n_pad + base + rec];
+    }
+    __syncthreads();
+    // dH3 = (W4 dY) * relu'(H3), dH2, dH1 likewise.
+    const __bf16 *WN[3] = {net.p + net.L.w4n, net.p + net.L.w3n, net.p + net.L.w2n};
+    __bf16 *src = d4, *dst = g0;
+    for (int layer = 3; layer >= 1; --layer) {
+      floatx4 acc[kHid / 16][4];
+      if (layer == 3) dense_bwd<kHid, kOut>(WN[0], src, kPitchOut, acc);
+      else dense_bwd<kHid, kHid>(WN[3 - layer], src, kPitch, acc);
+      __syncthreads();
+#pragma unroll
+      for (int mg = 0; mg < kHid / 16; ++mg)
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int k = mg * 16 + lo + j, rec = rg * 16 + lr;
+            const float h = (float)acts[((int64_t)layer * kHid + k) * n_pad + base + rec];
+            const __bf16 v = (__bf16)(h > 0.f ? acc[mg][rg][j] : 0.f);
+            dst[rec * kPitch + k] = v;
+            dacts[((int64_t)(layer - 1) * kHid + k) * n_pad + base + rec] = v;
+          }
+      __syncthreads();
+      src = dst;
+      dst = dst == g0 ? g1 : g0;
+    }
+    // dX0 = W1 dH1, grid features only.
+    {
+      floatx4 acc[32 / 16][4];
+      dense_bwd<32, kHid>(net.p + net.L.w1n, src, kPitch, acc);
+#pragma unroll
+      for (int mg = 0; mg < 2; ++mg)
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) dx[(rg * 16 + lr) * 33 + mg * 16 + lo + j] = acc[mg][rg][j];
+    }
+    __syncthreads();
+    const int64_t r = base + l;
+    if (r < n) {
+      const uint32_t id = index ? index[r] : (uint32_t)r;
+      const float4 p = pos[id];
+      const float p01[3] = {(p.x - net.bb_lo[0]) * net.bb_inv[0], (p.y - net.bb_lo[1]) * net.bb_inv[1],
+                            (p.z - net.bb_lo[2]) * net.bb_inv[2]};
+      for (int lv = 0; lv < net.L.levels; ++lv) {
+        uint32_t idx[8];
+        float w[8];
+        grid_corners(p01, net.res[lv], net.L.log2_table, idx, w);
+        const float d0 = dx[l * 33 + 2 * lv], d1 = dx[l * 33 + 2 * lv + 1];
+        float *g = grad + net.L.grid + (int64_t)lv * T * kFeat;
+        for (int k = 0; k < 8; ++k) {
+          if (d0 != 0.f) atomicAdd(g + 2 * (int64_t)idx[k], w[k] * d0);
+          if (d1 != 0.f) atomicAdd(g + 2 * (int64_t)idx[k] + 1, w[k] * d1);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// dWT[o][k] += sum_rec dH[o][rec] X[k][rec];  db[o] += sum_rec dH[o][rec], over chunks of records.
+template <int OUT>
+__global__ void __launch_bounds__(kTile) k_wgrad(int64_t n_pad, int64_t chunk, const __bf16 *__restrict__ dH,
+                                                const __bf16 *__restrict__ X, float *__restrict__ dWT, float *__restrict__ db) {
+  const int l = threadIdx.x, lr = l & 15, lk = (l >> 4) * 8, lo = (l >> 4) * 4;
+  const int64_t r0 = blockIdx.x * chunk, r1 = r0 + chunk < n_pad ? r0 + chunk : n_pad;
+  floatx4 acc[OUT / 16][4];
+#pragma unroll
+  for (int og = 0; og < OUT / 16; ++og)
+#pragma unroll
+    for (int kg = 0; kg < 4; ++kg) acc[og][kg] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t rec = r0; rec < r1; rec += 32) {
+    bf16x8 b[4];
+#pragma unroll
+    for (int kg = 0; kg < 4; ++kg) b[kg] = ld8(X + (int64_t)(kg * 16 + lr) * n_pad + rec + lk);
+#pragma unroll
+    for (int og = 0; og < OUT / 16; ++og) {
+      const bf16x8 a = ld8(dH + (int64_t)(og * 16 + lr) * n_pad + rec + lk);
+#pragma unroll
+      for (int kg = 0; kg < 4; ++kg) acc[og][kg] = mfma(a, b[kg], acc[og][kg]);
+    }
+  }
+#pragma unroll
+  for (int og = 0; og < OUT / 16; ++og)
+#pragma unroll
+    for (int kg = 0; kg < 4; ++kg)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) atomicAdd(dWT + (og * 16 + lo + j) * kIn + kg * 16 + lr, acc[og][kg][j]);
+  for (int o = l; o < OUT; o += kTile) {
+    float s = 0.f;
+    for (int64_t rec = r0; rec < r1; ++rec) s += (float)dH[(int64_t)o * n_pad + rec];
+    atomicAdd(db + o, s);
+  }
+}
+
+// Adam on the fp32 master copy; refreshes the bf16 copies (both weight orientations).  Hash-grid
+// entries with a zero gradient are left alone.
+__global__ void k_adam(NetLayout L, float *__restrict__ p, float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       __bf16 *__restrict__ pb, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L.total; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gr = g[i];
+    if (i < L.w1 && gr == 0.f) continue;
+    g[i] = 0.f;
+    const float mi = b1 * m[i] + (1.f - b1) * gr;
+    const float vi = b2 * v[i] + (1.f - b2) * gr * gr;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    p[i] = pi;
+    const __bf16 pbv = (__bf16)pi;
+    pb[i] = pbv;
+    if (i >= L.w1 && i < L.b1) {
+      int64_t e = i - L.w1;
+      if (e < (int64_t)kHid * kIn) pb[L.w1n + (e % kIn) * kHid + e / kIn] = pbv;
+      else if ((e -= kHid * kIn) < (int64_t)kHid * kHid) pb[L.w2n + (e % kHid) * kHid + e / kHid] = pbv;
+      else if ((e -= kHid * kHid) < (int64_t)kHid * kHid) pb[L.w3n + (e % kHid) * kHid + e / kHid] = pbv;
+      else if ((e -= kHid * kHid) < (int64_t)kOut * kHid) pb[L.w4n + (e % kHid) * kOut + e / kHid] = pbv;
+    }
+  }
+}
+
+__global__ void k_weight_sum(int64_t n, const float4 *__restrict__ wi_w, double *sum) {
+  double s = 0.;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float w = wi_w[i].w;
+    if (w > 0.f && isfinite(w)) s += w;
+  }
+  if (s != 0.) atomicAdd(sum, s);
+}
+
+__global__ void k_batch_index(int64_t n_records, int64_t batch, uint64_t seed, uint64_t step, uint32_t *index) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < batch; i += (int64_t)gridDim.x * blockDim.x) {
+    // PCG32 output function on a per-(step, i) state: uniform record index with replacement.
+    uint64_t s = (seed + 0x9E3779B97F4A7C15ull * (step + 1)) ^ (0xD1B54A32D192ED03ull * (uint64_t)(i + 1));
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    const uint32_t xs = (uint32_t)(((s >> 18u) ^ s) >> 27u), rot = (uint32_t)(s >> 59u);
+    const uint32_t x = (xs >> rot) | (xs << ((32u - rot) & 31u));
+    index[i] = (uint32_t)(((uint64_t)x * (uint64_t)n_records) >> 32);
+  }
+}
+
+__global__ void k_zero_f64(double *p) { *p = 0.; }
+__global__ void k_zero_f32(float *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+void HipCheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+int Blocks(int64_t n, int block, int cap) {
+  int64_t b = (n + block - 1) / block;
+  return (int)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+}  // namespace
+
+NeuralGuide::NeuralGuide(const NeuralGuideConfig &cfg, const float bb_lo[3], const float bb_hi[3], uint64_t seed)
+    : cfg_(cfg), seed_(seed) {
+  if (cfg.levels < 1 || cfg.levels > kMaxLevels || cfg.log2_table < 4 || cfg.log2_table > 24 || cfg.batch < kTile)
+    throw std::invalid_argument("bad neural guide configuration");
+  L_ = make_layout(cfg.levels, cfg.log2_table);
+  batch_ = (cfg.batch + kTile - 1) / kTile * kTile;
+  for (int a = 0; a < 3; ++a) {
+    const float ext = std::max(bb_hi[a] - bb_lo[a], 1e-6f);
+    bb_lo_[a] = bb_lo[a] - 1e-3f * ext;
+    bb_inv_[a] = 1.f / (ext * 1.002f);
+  }
+  const double b = cfg.levels > 1 ? std::exp((std::log(cfg.max_res) - std::log(cfg.base_res)) / (cfg.levels - 1)) : 1.0;
+  for (int lv = 0; lv < kMaxLevels; ++lv) res_[lv] = lv < cfg.levels ? (float)std::floor(cfg.base_res * std::pow(b, lv)) : 1.f;
+  // Initial parameters: grid U(-1e-4, 1e-4), weights U(-sqrt(6 / (fan_in + fan_out)), +..) on the used
+  // rows and columns (the output layer scaled by 0.1), biases zero.
+  std::vector<float> p(L_.total, 0.f);
+  std::mt19937_64 rng(seed ^ 0x5DEECE66Dull);
+  std::uniform_real_distribution<float> U(-1.f, 1.f);
+  for (int64_t i = L_.grid; i < L_.w1; ++i) p[i] = 1e-4f * U(rng);
+  auto init = [&](int64_t off, int out_used, int out_rows, int in_used, float scale) {
+    const float a = scale * std::sqrt(6.f / (float)(in_used + out_used));
+    for (int o = 0; o < out_used; ++o)
+      for (int k = 0; k < in_used; ++k) p[off + (int64_t)o * kIn + k] = a * U(rng);
+    (void)out_rows;
+  };
+  init(L_.w1, kHid, kHid, 38, 1.f);
+  init(L_.w2, kHid, kHid, kHid, 1.f);
+  init(L_.w3, kHid, kHid, kHid, 1.f);
+  init(L_.w4, kOutUsed, kOut, kHid, 0.1f);
+  SetParams(p.data());
+  HipCheck(hipMalloc(&g_, L_.total * sizeof(float)), "hipMalloc nn");
+  HipCheck(hipMalloc(&m_, L_.total * sizeof(float)), "hipMalloc nn");
+  HipCheck(hipMalloc(&v_, L_.total * sizeof(float)), "hipMalloc nn");
+  HipCheck(hipMemset(g_, 0, L_.total * sizeof(float)), "hipMemset nn");
+  HipCheck(hipMemset(m_, 0, L_.total * sizeof(float)), "hipMemset nn");
+  HipCheck(hipMemset(v_, 0, L_.total * sizeof(float)), "hipMemset nn");
+  HipCheck(hipMalloc(&index_, batch_ * sizeof(uint32_t)), "hipMalloc nn");
+  HipCheck(hipMalloc(&out_, batch_ * kOut * sizeof(float)), "hipMalloc nn");
+  HipCheck(hipMalloc(&acts_, 4 * (size_t)kHid * batch_ * sizeof(__bf16)), "hipMalloc nn");
+  HipCheck(hipMalloc(&dacts_, 3 * (size_t)kHid * batch_ * sizeof(__bf16)), "hipMalloc nn");
+  HipCheck(hipMalloc(&dyT_, (size_t)kOut * batch_ * sizeof(__bf16)), "hipMalloc nn");
+  HipCheck(hipMalloc(&wsum_, sizeof(double)), "hipMalloc nn");
+  HipCheck(hipMalloc(&loss_, sizeof(float)), "hipMalloc nn");
+}
+
+NeuralGuide::~NeuralGuide() {
+  for (void *q : {(void *)p_, (void *)pb_, (void *)g_, (void *)m_, (void *)v_, (void *)index_, (void *)out_, (void *)acts_,
+                  (void *)dacts_, (void *)dyT_, (void *)wsum_, (void *)loss_})
+    if (q) (void)hipFree(q);
+}
+
+void NeuralGuide::SetParams(const float *host) {
+  if (!p_) HipCheck(hipMalloc(&p_, L_.total * sizeof(float)), "hipMalloc nn");
+  if (!pb_) HipCheck(hipMalloc(&pb_, L_.total_bf16 * sizeof(__bf16)), "hipMalloc nn");
+  HipCheck(hipMemcpy(p_, host, L_.total * sizeof(float), hipMemcpyHostToDevice), "upload nn");
+  std::vector<__bf16> b(L_.total_bf16, (__bf16)0.f);
+  for (int64_t i = 0; i < L_.total; ++i) b[i] = (__bf16)host[i];
+  for (int o = 0; o < kHid; ++o)
+    for (int k = 0; k < kIn; ++k) {
+      b[L_.w1n + (int64_t)k * kHid + o] = (__bf16)host[L_.w1 + (int64_t)o * kIn + k];
+      b[L_.w2n + (int64_t)k * kHid + o] = (__bf16)host[L_.w2 + (int64_t)o * kHid + k];
+      b[L_.w3n + (int64_t)k * kHid + o] = (__bf16)host[L_.w3 + (int64_t)o * kHid + k];
+    }
+  for (int o = 0; o < kOut; ++o)
+    for (int k = 0; k < kHid; ++k) b[L_.w4n + (int64_t)k * kOut + o] = (__bf16)host[L_.w4 + (int64_t)o * kHid + k];
+  HipCheck(hipMemcpy(pb_, b.data(), L_.total_bf16 * sizeof(__bf16), hipMemcpyHostToDevice), "upload nn");
+}
+
+void NeuralGuide::GetParams(float *host) const {
+  HipCheck(hipMemcpy(host, p_, L_.total * sizeof(float), hipMemcpyDeviceToHost), "download nn");
+}
+
+NetView NeuralGuide::View() const {
+  NetView v;
+  v.p = pb_;
+  v.pf = p_;
+  v.L = L_;
+  for (int lv = 0; lv < kMaxLevels; ++lv) v.res[lv] = res_[lv];
+  for (int a = 0; a < 3; ++a) {
+    v.bb_lo[a] = bb_lo_[a];
+    v.bb_inv[a] = bb_inv_[a];
+  }
+  return v;
+}
+
+void NeuralGuide::Forward(const float4 *pos, const float4 *nrm, const float4 *wo, const uint32_t *index, int64_t n, float *out,
+                          bool keep_acts, hipStream_t st) {
+  const int64_t n_pad = (n + kTile - 1) / kTile * kTile;
+  if (keep_acts && n_pad > batch_) throw std::invalid_argument("batch larger than the training buffers");
+  hipLaunchKernelGGL(k_forward, dim3(Blocks(n_pad, kTile, 8192)), dim3(kTile), 0, st, View(), n, n_pad, pos, nrm, wo, index, out,
+                     keep_acts ? acts_ : nullptr);
+  HipCheck(hipGetLastError(), "k_forward");
+}
+
+void NeuralGuide::Step(const float4 *pos_w, const float4 *dir_w, const float4 *nrm, const float4 *wo, const uint32_t *index,
+                       int64_t n, const double *weight_sum, int64_t weight_count, bool apply, hipStream_t st) {
+  const int64_t n_pad = (n + kTile - 1) / kTile * kTile;
+  if (n_pad > batch_) throw std::invalid_argument("batch larger than the training buffers");
+  Forward(pos_w, nrm, wo, index, n, out_, true, st);
+  hipLaunchKernelGGL(k_loss, dim3(Blocks(n_pad, 256, 4096)), dim3(256), 0, st, n, n_pad, out_, dir_w, index, weight_sum,
+                     weight_count, 1.f / (float)n, dyT_, loss_);
+  hipLaunchKernelGGL(k_backward, dim3(Blocks(n_pad, kTile, 8192)), dim3(kTile), 0, st, View(), n, n_pad, dyT_, acts_, dacts_,
+                     pos_w, index, g_);
+  const int64_t chunk = 1024;
+  const int nb = (int)((n_pad + chunk - 1) / chunk);
+  hipLaunchKernelGGL(k_wgrad<kHid>, dim3(nb), dim3(kTile), 0, st, n_pad, chunk, dacts_, acts_, g_ + L_.w1, g_ + L_.b1);
+  hipLaunchKernelGGL(k_wgrad<kHid>, dim3(nb), dim3(kTile), 0, st, n_pad, chunk, dacts_ + (size_t)kHid * n_pad, acts_ + (size_t)kHid * n_pad,
+                     g_ + L_.w2, g_ + L_.b2);
+  hipLaunchKernelGGL(k_wgrad<kHid>, dim3(nb), dim3(kTile), 0, st, n_pad, chunk, dacts_ + 2 * (size_t)kHid * n_pad,
+                     acts_ + 2 * (size_t)kHid * n_pad, g_ + L_.w3, g_ + L_.b3);
+  hipLaunchKernelGGL(k_wgrad<kOut>, dim3(nb), dim3(kTile), 0, st, n_pad, chunk, dyT_, acts_ + 3 * (size_t)kHid * n_pad, g_ + L_.w4,
+                     g_ + L_.b4);
+  HipCheck(hipGetLastError(), "nn backward");
+  if (apply) ApplyAdam(st);
+}
+
+void NeuralGuide::ApplyAdam(hipStream_t st) {
+  ++t_;
+  const float b1 = 0.9f, b2 = 0.99f;
+  const float bc1 = 1.f - std::pow(b1, (float)t_), bc2 = 1.f - std::pow(b2, (float)t_);
+  hipLaunchKernelGGL(k_adam, dim3(Blocks(L_.total, 256, 16384)), dim3(256), 0, st, L_, p_, g_, m_, v_, pb_, cfg_.lr, b1, b2, 1e-10f,
+                     bc1, bc2);
+  HipCheck(hipGetLastError(), "k_adam");
+}
+
+void NeuralGuide::TrainOnRecords(const float4 *pos_w, const float4 *dir_w, const float4 *nrm, const float4 *wo, int64_t n,
+                                 hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, st, wsum_);
+  hipLaunchKernelGGL(k_weight_sum, dim3(Blocks(n, 256, 2048)), dim3(256), 0, st, n, dir_w, wsum_);
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, st, loss_, (int64_t)1);
+  const int64_t steps = std::max<int64_t>(1, (n + batch_ - 1) / batch_);
+  for (int64_t s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(k_batch_index, dim3(Blocks(batch_, 256, 1024)), dim3(256), 0, st, n, batch_, seed_, step_++, index_);
+    Step(pos_w, dir_w, nrm, wo, index_, batch_, wsum_, n, true, st);
+  }
+  steps_ += steps;
+}
+
+float NeuralGuide::LastLoss() const {
+  float l = 0.f;
+  HipCheck(hipMemcpy(&l, loss_, sizeof(float), hipMemcpyDeviceToHost), "loss");
+  return l;
+}
+
+void NeuralGuide::Gradient(float *host) const {
+  HipCheck(hipMemcpy(host, g_, L_.total * sizeof(float), hipMemcpyDeviceToHost), "gradient");
+}
+void NeuralGuide::ZeroGradient(hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_f32, dim3(Blocks(L_.total, 256, 16384)), dim3(256), 0, st, g_, L_.total);
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, st, loss_, (int64_t)1);
+}
+
+}  // namespace nn
+}  // namespace ttgpu

@@ -1,0 +1,109 @@
+// Learned guiding network on the GPU: a multiresolution hash-grid encoding of
+// the position, the surface normal and the outgoing direction feed an MLP
+// (38 -> 64 -> 64 -> 64 -> 80, ReLU) whose 80 outputs parametrise a 16-lobe
+// von Mises-Fisher mixture of the incident radiance.
+//
+// The dense layers run on MFMA (v_mfma_f32_16x16x32_bf16: bf16 operands, fp32
+// accumulation), one 64-thread workgroup per tile of 64 records; activations
+// pass between layers through LDS.  The mixture parametrisation per lobe j is
+// (logit, m.xyz, s): weights softmax(logit), mean m/|m|, concentration
+// softplus(s) + kKappaMin, at most kKappaMax.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttgpu {
+namespace nn {
+
+constexpr int kTile = 64;          // records per workgroup (one wavefront)
+constexpr int kIn = 64;            // input width, padded: 32 grid features + normal + direction + zeros
+constexpr int kHid = 64;
+constexpr int kOut = 96;           // 80 outputs, padded to a multiple of 32
+constexpr int kOutUsed = 80;
+constexpr int kLobes = 16;
+constexpr int kMaxLevels = 16;
+constexpr int kFeat = 2;           // features per level
+constexpr int kPitch = 72;         // LDS row pitch in bf16 (64 + 8) for conflict-free 16-byte reads
+constexpr int kPitchOut = 104;     // LDS row pitch for 96-wide rows
+constexpr float kKappaMin = 0.05f, kKappaMax = 100.f;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+// Parameter layout (fp32 master copy and bf16 working copy share it):
+//   grid [levels][table][2] | W1T [64][64] | W2T [64][64] | W3T [64][64] | W4T [96][64] |
+//   b1 [64] | b2 [64] | b3 [64] | b4 [96]
+// WkT[out][in] is the transposed weight (y = WkT x + bk).  Padding rows/cols are kept at zero.
+// The bf16 working copy continues after `total` with the untransposed weights used by the
+// backward pass: W1N [64 in][64 out] | W2N | W3N | W4N [64][96].
+struct NetLayout {
+  int levels, log2_table;
+  int64_t grid, w1, w2, w3, w4, b1, b2, b3, b4, total;
+  int64_t w1n, w2n, w3n, w4n, total_bf16;
+};
+__host__ __device__ inline NetLayout make_layout(int levels, int log2_table) {
+  NetLayout L;
+  L.levels = levels;
+  L.log2_table = log2_table;
+  L.grid = 0;
+  L.w1 = (int64_t)levels * ((int64_t)1 << log2_table) * kFeat;
+  L.w2 = L.w1 + kHid * kIn;
+  L.w3 = L.w2 + kHid * kHid;
+  L.w4 = L.w3 + kHid * kHid;
+  L.b1 = L.w4 + kOut * kHid;
+  L.b2 = L.b1 + kHid;
+  L.b3 = L.b2 + kHid;
+  L.b4 = L.b3 + kHid;
+  L.total = L.b4 + kOut;
+  L.w1n = (L.total + 7) / 8 * 8;  // 16-byte aligned
+  L.w2n = L.w1n + kIn * kHid;
+  L.w3n = L.w2n + kHid * kHid;
+  L.w4n = L.w3n + kHid * kHid;
+  L.total_bf16 = L.w4n + kHid * kOut;
+  return L;
+}
+
+// Per-path guide of the renderer: the 16-lobe mixture of one query.
+struct Mix16 {
+  float4 lobe[kLobes];  // mean direction, concentration
+  float wn[kLobes];     // weight x normalisation k / (2 pi (1 - e^{-2k}))
+  float cdf[kLobes];    // cumulative weights (last entry 2: every u in [0, 1) selects a lobe)
+};
+
+// Everything a kernel needs to evaluate the network.
+struct NetView {
+  const __bf16 *p;          // bf16 parameters (layout above)
+  const float *pf;          // fp32 parameters (biases are read from here)
+  NetLayout L;
+  float res[kMaxLevels];    // grid resolution per level
+  float bb_lo[3], bb_inv[3];// position normalisation to [0, 1]^3
+};
+
+// Hash of integer grid coordinates (Teschner et al. 2003 primes), or dense index when the level fits.
+__host__ __device__ inline uint32_t grid_index(uint32_t x, uint32_t y, uint32_t z, uint32_t dim, bool dense, uint32_t mask) {
+  if (dense) return (x + dim * (y + dim * z)) & mask;
+  return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & mask;
+}
+
+// Trilinear corners of one level: indices and weights.
+__host__ __device__ inline void grid_corners(const float p01[3], float res, int log2_table, uint32_t idx[8], float w[8]) {
+  const uint32_t mask = (1u << log2_table) - 1u;
+  const uint32_t dim = (uint32_t)res + 2u;
+  const bool dense = (uint64_t)dim * dim * dim <= ((uint64_t)1 << log2_table);
+  float f[3];
+  uint32_t c[3];
+  for (int a = 0; a < 3; ++a) {
+    float x = fminf(fmaxf(p01[a], 0.f), 1.f) * res;
+    float fl = floorf(x);
+    f[a] = x - fl;
+    c[a] = (uint32_t)fl;
+  }
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    idx[k] = grid_index(c[0] + dx, c[1] + dy, c[2] + dz, dim, dense, mask);
+    w[k] = (dx ? f[0] : 1.f - f[0]) * (dy ? f[1] : 1.f - f[1]) * (dz ? f[2] : 1.f - f[2]);
+  }
+}
+
+}  // namespace nn
+}  // namespace ttgpu

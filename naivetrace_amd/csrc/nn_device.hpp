@@ -1,0 +1,201 @@
+// Device functions of the learned guide: hash-grid encoding, MFMA dense layers
+// on 64-record tiles, the 16-lobe mixture and the gradient of its negative
+// log-likelihood.  See nn.hpp for shapes and the parameter layout.
+//
+// v_mfma_f32_16x16x32_bf16 operand layout (checked on gfx950): lane l holds
+// A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16] for i < 8, and the
+// result D[4 (l / 16) + j][l % 16] for j < 4.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "nn.hpp"
+
+namespace ttgpu {
+namespace nn {
+
+__device__ __forceinline__ bf16x8 ld8(const __bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+__device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Network inputs of one record as a row of 64 bf16: 2 features x levels, normal, outgoing direction, zeros.
+__device__ inline void encode_row(const NetView &net, float3 pos, float3 n, float3 wo, __bf16 *row) {
+  const float p01[3] = {(pos.x - net.bb_lo[0]) * net.bb_inv[0], (pos.y - net.bb_lo[1]) * net.bb_inv[1],
+                        (pos.z - net.bb_lo[2]) * net.bb_inv[2]};
+  const int64_t T = (int64_t)1 << net.L.log2_table;
+  for (int lv = 0; lv < net.L.levels; ++lv) {
+    uint32_t idx[8];
+    float w[8];
+    grid_corners(p01, net.res[lv], net.L.log2_table, idx, w);
+    const uint32_t *g = reinterpret_cast<const uint32_t *>(net.p + net.L.grid + (int64_t)lv * T * kFeat);
+    float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t v = g[idx[k]];  // two bf16 features of the entry
+      f0 += w[k] * __uint_as_float(v << 16);
+      f1 += w[k] * __uint_as_float(v & 0xFFFF0000u);
+    }
+    row[2 * lv] = (__bf16)f0;
+    row[2 * lv + 1] = (__bf16)f1;
+  }
+  for (int f = 2 * net.L.levels; f < 32; ++f) row[f] = (__bf16)0.f;
+  row[32] = (__bf16)n.x; row[33] = (__bf16)n.y; row[34] = (__bf16)n.z;
+  row[35] = (__bf16)wo.x; row[36] = (__bf16)wo.y; row[37] = (__bf16)wo.z;
+  for (int f = 38; f < kIn; ++f) row[f] = (__bf16)0.f;
+}
+
+// acc[og][rg] = Y[og*16 + 4 (l/16) + j][rg*16 + l%16] = sum_k WT[out][k] X[rec][k], K = 64.
+template <int OUT>
+__device__ __forceinline__ void dense_fwd(const __bf16 *__restrict__ WT, const __bf16 *X, floatx4 (&acc)[OUT / 16][4]) {
+  const int l = threadIdx.x, lr = l & 15, lk = (l >> 4) * 8;
+#pragma unroll
+  for (int og = 0; og < OUT / 16; ++og) {
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg) acc[og][rg] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kIn; ks += 32) {
+      const bf16x8 a = ld8(WT + (og * 16 + lr) * kIn + ks + lk);
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg) acc[og][rg] = mfma(a, ld8(X + (rg * 16 + lr) * kPitch + ks + lk), acc[og][rg]);
+    }
+  }
+}
+
+// acc[mg][rg] = dX[mg*16 + 4 (l/16) + j][rg*16 + l%16] = sum_o WN[k][o] dY[rec][o], reduction width K.
+template <int M, int K>
+__device__ __forceinline__ void dense_bwd(const __bf16 *__restrict__ WN, const __bf16 *dY, int pitch, floatx4 (&acc)[M / 16][4]) {
+  const int l = threadIdx.x, lr = l & 15, lk = (l >> 4) * 8;
+#pragma unroll
+  for (int mg = 0; mg < M / 16; ++mg) {
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg) acc[mg][rg] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K; ks += 32) {
+      const bf16x8 a = ld8(WN + (mg * 16 + lr) * K + ks + lk);
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg) acc[mg][rg] = mfma(a, ld8(dY + (rg * 16 + lr) * pitch + ks + lk), acc[mg][rg]);
+    }
+  }
+}
+
+// Forward pass of one 64-record tile whose input rows are in `x` (LDS, pitch kPitch).  Hidden
+// activations go through LDS buffers h0/h1; `acts` (optional, feature-major [4][64][n_pad]) receives
+// the layer inputs X0, H1, H2, H3 for the backward pass; the 96 raw outputs of record `rec` of the
+// tile land in out_lds[rec * kPitchOut32 + o] (fp32).
+constexpr int kPitchOut32 = 97;
+__device__ inline void forward_tile(const NetView &net, __bf16 *x, __bf16 *h0, __bf16 *h1, float *out_lds, __bf16 *acts,
+                                    int64_t n_pad, int64_t tile_base) {
+  const int l = threadIdx.x, lr = l & 15, lo = (l >> 4) * 4;
+  const __bf16 *W[3] = {net.p + net.L.w1, net.p + net.L.w2, net.p + net.L.w3};
+  const float *B[3] = {net.pf + net.L.b1, net.pf + net.L.b2, net.pf + net.L.b3};
+  __bf16 *bufs[4] = {x, h0, h1, h0};
+  if (acts) {
+    // X0 (feature-major) for the weight gradient of layer 1.
+    for (int f = 0; f < kIn; ++f) acts[(int64_t)f * n_pad + tile_base + l] = x[l * kPitch + f];
+  }
+  __syncthreads();
+  for (int layer = 0; layer < 3; ++layer) {
+    floatx4 acc[kHid / 16][4];
+    dense_fwd<kHid>(W[layer], bufs[layer], acc);
+    __syncthreads();
+    __bf16 *y = bufs[layer + 1];
+#pragma unroll
+    for (int og = 0; og < kHid / 16; ++og)
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int o = og * 16 + lo + j, rec = rg * 16 + lr;
+          const __bf16 v = (__bf16)fmaxf(acc[og][rg][j] + B[layer][o], 0.f);
+          y[rec * kPitch + o] = v;
+          if (acts) acts[((int64_t)(layer + 1) * kHid + o) * n_pad + tile_base + rec] = v;
+        }
+    __syncthreads();
+  }
+  floatx4 acc[kOut / 16][4];
+  dense_fwd<kOut>(net.p + net.L.w4, bufs[3], acc);
+  const float *b4 = net.pf + net.L.b4;
+#pragma unroll
+  for (int og = 0; og < kOut / 16; ++og)
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int o = og * 16 + lo + j, rec = rg * 16 + lr;
+        out_lds[rec * kPitchOut32 + o] = acc[og][rg][j] + b4[o];
+      }
+  __syncthreads();
+}
+
+// 16-lobe mixture (Mix16, nn.hpp) from the 80 raw outputs of one record.
+__device__ __forceinline__ float softplus(float s) { return s > 15.f ? s : log1pf(expf(s)); }
+
+__device__ inline void make_mixture(const float *raw, Mix16 &m) {
+  float mx = -3.4e38f;
+  for (int j = 0; j < kLobes; ++j) mx = fmaxf(mx, raw[5 * j]);
+  float e[kLobes], sum = 0.f;
+  for (int j = 0; j < kLobes; ++j) {
+    e[j] = expf(raw[5 * j] - mx);
+    sum += e[j];
+  }
+  float acc = 0.f;
+  for (int j = 0; j < kLobes; ++j) {
+    const float pi = e[j] / sum;
+    const float mxv = raw[5 * j + 1], myv = raw[5 * j + 2], mzv = raw[5 * j + 3];
+    const float len = sqrtf(mxv * mxv + myv * myv + mzv * mzv);
+    const float inv = len > 1e-12f ? 1.f / len : 0.f;
+    const float k = fminf(softplus(raw[5 * j + 4]) + kKappaMin, kKappaMax);
+    m.lobe[j] = len > 1e-12f ? make_float4(mxv * inv, myv * inv, mzv * inv, k) : make_float4(0.f, 0.f, 1.f, k);
+    m.wn[j] = pi * k / (2.f * 3.14159265358979f * -expm1f(-2.f * k));
+    acc += pi;
+    m.cdf[j] = acc;
+  }
+  m.cdf[kLobes - 1] = 2.f;
+}
+
+// Gradient of  -w log q(wi)  with respect to the 96 raw outputs (entries 80..95 get zero); returns
+// the loss, or 0 with a zero gradient when q underflows.
+__device__ inline float mixture_nll_grad(const float *raw, float3 wi, float w, float *g) {
+  for (int o = 0; o < kOut; ++o) g[o] = 0.f;
+  float mx = -3.4e38f;
+  for (int j = 0; j < kLobes; ++j) mx = fmaxf(mx, raw[5 * j]);
+  float pi[kLobes], v[kLobes], sum = 0.f;
+  for (int j = 0; j < kLobes; ++j) {
+    pi[j] = expf(raw[5 * j] - mx);
+    sum += pi[j];
+  }
+  float q = 0.f;
+  for (int j = 0; j < kLobes; ++j) {
+    pi[j] /= sum;
+    const float mxv = raw[5 * j + 1], myv = raw[5 * j + 2], mzv = raw[5 * j + 3];
+    const float len = fmaxf(sqrtf(mxv * mxv + myv * myv + mzv * mzv), 1e-12f);
+    const float k = fminf(softplus(raw[5 * j + 4]) + kKappaMin, kKappaMax);
+    const float c = (mxv * wi.x + myv * wi.y + mzv * wi.z) / len;
+    v[j] = k / (2.f * 3.14159265358979f * -expm1f(-2.f * k)) * expf(k * (c - 1.f));
+    q += pi[j] * v[j];
+  }
+  if (!(q > 1e-30f) || !isfinite(q) || !(w > 0.f)) return 0.f;
+  for (int j = 0; j < kLobes; ++j) {
+    const float r = pi[j] * v[j] / q;
+    const float mxv = raw[5 * j + 1], myv = raw[5 * j + 2], mzv = raw[5 * j + 3];
+    const float len = fmaxf(sqrtf(mxv * mxv + myv * myv + mzv * mzv), 1e-12f);
+    const float s = raw[5 * j + 4];
+    const float sp = softplus(s);
+    const float k = sp + kKappaMin;
+    const float c = (mxv * wi.x + myv * wi.y + mzv * wi.z) / len;
+    g[5 * j] = w * (pi[j] - r);
+    // d(-w log q)/dm = -w r k (wi - c mu) / |m|
+    const float f = -w * r * k / len;
+    g[5 * j + 1] = f * (wi.x - c * mxv / len);
+    g[5 * j + 2] = f * (wi.y - c * myv / len);
+    g[5 * j + 3] = f * (wi.z - c * mzv / len);
+    if (k < kKappaMax) {
+      const float e2 = expf(-2.f * k);
+      const float dlogv = 1.f / k - 2.f * e2 / (1.f - e2) + (c - 1.f);
+      g[5 * j + 4] = -w * r * dlogv / (1.f + expf(-s));  // d softplus / ds = sigmoid(s)
+    }
+  }
+  return -w * logf(q);
+}
+
+}  // namespace nn
+}  // namespace ttgpu

@@ -1,0 +1,65 @@
+// Host side of the learned guide: parameters, optimiser state and training
+// buffers on the device, and the training step (kernels in nn.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nn.hpp"
+
+namespace ttgpu {
+namespace nn {
+
+struct NeuralGuideConfig {
+  int levels = 16;          // hash-grid levels
+  int log2_table = 19;      // entries per level = 2^log2_table
+  float base_res = 16.f;    // coarsest grid resolution
+  float max_res = 1024.f;   // finest grid resolution
+  int batch = 65536;        // records per Adam step
+  float lr = 1e-2f;
+};
+
+class NeuralGuide {
+ public:
+  NeuralGuide(const NeuralGuideConfig &cfg, const float bb_lo[3], const float bb_hi[3], uint64_t seed);
+  ~NeuralGuide();
+  NeuralGuide(const NeuralGuide &) = delete;
+  NeuralGuide &operator=(const NeuralGuide &) = delete;
+
+  NetView View() const;
+  const NetLayout &Layout() const { return L_; }
+  void SetParams(const float *host);   // fp32 master copy; refreshes the bf16 copies
+  void GetParams(float *host) const;
+  // Raw outputs [n][96] of `n` records (optionally gathered through `index`).
+  void Forward(const float4 *pos, const float4 *nrm, const float4 *wo, const uint32_t *index, int64_t n, float *out, bool keep_acts,
+               hipStream_t st);
+  // Forward + loss gradient + backward into the gradient buffer (accumulating); Adam when `apply`.
+  // Record weights are divided by weight_sum / weight_count (1 when weight_sum is null).
+  void Step(const float4 *pos_w, const float4 *dir_w, const float4 *nrm, const float4 *wo, const uint32_t *index, int64_t n,
+            const double *weight_sum, int64_t weight_count, bool apply, hipStream_t st);
+  void ApplyAdam(hipStream_t st);
+  // One pass over `n` device records: ceil(n / batch) Adam steps on batches drawn with replacement.
+  void TrainOnRecords(const float4 *pos_w, const float4 *dir_w, const float4 *nrm, const float4 *wo, int64_t n, hipStream_t st);
+  float LastLoss() const;              // summed loss of the last pass (or Step calls since ZeroGradient)
+  void Gradient(float *host) const;
+  void ZeroGradient(hipStream_t st);
+  int64_t steps() const { return steps_; }
+
+ private:
+  NeuralGuideConfig cfg_;
+  NetLayout L_;
+  uint64_t seed_;
+  int64_t batch_ = 0, step_ = 0, steps_ = 0;
+  int t_ = 0;
+  float res_[kMaxLevels];
+  float bb_lo_[3], bb_inv_[3];
+  float *p_ = nullptr, *g_ = nullptr, *m_ = nullptr, *v_ = nullptr;
+  __bf16 *pb_ = nullptr;
+  uint32_t *index_ = nullptr;
+  float *out_ = nullptr;
+  __bf16 *acts_ = nullptr, *dacts_ = nullptr, *dyT_ = nullptr;
+  double *wsum_ = nullptr;
+  float *loss_ = nullptr;
+};
+
+}  // namespace nn
+}  // namespace ttgpu

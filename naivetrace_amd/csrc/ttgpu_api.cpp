@@ -21,6 +21,7 @@
 #include "device_scene.hpp"
 #include "guiding.hpp"
 #include "kernels.hpp"
+#include "nn_host.hpp"
 
 #include "spectral_tables.inc"
 
@@ -93,7 +94,11 @@ struct ttgpu_handle {
   int num_triangles = 0;
   // guided integrator
   bool guided = false, trained = false;
-  std::unique_ptr<guide::Recorder> recorder;
+  std::unique_ptr<guide::Recorder> recorder;       // vMF-tree guide
+  std::unique_ptr<nn::NeuralGuide> net;            // learned guide
+  bool learned = false;
+  int train_rounds = 0;
+  float last_loss = 0.f;
   void *d_guide_nodes = nullptr, *d_guide_cells = nullptr;
   int64_t records_last_pass = 0, records_dropped = 0;
 
@@ -177,8 +182,7 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (p.num_wavelengths != 0 && p.num_wavelengths != 4) bad("the GPU path traces 4 wavelengths per path");
   if (p.sample_mode < 0 || p.sample_mode > 2) bad("bad sample_mode");
   if (p.algo != TTGPU_ALGO_PT2 && p.algo != TTGPU_ALGO_PTG) bad("unknown algorithm");
-  if (p.algo == TTGPU_ALGO_PTG && p.guide_kind == TTGPU_GUIDE_LEARNED)
-    bad("learned guiding is not available in this build; use TTGPU_GUIDE_VMF_TREE");
+  if (p.guide_kind < TTGPU_GUIDE_NONE || p.guide_kind > TTGPU_GUIDE_LEARNED) bad("unknown guide kind");
   if (s.num_triangles < 0 || s.num_spheres < 0 || s.num_vertices < 0) bad("negative counts");
   if (s.num_triangles > 0 && (!s.vertices || !s.indices || !s.tri_material)) bad("triangles need vertices, indices, materials");
   if (s.num_spheres > 0 && (!s.spheres || !s.sphere_material)) bad("spheres need data and materials");
@@ -627,6 +631,16 @@ void AllocateQueues(ttgpu_handle *h) {
     q.rec_dir = h->Alloc<float4>(q.rec_capacity);
     h->stats.state_bytes += (size_t)N * (48 * nvtx + 16) + (size_t)q.rec_capacity * 32;
   }
+  q.gv_nrm = q.gv_wo = q.rec_nrm = q.rec_wo = q.gmix = nullptr;
+  if (h->learned) {
+    const int nvtx = q.max_vertices;
+    q.gv_nrm = h->Alloc<float4>((size_t)N * nvtx);
+    q.gv_wo = h->Alloc<float4>((size_t)N * nvtx);
+    q.rec_nrm = h->Alloc<float4>(q.rec_capacity);
+    q.rec_wo = h->Alloc<float4>(q.rec_capacity);
+    q.gmix = h->Alloc<float4>((size_t)N * (sizeof(nn::Mix16) / sizeof(float4)));
+    h->stats.state_bytes += (size_t)N * (32 * nvtx + sizeof(nn::Mix16)) + (size_t)q.rec_capacity * 32;
+  }
   const size_t npix = (size_t)h->npix;
   h->fb = h->Alloc<double>(3 * npix);
   h->fb_sq = h->Alloc<double>(3 * npix);
@@ -694,8 +708,26 @@ void CollectRecords(ttgpu_handle *h) {
   h->recorder->FinalizeRound(recs);
 }
 
+// After a recording pass: refit the vMF-tree recorder on the host, or train the network on the
+// device records (records stay on the GPU; only their count is read back).
+void LearnFromPass(ttgpu_handle *h) {
+  ++h->train_rounds;
+  if (!h->learned) {
+    CollectRecords(h);
+    return;
+  }
+  uint32_t counters[kNumCounters];
+  Check(hipMemcpy(counters, h->q.counters, sizeof(counters), hipMemcpyDeviceToHost), "read counters");
+  const int64_t n = std::min(counters[kCountRecords], h->q.rec_capacity);
+  h->records_last_pass = n;
+  h->records_dropped += counters[kStatRecordsDropped];
+  h->Timed("train", [&] { h->net->TrainOnRecords(h->q.rec_pos_w, h->q.rec_dir, h->q.rec_nrm, h->q.rec_wo, n, h->stream); });
+  h->CollectTimers();
+}
+
 int Fail(ttgpu_handle *h, int code, const std::string &msg) {
   if (h) h->err = msg;
+  else g_create_error = msg;  // calls without a handle report through ttgpu_create_error()
   return code;
 }
 
@@ -749,6 +781,7 @@ bool RenderPass(ttgpu_handle *h, int spp) {
     int nxt_idx = kCountB, cur_idx = kCountA;
     for (int b = 0; b < bounces; ++b) {
       h->Timed("trace", [&] { LaunchTrace(h->dsc, q, cur_q, cur_c, n, h->stream); });
+      if (h->learned) h->Timed("guide", [&] { LaunchGuideInfer(h->dsc, q, cur_q, cur_c, n, h->net->View(), h->stream); });
       LaunchResetCounter(q.counters, nxt_idx, h->stream);
       LaunchResetCounter(q.counters, kCountShadow, h->stream);
       h->Timed("shade", [&] { LaunchShade(h->dsc, q, cur_q, cur_c, nxt_q, nxt_c, n, h->fb, h->fb_sq, h->stream); });
@@ -853,6 +886,7 @@ int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_han
                         : 1.0;
     BuildTables(h, s, diag);
     h->guided = params->algo == TTGPU_ALGO_PTG;
+    h->learned = h->guided && params->guide_kind == TTGPU_GUIDE_LEARNED;
     if (h->guided) {
       // The guided integrator's MIS (power heuristic) always includes the light-selection
       // probability in the NEE pdf.
@@ -864,7 +898,10 @@ int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_han
     h->capacity = (uint32_t)cap;
     h->stats.wavefront_paths = (int32_t)cap;
     AllocateQueues(h);
-    if (h->guided) {
+    if (h->learned) {
+      const float flo[3] = {(float)lo[0], (float)lo[1], (float)lo[2]}, fhi[3] = {(float)hi[0], (float)hi[1], (float)hi[2]};
+      h->net = std::make_unique<nn::NeuralGuide>(nn::NeuralGuideConfig{}, flo, fhi, params->seed);
+    } else if (h->guided) {
       const int subdiv = params->guide_subdiv_factor > 0 ? params->guide_subdiv_factor : 100;
       h->recorder = std::make_unique<guide::Recorder>(subdiv, params->seed, params->num_threads);
       UploadGuide(h);
@@ -930,19 +967,21 @@ int ttgpu_run(ttgpu_handle *h, int32_t max_passes) {
       h->dsc.guide_record = 1;
       if (!h->stop.load()) {
         RenderPass(h, 1);
-        CollectRecords(h);
-        UploadGuide(h);
+        LearnFromPass(h);
+        if (!h->learned) UploadGuide(h);
         if (h->cb) h->cb(h->cb_user, 1);
       }
       for (int num = 2; !h->stop.load() && num <= h->params.guide_max_spp; num *= 2) {
         ResetFramebuffer(h);
         for (int k = 0; k < num && !h->stop.load(); ++k) {
           RenderPass(h, 1);
-          CollectRecords(h);
+          LearnFromPass(h);
         }
         if (h->cb) h->cb(h->cb_user, 1);
-        h->recorder->PrepareAdaptedStructures();
-        UploadGuide(h);
+        if (!h->learned) {
+          h->recorder->PrepareAdaptedStructures();
+          UploadGuide(h);
+        }
       }
       h->dsc.guide_record = 0;
       if (h->stop.load()) return TTGPU_ERR_INTERRUPTED;
@@ -1178,11 +1217,115 @@ int ttgpu_sample_phase(int32_t phase_kind, float g, const float wi[3], uint64_t 
   }, out_dir, out_value, out_pdf, 3, 1);
 }
 
+}  // extern "C"
+
+static nn::NeuralGuideConfig NnConfig(const ttgpu_nn_config &c) {
+  nn::NeuralGuideConfig k;
+  k.levels = c.levels;
+  k.log2_table = c.log2_table;
+  k.base_res = c.base_res;
+  k.max_res = c.max_res;
+  k.batch = c.batch;
+  k.lr = c.lr;
+  return k;
+}
+
+int64_t ttgpu_nn_param_count(const ttgpu_nn_config *cfg) {
+  if (!cfg || cfg->levels < 1 || cfg->levels > nn::kMaxLevels || cfg->log2_table < 4 || cfg->log2_table > 24) return -1;
+  return nn::make_layout(cfg->levels, cfg->log2_table).total;
+}
+
+namespace {
+struct DevBuf {
+  void *p = nullptr;
+  explicit DevBuf(size_t bytes) { Check(hipMalloc(&p, std::max<size_t>(bytes, 16)), "hipMalloc"); }
+  ~DevBuf() { (void)hipFree(p); }
+  template <class T> T *as() { return (T *)p; }
+};
+// n x 3 floats (and an optional n-vector in w) -> device float4 array.
+void UploadF4(DevBuf &b, int64_t n, const float *xyz, const float *w) {
+  std::vector<float4> v(n);
+  for (int64_t i = 0; i < n; ++i) v[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], w ? w[i] : 0.f);
+  if (n) Check(hipMemcpy(b.p, v.data(), n * sizeof(float4), hipMemcpyHostToDevice), "upload");
+}
+}  // namespace
+
+extern "C" {
+
+int ttgpu_nn_init_params(const ttgpu_nn_config *cfg, float *params) {
+  if (!cfg || !params) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    nn::NeuralGuide g(NnConfig(*cfg), cfg->bb_lo, cfg->bb_hi, cfg->seed);
+    g.GetParams(params);
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_nn_forward(const ttgpu_nn_config *cfg, const float *params, int64_t n, const float *pos, const float *nrm,
+                     const float *wo, float *out) {
+  if (!cfg || !params || n < 0 || (n > 0 && (!pos || !nrm || !wo || !out))) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    nn::NeuralGuide g(NnConfig(*cfg), cfg->bb_lo, cfg->bb_hi, cfg->seed);
+    g.SetParams(params);
+    DevBuf dp(n * sizeof(float4)), dn(n * sizeof(float4)), dw(n * sizeof(float4)), dout(n * nn::kOut * sizeof(float));
+    UploadF4(dp, n, pos, nullptr);
+    UploadF4(dn, n, nrm, nullptr);
+    UploadF4(dw, n, wo, nullptr);
+    g.Forward(dp.as<float4>(), dn.as<float4>(), dw.as<float4>(), nullptr, n, dout.as<float>(), false, 0);
+    Check(hipDeviceSynchronize(), "sync");
+    if (n) Check(hipMemcpy(out, dout.p, n * nn::kOut * sizeof(float), hipMemcpyDeviceToHost), "download");
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_nn_gradient(const ttgpu_nn_config *cfg, const float *params, int64_t n, const float *pos, const float *nrm,
+                      const float *wo, const float *wi, const float *weight, float *grad, float *loss) {
+  if (!cfg || !params || n <= 0 || !pos || !nrm || !wo || !wi || !weight || !grad || n > cfg->batch)
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    nn::NeuralGuide g(NnConfig(*cfg), cfg->bb_lo, cfg->bb_hi, cfg->seed);
+    g.SetParams(params);
+    DevBuf dp(n * sizeof(float4)), dn(n * sizeof(float4)), dw(n * sizeof(float4)), dd(n * sizeof(float4));
+    UploadF4(dp, n, pos, nullptr);
+    UploadF4(dn, n, nrm, nullptr);
+    UploadF4(dw, n, wo, nullptr);
+    UploadF4(dd, n, wi, weight);
+    g.ZeroGradient(0);
+    g.Step(dp.as<float4>(), dd.as<float4>(), dn.as<float4>(), dw.as<float4>(), nullptr, n, nullptr, 0, false, 0);
+    Check(hipDeviceSynchronize(), "sync");
+    g.Gradient(grad);
+    if (loss) *loss = g.LastLoss();
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_nn_train(const ttgpu_nn_config *cfg, float *params, int64_t n, const float *pos, const float *nrm, const float *wo,
+                   const float *wi, const float *weight, int32_t passes, float *loss_per_pass) {
+  if (!cfg || !params || n <= 0 || !pos || !nrm || !wo || !wi || !weight || passes < 0) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    nn::NeuralGuide g(NnConfig(*cfg), cfg->bb_lo, cfg->bb_hi, cfg->seed);
+    g.SetParams(params);
+    DevBuf dp(n * sizeof(float4)), dn(n * sizeof(float4)), dw(n * sizeof(float4)), dd(n * sizeof(float4));
+    std::vector<float> w(weight, weight + n);
+    UploadF4(dp, n, pos, w.data());
+    UploadF4(dn, n, nrm, nullptr);
+    UploadF4(dw, n, wo, nullptr);
+    UploadF4(dd, n, wi, w.data());
+    for (int k = 0; k < passes; ++k) {
+      g.TrainOnRecords(dp.as<float4>(), dd.as<float4>(), dn.as<float4>(), dw.as<float4>(), n, 0);
+      Check(hipDeviceSynchronize(), "sync");
+      if (loss_per_pass) loss_per_pass[k] = g.LastLoss() / (float)std::max<int64_t>(1, (n + cfg->batch - 1) / cfg->batch);
+    }
+    g.GetParams(params);
+    return TTGPU_OK;
+  })
+}
+
 int ttgpu_guide_info(ttgpu_handle *h, int32_t *cells, int32_t *rounds, int64_t *records_last_pass, int64_t *records_dropped) {
   if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
   if (!h->guided) return Fail(h, TTGPU_ERR_INVALID_ARGUMENT, "not a guided (TTGPU_ALGO_PTG) handle");
-  if (cells) *cells = (int32_t)h->recorder->cells().size();
-  if (rounds) *rounds = h->recorder->round();
+  if (cells) *cells = h->learned ? 0 : (int32_t)h->recorder->cells().size();
+  if (rounds) *rounds = h->train_rounds;
   if (records_last_pass) *records_last_pass = h->records_last_pass;
   if (records_dropped) *records_dropped = h->records_dropped;
   return TTGPU_OK;

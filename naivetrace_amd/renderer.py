@@ -234,6 +234,58 @@ def vmf_pdf_gpu(mixture, dirs):
     return p
 
 
+def _nn_call(fn, *args):
+    lib = abi.load_library()
+    rc = getattr(lib, fn)(*args)
+    if rc:
+        raise TTGpuError(rc, lib.ttgpu_create_error().decode())
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def nn_init_params(cfg):
+    """Initial parameter vector of the learned guide for `cfg` (abi.nn_config)."""
+    n = abi.load_library().ttgpu_nn_param_count(C.byref(cfg))
+    p = np.zeros(n, np.float32)
+    _nn_call("ttgpu_nn_init_params", C.byref(cfg), _f32p(p))
+    return p
+
+
+def nn_forward(cfg, params, pos, nrm, wo):
+    """Raw network outputs (n x 96) on the GPU."""
+    params = _f32(params)
+    pos, nrm, wo = (_f32(a, (-1, 3)) for a in (pos, nrm, wo))
+    out = np.zeros((len(pos), 96), np.float32)
+    _nn_call("ttgpu_nn_forward", C.byref(cfg), _f32p(params), len(pos), _f32p(pos), _f32p(nrm), _f32p(wo), _f32p(out))
+    return out
+
+
+def nn_gradient(cfg, params, pos, nrm, wo, wi, weight):
+    """Gradient of the mean of -weight log q(wi) over the records, and that mean, on the GPU."""
+    params = _f32(params)
+    pos, nrm, wo, wi = (_f32(a, (-1, 3)) for a in (pos, nrm, wo, wi))
+    weight = _f32(weight)
+    grad = np.zeros_like(params)
+    loss = C.c_float()
+    _nn_call("ttgpu_nn_gradient", C.byref(cfg), _f32p(params), len(pos), _f32p(pos), _f32p(nrm), _f32p(wo), _f32p(wi),
+             _f32p(weight), _f32p(grad), C.byref(loss))
+    return grad, loss.value
+
+
+def nn_train(cfg, params, pos, nrm, wo, wi, weight, passes):
+    """Trains a copy of `params` for `passes` passes over the records; returns (params, loss per pass)."""
+    params = _f32(params).copy()
+    pos, nrm, wo, wi = (_f32(a, (-1, 3)) for a in (pos, nrm, wo, wi))
+    weight = _f32(weight)
+    losses = np.zeros(passes, np.float32)
+    _nn_call("ttgpu_nn_train", C.byref(cfg), _f32p(params), len(pos), _f32p(pos), _f32p(nrm), _f32p(wo), _f32p(wi), _f32p(weight),
+             passes, _f32p(losses))
+    return params, losses
+
+
 class RenderingAlgo:
     """Mirror of ToyTrace's RenderingAlgo (src/renderingalgorithms_interface.hxx:13-30) over the GPU library."""
 

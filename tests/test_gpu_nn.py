@@ -1,0 +1,160 @@
+"""The learned guide on the GPU (bf16 MFMA forward/backward, Adam), through the C ABI.
+
+* Forward pass and gradients against the numpy oracle (oracle/ref_nn.py) that
+  rounds to bf16 where the GPU stores bf16; the oracle itself is pinned by finite
+  differences (tests/test_nn.py).  Tolerance: relative L2 error per block of 1e-2
+  (outputs) and 3e-2 (gradients): bf16 re-rounding after fp32 sums taken in a
+  different order moves single values by one bf16 step (2^-8 relative).
+* Training lowers the negative log-likelihood of records whose directions follow
+  a known position-dependent distribution.
+* The guided integrator with the learned guide: unbiased against the oracle's
+  Cornell render, and lower relMSE than pt2 at equal samples on the hidden-light box.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from naivetrace_amd import abi, configs
+from naivetrace_amd.renderer import GpuRenderer, nn_forward, nn_gradient, nn_init_params, nn_train
+from test_gpu_parity import GOLDEN, _zscores
+from test_gpu_guiding import _relmse
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import ref_nn  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(**kw):
+    d = dict(levels=16, log2_table=12, base_res=4.0, max_res=128.0, batch=4096, lr=1e-2, bb_lo=(-1, 0, -1), bb_hi=(1, 2, 1), seed=3)
+    d.update(kw)
+    return abi.nn_config(**d)
+
+
+def _net(cfg):
+    return ref_nn.Net(cfg.levels, cfg.log2_table, cfg.base_res, cfg.max_res, list(cfg.bb_lo), list(cfg.bb_hi))
+
+
+def _records(n, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform([-1, 0, -1], [1, 2, 1], (n, 3))
+    unit = lambda a: a / np.linalg.norm(a, axis=1, keepdims=True)
+    return pos, unit(rng.normal(size=(n, 3))), unit(rng.normal(size=(n, 3))), unit(rng.normal(size=(n, 3))), rng.exponential(1.0, n)
+
+
+def _params(cfg, seed):
+    p = nn_init_params(cfg)
+    L = _net(cfg).L
+    rng = np.random.default_rng(seed)
+    # Non-trivial grid features and larger output weights so every part of the network matters.
+    p[L.grid:L.w1] = rng.normal(0, 0.5, L.w1).astype(np.float32)
+    W4 = p[L.w4:L.b1].reshape(96, 64)
+    W4[:80] *= 10.0
+    return p
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_gpu_nn_forward_matches_oracle(gpu_lib):
+    cfg = _cfg()
+    p = _params(cfg, 1)
+    pos, nrm, wo, _, _ = _records(700, 2)
+    out = nn_forward(cfg, p, pos, nrm, wo)
+    ref = _net(cfg).forward(p, pos, nrm, wo, bf=True)
+    err = _rel(out[:, :80], ref[:, :80])
+    print(f"forward rel err {err:.2e}", flush=True)
+    assert err < 1e-2
+    assert np.all(out[:, 80:] == 0)
+
+
+def test_gpu_nn_gradient_matches_oracle(gpu_lib):
+    cfg = _cfg()
+    net = _net(cfg)
+    L = net.L
+    p = _params(cfg, 4)
+    pos, nrm, wo, wi, w = _records(1000, 5)
+    g, loss = nn_gradient(cfg, p, pos, nrm, wo, wi, w)
+    gr, lr_ = net.gradient(p, pos, nrm, wo, wi, w, bf=True)
+    print(f"loss gpu {loss:.6f} oracle {lr_:.6f}", flush=True)
+    assert abs(loss - lr_) < 1e-3 * abs(lr_) + 1e-5
+    for name, a, b in (("grid", L.grid, L.w1), ("W1", L.w1, L.w2), ("W2", L.w2, L.w3), ("W3", L.w3, L.w4), ("W4", L.w4, L.b1),
+                       ("b1", L.b1, L.b2), ("b2", L.b2, L.b3), ("b3", L.b3, L.b4), ("b4", L.b4, L.total)):
+        err = _rel(g[a:b], gr[a:b])
+        print(f"grad {name}: rel err {err:.2e} |g| {np.linalg.norm(gr[a:b]):.3e}", flush=True)
+        assert err < 3e-2, name
+
+
+def test_gpu_nn_training_lowers_nll(gpu_lib):
+    cfg = _cfg(batch=4096, lr=5e-3)
+    rng = np.random.default_rng(7)
+    n = 16384
+    pos = rng.uniform([-1, 0, -1], [1, 2, 1], (n, 3))
+    nrm = np.tile([0.0, 1.0, 0.0], (n, 1))
+    wo = np.tile([0.0, 0.0, 1.0], (n, 1))
+    # Directions concentrated around a direction that turns with x (a light seen from different places).
+    ang = 1.2 * pos[:, 0]
+    target = np.stack([np.sin(ang), np.cos(ang), np.zeros(n)], axis=1)
+    d = target + 0.15 * rng.normal(size=(n, 3))
+    wi = d / np.linalg.norm(d, axis=1, keepdims=True)
+    w = np.ones(n)
+    p0 = nn_init_params(cfg)
+    p, losses = nn_train(cfg, p0, pos, nrm, wo, wi, w, 40)
+    print("loss per pass", losses[:3], losses[-3:], flush=True)
+    assert np.all(np.isfinite(losses))
+    # The uniform density on the sphere has -log q = log(4 pi) = 2.53; a fitted lobe goes far below.
+    assert losses[-1] < losses[0] - 1.0 and losses[-1] < 1.0
+
+
+def _ptg_learned(w, h, depth, guide_max_spp, spp, seed):
+    p = abi.default_params(w, h, spp=spp, max_depth=depth, seed=seed, algo=abi.ALGO_PTG)
+    p.guide_kind = abi.GUIDE_LEARNED
+    p.guide_max_spp = guide_max_spp
+    return p
+
+
+def test_gpu_learned_ptg_unbiased_cornell(gpu_lib):
+    f = np.load(os.path.join(GOLDEN, "cornell32_consistent_oracle.npz"))
+    w, h, depth, spp = int(f["width"]), int(f["height"]), int(f["max_depth"]), int(f["spp"])
+    with GpuRenderer(configs.cornell_box(w, h), _ptg_learned(w, h, depth, 16, spp, 31)) as r:
+        r.run()
+        rgb, sq, n = r.framebuffer()
+        info = r.guide_info()
+    print("learned guide info", info, flush=True)
+    assert info["rounds"] == 1 + 2 + 4 + 8 + 16 and info["records_last_pass"] > 0
+    assert np.all(n == spp)
+    mean = rgb / spp
+    var = np.maximum(sq / spp - mean ** 2, 0) * spp / (spp - 1)
+    z = _zscores(mean, np.sqrt(var / spp), f["mean"], f["stderr"])
+    frac3 = float(np.mean(np.abs(z) > 3))
+    img, img_ref = mean.mean(axis=(0, 1)), f["mean"].mean(axis=(0, 1))
+    img_se = np.sqrt((var / spp).sum(axis=(0, 1)) + (f["stderr"] ** 2).sum(axis=(0, 1))) / (w * h)
+    print(f"learned ptg cornell: mean z {z.mean():+.4f} |z|>3 {frac3:.4f} image {img} oracle {img_ref} +- {img_se}", flush=True)
+    assert frac3 < 0.015
+    assert abs(z.mean()) < 5.0 / np.sqrt(z.size) + 0.05
+    assert np.all(np.abs(img - img_ref) < 4 * img_se + 1e-3 * np.abs(img_ref))
+
+
+def test_gpu_learned_ptg_reduces_error_hidden_light(gpu_lib):
+    w = h = 64
+    s = configs.cornell_hidden_light(w, h)
+    pref = abi.default_params(w, h, max_depth=8, seed=99)
+    pref.mis_mode = abi.MIS_CONSISTENT
+    with GpuRenderer(s, pref) as r:
+        r.render_spp(32768)
+        ref = r.mean_image()
+    spp = 64
+    pp = abi.default_params(w, h, spp=spp, max_depth=8, seed=5)
+    pp.mis_mode = abi.MIS_CONSISTENT
+    with GpuRenderer(s, pp) as r:
+        r.run()
+        plain = r.mean_image()
+    with GpuRenderer(s, _ptg_learned(w, h, 8, 64, spp, 7)) as r:
+        r.run()
+        learned = r.mean_image()
+    e_plain, e_learned = _relmse(plain, ref), _relmse(learned, ref)
+    print(f"hidden light {spp} spp: relMSE plain {e_plain:.5f} learned {e_learned:.5f} ratio {e_plain / e_learned:.2f}", flush=True)
+    assert e_learned < e_plain

@@ -40,6 +40,11 @@ def parse():
     ap.add_argument("--wavefront", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (PMC byte-counter calibration) and exit")
+    ap.add_argument("--relmse-budget", type=float, default=1.0,
+                    help="seconds per method for the equal-time relMSE comparison (0 = skip)")
+    ap.add_argument("--relmse-size", type=int, default=256)
+    ap.add_argument("--relmse-ref-spp", type=int, default=65536)
+    ap.add_argument("--guide-max-spp", type=int, default=16)
     ap.add_argument("--summarize-pmc", default="",
                     help="print the markdown traffic summary of a directory of rocprofv3 --pmc CSV runs and exit")
     return ap.parse_args()
@@ -116,8 +121,93 @@ def copy_calibration(mib, repeats=4):
     print(json.dumps({"copy_calibration_bytes": n, "copies": repeats}), flush=True)
 
 
-def cpu_baseline(scene, width, height, depth, rows):
-    """Oracle (FP64 C++ restatement of pt2, all host threads) on a bounded sample: `rows` rows x full width x 1 spp."""
+def relmse(img, ref, eps=1e-2):
+    """mean over pixels and RGB channels of (I - I_ref)^2 / (I_ref^2 + eps), linear RGB."""
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def relmse_equal_time(budget_s, size, guide_max_spp, ref_spp):
+    """Cornell box with the light hidden behind a baffle (indirect light dominates): relMSE of plain
+    pt2, vMF-tree guiding and learned guiding after the same wall time each (guide training
+    included; scene upload and BVH build excluded), against a pt2 render with `ref_spp` spp."""
+    import threading
+    from naivetrace_amd import abi, configs
+    from naivetrace_amd.renderer import GpuRenderer
+    scene = configs.cornell_hidden_light(size, size)
+    pref = abi.default_params(size, size, max_depth=8, seed=1234)
+    pref.mis_mode = abi.MIS_CONSISTENT
+    t0 = time.perf_counter()
+    with GpuRenderer(scene, pref) as r:
+        r.render_spp(ref_spp)
+        ref = r.mean_image()
+    t_ref = time.perf_counter() - t0
+    n_sweeps = int(np.log2(guide_max_spp)) if guide_max_spp >= 2 else 0
+
+    def timed(params, guided):
+        with GpuRenderer(scene, params) as r:
+            stamps = []
+            r.set_pass_callback(lambda complete: stamps.append(time.perf_counter()))
+            timer = threading.Timer(budget_s, r.request_stop)
+            t0 = time.perf_counter()
+            timer.start()
+            r.run()
+            timer.cancel()
+            dt = time.perf_counter() - t0
+            _, _, n = r.framebuffer()
+            res = {"relmse": relmse(r.mean_image(), ref), "spp": float(n.mean()), "wall_s": round(dt, 3)}
+            if guided and len(stamps) > n_sweeps:
+                res["training_s"] = round(stamps[n_sweeps] - t0, 3)
+            return res
+
+    out = {"scene": f"cornell_hidden_light {size}x{size}, depth 8", "budget_s": budget_s, "reference_spp": ref_spp,
+           "reference_s": round(t_ref, 2), "guide_max_spp": guide_max_spp}
+    p = abi.default_params(size, size, max_depth=8, seed=5)
+    p.mis_mode = abi.MIS_CONSISTENT
+    out["plain"] = timed(p, False)
+    for name, kind in (("vmf_tree", abi.GUIDE_VMF_TREE), ("learned", abi.GUIDE_LEARNED)):
+        g = abi.default_params(size, size, max_depth=8, seed=6, algo=abi.ALGO_PTG)
+        g.guide_kind = kind
+        g.guide_max_spp = guide_max_spp
+        out[name] = timed(g, True)
+    for name in ("vmf_tree", "learned"):
+        out[f"gain_{name}"] = round(out["plain"]["relmse"] / max(out[name]["relmse"], 1e-30), 3)
+    return out
+
+
+NN_FWD_FLOP_PER_RECORD = 2 * (38 * 64 + 64 * 64 + 64 * 64 + 64 * 80)  # 31,488: dense layers, true widths
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def nn_inference_roofline(size, guide_max_spp):
+    """Batched inference of the learned guide inside the wavefront (k_guide_infer): algorithmic
+    dense-layer FLOPs (NN_FWD_FLOP_PER_RECORD per queued path) over its HIP-event time."""
+    from naivetrace_amd import abi, configs
+    from naivetrace_amd.renderer import GpuRenderer
+    scene = configs.cornell_hidden_light(size, size)
+    g = abi.default_params(size, size, spp=64, max_depth=8, seed=6, algo=abi.ALGO_PTG)
+    g.guide_kind = abi.GUIDE_LEARNED
+    g.guide_max_spp = guide_max_spp
+    with GpuRenderer(scene, g) as r:
+        r.enable_kernel_timing(True)
+        r.run()
+        st = r.stats()
+        infer_ms, infer_n = r.kernel_time("guide")
+        train_ms, train_n = r.kernel_time("train")
+        trace_ms, _ = r.kernel_time("trace")
+    # Every queued path of every bounce is one query; rays_closest and the "guide" timer both
+    # accumulate over all passes of the handle (training and rendering).
+    records = st["rays_closest"]
+    achieved = records * NN_FWD_FLOP_PER_RECORD / max(infer_ms * 1e-3, 1e-12) / 1e12
+    return {"kernel": "k_guide_infer", "bound": "mfma", "unit": "TFLOP/s", "achieved": round(achieved, 2),
+            "peak": BF16_DENSE_PEAK_TFLOPS, "frac": round(achieved / BF16_DENSE_PEAK_TFLOPS, 5),
+            "flop_per_record": NN_FWD_FLOP_PER_RECORD, "queries": records, "launches": infer_n,
+            "avg_launch_ms": round(infer_ms / max(infer_n, 1), 4), "trace_ms_total": round(trace_ms, 2),
+            "train_ms_total": round(train_ms, 2), "train_passes": train_n}
+
+
+def cpu_baseline(scene, width, height, depth, rows, spp=2):
+    """Oracle (FP64 C++ restatement of pt2, all host threads) on a bounded sample: `rows` rows x full
+    width x `spp` spp (the full 1920x1080 frame at 2 spp takes about 10 s on 16 cores)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from naivetrace_amd import abi
@@ -127,12 +217,11 @@ def cpu_baseline(scene, width, height, depth, rows):
     # rows of the full frame (pixel footprint 2 tan(fov/2)/width in both directions).
     p = abi.default_params(width, rows, max_depth=depth, seed=5)
     crop = scene
-    t0 = time.time()
-    oracle_lib.render(crop, p, 1, threads=threads, with_sq=False)
-    dt = time.time() - t0
-    paths = width * rows
+    oracle_lib.render(crop, p, spp, threads=threads, with_sq=False)
+    dt = oracle_lib.last_render_seconds()
+    paths = width * rows * spp
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} rows x {width} px x 1 spp of the same scene ({paths} paths, {dt:.1f} s, oracle BVH build excluded)"}
+            "sample": f"{rows} rows x {width} px x {spp} spp of the same scene ({paths} paths, {dt:.1f} s, oracle BVH build excluded)"}
 
 
 def main():
@@ -197,9 +286,15 @@ def main():
     ktimes = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in times.items()}
     r.close()
 
+    guided = None
+    nn_roof = None
+    if args.relmse_budget > 0:
+        guided = relmse_equal_time(args.relmse_budget, args.relmse_size, args.guide_max_spp, args.relmse_ref_spp)
+        nn_roof = nn_inference_roofline(args.relmse_size, args.guide_max_spp)
+
     cpu = None
     if args.cpu_baseline:
-        rows = args.cpu_sample_rows or 64
+        rows = args.cpu_sample_rows or args.height
         try:
             cpu = cpu_baseline(scene, args.width, args.height, args.depth, rows)
         except Exception as e:  # reported, not fatal for the GPU number
@@ -215,6 +310,8 @@ def main():
                    "triangles": scene.num_triangles, "wavefront_paths": st["wavefront_paths"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "relmse_equal_time": guided,
+        "roofline_nn": nn_roof,
         "detail": {"rays_closest_per_path": round(rays_closest / paths_t, 3), "rays_shadow_per_path": round(rays_shadow / paths_t, 3),
                    "mrays_per_s": round((rays_closest + rays_shadow) / (sum(v[0] for v in times.values()) * 1e-3) / 1e6, 1),
                    "kernel_ms": ktimes, "bvh_build_ms": round(st0["bvh_build_ms"], 1), "scene_gen_s": round(gen_s, 2),

@@ -131,6 +131,7 @@ struct DeviceScene {
   uint64_t cam_tracker;            // packed medium tracker at the camera
   DevCamera cam;
   int sample_mode, mis_mode, max_depth;
+  int persistent;                  // 1: persistent traversal kernels with dynamic ray fetch
   // guiding (guided integrator only)
   const GuideNode *guide_nodes;
   const GuideCell *guide_cells;
@@ -190,7 +191,7 @@ enum FlagBits : uint32_t {
 
 enum Counter {
   kCountA = 0, kCountB = 1, kCountShadow = 2, kCountRecords = 3, kStatClosest = 4, kStatShadow = 5, kStatPaths = 6,
-  kStatRecordsDropped = 7, kNumCounters = 8
+  kStatRecordsDropped = 7, kWorkTrace = 8, kNumCounters = 16
 };
 
 }  // namespace ttgpu

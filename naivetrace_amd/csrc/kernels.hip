@@ -123,6 +123,43 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DeviceScene sc, PathQueue
   if (gtid() == 0) atomicAdd(stats + kStatClosest, count);
 }
 
+// ---- persistent closest-hit traversal with dynamic ray fetch -------------------------------
+constexpr int kStepsPerFetch = 16;  // traversal steps between refills of idle lanes
+
+__global__ void __launch_bounds__(kTraceBlock) k_trace_dyn(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue,
+                                                           const uint32_t *__restrict__ count_ptr, uint32_t *work,
+                                                           uint32_t *stats) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  int *stack = lds_stack + threadIdx.x;
+  int spill[kSpillStack];
+  const uint32_t count = *count_ptr;
+  Trav t;
+  bool active = false;
+  uint32_t i = 0;
+  for (;;) {
+    const bool need = !active;
+    const uint32_t got = wave_append(work, need);
+    if (need && got < count) {
+      i = queue[got];
+      Ray r;
+      r.org = xyz(q.org[i]);
+      r.dir = xyz(q.dir[i]);
+      r.tmax = 3.0e38f;
+      trav_init(t, r);
+      active = true;
+    }
+    if (__ballot(active) == 0) break;
+    for (int it = 0; it < kStepsPerFetch && active; ++it) {
+      if (!trav_step<false>(t, sc.nodes, sc.prims, stack, spill)) {
+        q.hit[i] = t.hit.prim >= 0 ? make_float4(t.hit.t, t.hit.u, t.hit.v, as_float(t.hit.prim))
+                                   : make_float4(0.f, 0.f, 0.f, as_float(-1));
+        active = false;
+      }
+    }
+  }
+  if (gtid() == 0) atomicAdd(stats + kStatClosest, count);
+}
+
 // ---- environment lights (src/light.hxx:148-288, src/light.cxx:29-189) -------------
 __device__ __forceinline__ uint32_t upper_bound(const float *cdf, uint32_t n, float r) {
   uint32_t lo = 0, hi = n;  // first index with r < cdf[i]
@@ -1113,8 +1150,26 @@ void LaunchRaygen(const DeviceScene &sc, const PathQueues &q, uint64_t base, uin
                   uint64_t seed, hipStream_t st) {
   hipLaunchKernelGGL(k_raygen, dim3(Blocks(count, 256, 65535)), dim3(256), 0, st, sc, q, base, count, npix, spp_base, seed);
 }
+// Workgroups that fit on the device at once for a persistent kernel (computed on first use).
+template <class K>
+static int ResidentBlocks(K kernel, int block) {
+  int dev = 0, per_cu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+    return 1024;
+  return per_cu * prop.multiProcessorCount;
+}
+
 void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue, const uint32_t *count, uint32_t max_count,
                  hipStream_t st) {
+  if (sc.persistent) {
+    static const int resident = ResidentBlocks(k_trace_dyn, kTraceBlock);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, q.counters, (int)kWorkTrace);
+    hipLaunchKernelGGL(k_trace_dyn, dim3(Blocks(max_count, kTraceBlock, resident)), dim3(kTraceBlock), 0, st, sc, q, queue, count,
+                       q.counters + kWorkTrace, q.counters);
+    return;
+  }
   hipLaunchKernelGGL(k_trace, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, queue, count,
                      q.counters);
 }

@@ -209,4 +209,82 @@ TT_HD bool traverse(const BvhNode *__restrict__ nodes, const PrimRecord *__restr
   return hit.prim >= 0;
 }
 
+// Resumable traversal for persistent kernels with dynamic ray fetch: a lane advances its ray one
+// node at a time and, when the ray is finished, takes the next one while the rest of the wave keeps
+// traversing (persistent while-while with dynamic fetch, Aila & Laine 2009).
+struct Trav {
+  RayPre p;
+  float3 org, dir;
+  float tmax;
+  HitRec hit;
+  int node, sp;
+};
+
+TT_HD void trav_init(Trav &t, const Ray &r) {
+  t.p = precompute(r);
+  t.org = r.org;
+  t.dir = r.dir;
+  t.tmax = r.tmax;
+  t.hit.prim = -1;
+  t.node = 0;
+  t.sp = 0;
+}
+
+// Processes the current node (testing leaf primitives right away); returns false when the
+// traversal is finished (stack empty, or an occluder found for any-hit queries).
+template <bool kAnyHit>
+TT_HD bool trav_step(Trav &t, const BvhNode *__restrict__ nodes, const PrimRecord *__restrict__ prims, int *stack, int *spill) {
+  const BvhNode n = nodes[t.node];
+  float tl, tr;
+  box2(n, t.p, t.tmax, tl, tr);
+  const int children[2] = {n.ch.x, n.ch.y};
+  const int counts[2] = {n.ch.z, n.ch.w};
+  const float tc[2] = {tl, tr};
+  int inner[2];
+  float tin[2];
+  int ninner = 0;
+  Ray r;
+  r.org = t.org;
+  r.dir = t.dir;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (!(tc[c] <= t.tmax)) continue;
+    if (children[c] < 0) {
+      const int first = ~children[c];
+      for (int k = 0; k < counts[c]; ++k) {
+        float th, u, v;
+        if (intersect_prim(prims[first + k], t.p, r, t.tmax, th, u, v)) {
+          t.tmax = th;
+          t.hit.t = th; t.hit.u = u; t.hit.v = v; t.hit.prim = first + k;
+          if (kAnyHit) return false;
+        }
+      }
+    } else {
+      inner[ninner] = children[c];
+      tin[ninner] = tc[c];
+      ++ninner;
+    }
+  }
+  int next = -1;
+  if (ninner == 2) {
+    const bool swap = tin[1] < tin[0];
+    next = swap ? inner[1] : inner[0];
+    const int far = swap ? inner[0] : inner[1];
+    // The host rejects trees deeper than kLdsStack + kSpillStack, so this bound never drops a node.
+    if (t.sp < kLdsStack) stack[t.sp * kTraceBlock] = far;
+    else if (t.sp < kLdsStack + kSpillStack) spill[t.sp - kLdsStack] = far;
+    ++t.sp;
+    if (t.sp > kLdsStack + kSpillStack) t.sp = kLdsStack + kSpillStack;
+  } else if (ninner == 1) {
+    next = inner[0];
+  }
+  if (next < 0) {
+    if (t.sp == 0) return false;
+    --t.sp;
+    next = t.sp < kLdsStack ? stack[t.sp * kTraceBlock] : spill[t.sp - kLdsStack];
+  }
+  t.node = next;
+  return true;
+}
+
 }  // namespace ttgpu

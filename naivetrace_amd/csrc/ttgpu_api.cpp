@@ -7,6 +7,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -587,6 +588,9 @@ void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
   dc.xmin = (float)-a; dc.ymin = (float)-b; dc.xper = (float)(2. * a / W); dc.yper = (float)(2. * b / H);
   dc.fish_delta = (float)(2. / std::min(W, H));
   dc.width = W; dc.height = H;
+  // Persistent traversal with dynamic ray fetch unless TTGPU_PERSISTENT_TRAVERSAL=0 (one ray per thread).
+  const char *pt = std::getenv("TTGPU_PERSISTENT_TRAVERSAL");
+  h->dsc.persistent = (pt && pt[0] == '0') ? 0 : 1;
   h->dsc.sample_mode = h->params.sample_mode;
   h->dsc.mis_mode = h->params.mis_mode;
   h->dsc.max_depth = h->params.max_ray_depth;

@@ -13,6 +13,7 @@
 //    contribution would be multiplied by zero).
 #include "ref_shading.hpp"
 #include "ttref.h"
+#include <chrono>
 #include <thread>
 #include <atomic>
 #include <mutex>
@@ -941,6 +942,10 @@ static thread_local std::string g_err;
 
 extern "C" const char *ttref_last_error(void) { return g_err.c_str(); }
 
+// Wall time of the path tracing of the last ttref_render call (scene and BVH build excluded).
+static thread_local double g_render_seconds = 0.;
+extern "C" double ttref_last_render_seconds(void) { return g_render_seconds; }
+
 template <int NW>
 static void RenderImpl(const Scene &sc, const ttgpu_params &p, int spp, int spp_offset, int threads, double *rgb, double *rgb_sq) {
   Engine<NW> eng(sc, p);
@@ -979,9 +984,11 @@ extern "C" int ttref_render(const ttgpu_scene *scene, const ttgpu_params *params
     if (!scene || !params || !rgb || spp < 0) throw std::invalid_argument("null argument");
     if (params->width <= 0 || params->height <= 0) throw std::invalid_argument("bad resolution");
     Scene sc(*scene);
+    const auto t0 = std::chrono::steady_clock::now();
     if (params->num_wavelengths == 3) RenderImpl<3>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);
     else if (params->num_wavelengths == 4) RenderImpl<4>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);
     else throw std::invalid_argument("num_wavelengths must be 3 or 4");
+    g_render_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();

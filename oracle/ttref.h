@@ -7,6 +7,8 @@
 extern "C" {
 #endif
 const char *ttref_last_error(void);
+/* Wall time in seconds of the path tracing in the last ttref_render (scene/BVH build excluded). */
+double ttref_last_render_seconds(void);
 /* Adds the sums of `spp` paths per pixel (sample indices spp_offset ..) into rgb
  * (and squared per-path sums into rgb_sq when non-NULL). */
 int ttref_render(const ttgpu_scene *scene, const ttgpu_params *params, int32_t spp, int32_t spp_offset,

@@ -1,0 +1,81 @@
+// Exercises include/ttgpu_algo.hpp from C++: a scene built from plain arrays, the
+// RenderingAlgo interface, exceptions for invalid input.
+//   wrapper_demo invalid   -> builds a scene with a bad material index, expects std::invalid_argument
+//   wrapper_demo render N  -> diffuse plane (albedo 0.6) under a uniform dome (L = 1.5), N spp,
+//                             prints the image-mean linear RGB (closed form: 0.6 * 1.5 per bin)
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "ttgpu_algo.hpp"
+
+int main(int argc, char **argv) {
+  const bool invalid = argc > 1 && !std::strcmp(argv[1], "invalid");
+  const int spp = argc > 2 ? std::atoi(argv[2]) : 16;
+  const float big = 1.0e4f;
+  std::vector<float> v = {-big, 0, big, big, 0, big, big, 0, -big, -big, 0, -big};
+  std::vector<uint32_t> idx = {0, 1, 2, 0, 2, 3};
+  std::vector<int32_t> mat = {0, invalid ? 7 : 0};
+  ttgpu_shader sh;
+  std::memset(&sh, 0, sizeof(sh));
+  sh.kind = TTGPU_SHADER_DIFFUSE;
+  for (float &r : sh.reflectance) r = 0.6f;
+  ttgpu_material m{0, -1, -1, -1};
+  ttgpu_env_light dome;
+  std::memset(&dome, 0, sizeof(dome));
+  dome.kind = TTGPU_ENV_DOME;
+  dome.dir[1] = 1.0;
+  for (float &s : dome.spectrum) s = 1.5f;
+  ttgpu_scene sc;
+  std::memset(&sc, 0, sizeof(sc));
+  sc.num_vertices = 4;
+  sc.vertices = v.data();
+  sc.num_triangles = 2;
+  sc.indices = idx.data();
+  sc.tri_material = mat.data();
+  sc.num_shaders = 1;
+  sc.shaders = &sh;
+  sc.num_materials = 1;
+  sc.materials = &m;
+  sc.empty_space_medium = -1;
+  sc.num_env_lights = 1;
+  sc.env_lights = &dome;
+  sc.camera.kind = TTGPU_CAMERA_PINHOLE;
+  sc.camera.pos[1] = 1.0;
+  sc.camera.dir[1] = -1.0;  // 45 degrees down: with a 30 degree field of view every pixel sees the plane
+  sc.camera.dir[2] = -1.0;
+  sc.camera.up[1] = 1.0;
+  sc.camera.fov_deg = 30.0;
+  ttgpu_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.width = p.height = 32;
+  p.max_ray_depth = 3;
+  p.max_samples_per_pixel = spp;
+  p.num_wavelengths = 4;
+  p.seed = 5;
+  try {
+    auto algo = ttgpu::RenderAlgorithmFactory(sc, p, "pt2");
+    int passes = 0;
+    algo->SetInterruptCallback([&](bool complete) { passes += complete; });
+    algo->InitializeScene();
+    algo->Run();
+    auto *gpu = static_cast<ttgpu::GpuRenderingAlgo *>(algo.get());
+    std::vector<double> rgb;
+    std::vector<uint64_t> n;
+    gpu->ReadFramebuffer(rgb, n);
+    double mean[3] = {0, 0, 0};
+    for (size_t i = 0; i < n.size(); ++i)
+      for (int c = 0; c < 3; ++c) mean[c] += rgb[3 * i + c] / (double)n[i] / (double)n.size();
+    auto img = algo->GenerateImage();
+    std::printf("passes %d spp %llu mean %.6f %.6f %.6f image %dx%d\n", passes, (unsigned long long)n[0], mean[0], mean[1], mean[2],
+                img->width, img->height);
+  } catch (const std::invalid_argument &e) {
+    std::printf("invalid_argument: %s\n", e.what());
+    return invalid ? 0 : 1;
+  } catch (const std::exception &e) {
+    std::printf("error: %s\n", e.what());
+    return 1;
+  }
+  return invalid ? 1 : 0;
+}

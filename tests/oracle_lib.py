@@ -29,6 +29,7 @@ def lib():
         P = C.POINTER
         f64p, i32p = P(C.c_double), P(C.c_int32)
         L.ttref_last_error.restype = C.c_char_p
+        L.ttref_last_render_seconds.restype = C.c_double
         L.ttref_render.argtypes = [P(abi.SceneDesc), P(abi.Params), C.c_int32, C.c_int32, C.c_int32, f64p, f64p]
         L.ttref_sample_shader.argtypes = [P(abi.Shader), f64p, f64p, f64p, i32p, C.c_double, C.c_uint64, C.c_int64,
                                           f64p, f64p, f64p]
@@ -73,6 +74,10 @@ def render(scene, params, spp, spp_offset=0, threads=0, with_sq=True):
     if rc != 0:
         raise RuntimeError(lib().ttref_last_error().decode())
     return rgb, sq
+
+
+def last_render_seconds():
+    return lib().ttref_last_render_seconds()
 
 
 def sample_shader(shader, wi, ng, ns, bins, roughness, seed, count):

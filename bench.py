@@ -5,8 +5,9 @@ Workload (the at-scale configuration): seeded procedural scene of 1,008,288
 triangles (displaced 512x512 heightfield + 200 geodesic icospheres), GGX
 metal / glass / mirror / Lambert materials, procedural 1024x512 HDR sky with a
 sun disc, pinhole 1920x1080, max depth 16, 4 wavelengths per path, FP32
-tracing/shading, FP64 framebuffer.  One step = one sample per pixel over the
-full frame (2,073,600 camera paths).  Prints ONE JSON line.
+tracing/shading, FP64 framebuffer.  One step = one pass of 8 samples per pixel
+over the full frame (16,588,800 camera paths through the regenerating wavefront);
+the default 32 steps render the 256-spp frame.  Prints ONE JSON line.
 """
 import argparse
 import json
@@ -30,8 +31,10 @@ SHADOW_BYTES_PER_RAY = 36 + 32  # shadow ray 32 + contribution 16 + flags, radia
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp-per-step", type=int, default=8,
+                    help="samples per pixel of one step (one pass over the frame); 32 steps x 8 = the 256-spp frame")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=16)
@@ -245,30 +248,31 @@ def main():
     p.wavefront_paths = args.wavefront
     r = GpuRenderer(scene, p)
     st0 = r.stats()
+    spp = args.spp_per_step
     for _ in range(args.warmup):
-        r.render_spp(1)
+        r.render_spp(spp)
     r.reset()
     r.enable_kernel_timing(False)
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        r.render_spp(1)
+        r.render_spp(spp)
     t2 = time.perf_counter()
     st = r.stats()
     dt = t2 - t1
-    paths = args.width * args.height * args.steps
+    paths = args.width * args.height * args.steps * spp
     value = paths / dt / 1e6
 
     # Kernel timing (HIP events on the library's own stream) over a separate timed pass.
     r.enable_kernel_timing(True)
     c0 = r.stats()
-    nt = 2
+    nt = 1
     for _ in range(nt):
-        r.render_spp(1)
+        r.render_spp(spp)
     c1 = r.stats()
     times = {k: r.kernel_time(k) for k in ("raygen", "trace", "shade", "shadow")}
     rays_closest = c1["rays_closest"] - c0["rays_closest"]
     rays_shadow = c1["rays_shadow"] - c0["rays_shadow"]
-    paths_t = args.width * args.height * nt
+    paths_t = args.width * args.height * nt * spp
     dom = max(times, key=lambda k: times[k][0])
     ms_tot, launches = times[dom]
     avg_ms = ms_tot / max(launches, 1)
@@ -306,7 +310,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "terrain_instances: 1,008,288 tris (512x512 heightfield + 200 icospheres f=11), GGX/glass/mirror/"
                                "Lambert, 1024x512 HDR sky + sun disc, pinhole, 4 wavelengths/path",
-                   "width": args.width, "height": args.height, "spp_per_step": 1, "max_depth": args.depth,
+                   "width": args.width, "height": args.height, "spp_per_step": spp, "spp_total": spp * args.steps, "max_depth": args.depth,
                    "triangles": scene.num_triangles, "wavefront_paths": st["wavefront_paths"]},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -161,6 +161,10 @@ struct PathQueues {
   // queues
   uint32_t *queue_a, *queue_b, *shadow_queue;
   uint32_t *counters;   // [0] queue_a count, [1] queue_b count, [2] shadow count, [3..] stats
+  // path regeneration: slots of the paths finished in the current iteration, and the pass's
+  // progress ([0] camera paths issued so far, [1] paths queued for the next iteration)
+  uint32_t *free_slots;
+  unsigned long long *progress;
   // guided integrator: per-vertex state, depth-major ([vertex * capacity + path]), and records
   float4 *gv_pos_pdf;   // scatter position, mixture pdf of the sampled direction (<= 0: no record)
   float4 *gv_dir_acc;   // sampled direction, accumulated incident radiance (mean over wavelengths)
@@ -191,7 +195,7 @@ enum FlagBits : uint32_t {
 
 enum Counter {
   kCountA = 0, kCountB = 1, kCountShadow = 2, kCountRecords = 3, kStatClosest = 4, kStatShadow = 5, kStatPaths = 6,
-  kStatRecordsDropped = 7, kWorkTrace = 8, kNumCounters = 16
+  kStatRecordsDropped = 7, kWorkTrace = 8, kCountFree = 9, kNumCounters = 16
 };
 
 }  // namespace ttgpu

@@ -31,13 +31,6 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
 __device__ __forceinline__ uint32_t grid_stride() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-__global__ void k_set_counters(uint32_t *counters, uint32_t a, uint32_t b, uint32_t shadow) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    counters[kCountA] = a;
-    counters[kCountB] = b;
-    counters[kCountShadow] = shadow;
-  }
-}
 __global__ void k_reset_counter(uint32_t *counters, int which) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counters[which] = 0;
 }
@@ -64,44 +57,92 @@ __device__ __forceinline__ bool camera_ray(const DevCamera &c, int px, int py, f
   return false;
 }
 
-// Camera paths for global indices [base, base+count): pixel = g % npix, sample = spp_base + g / npix.
+// Camera path number g of the pass in path slot `slot`: pixel = g % npix, sample = spp_base + g / npix.
 // Lambda selection: LambdaSelectionStrategy::WithWeights (src/shader_util.hxx:83-91) with 4 strata.
-__global__ void __launch_bounds__(256) k_raygen(DeviceScene sc, PathQueues q, uint64_t base, uint32_t count, uint32_t npix,
-                                                uint32_t spp_base, uint64_t seed) {
+__device__ __forceinline__ void init_camera_path(const DeviceScene &sc, const PathQueues &q, uint32_t i, uint64_t g, uint32_t npix,
+                                                 uint32_t spp_base, uint64_t seed) {
+  const uint32_t pixel = (uint32_t)(g % npix);
+  const uint64_t sample = spp_base + g / npix;
+  Pcg rng;
+  rng.seed(seed, sample * npix + pixel);
+  int main_idx = (int)rng.bounded(kStrata);
+  int idx[4] = {main_idx, main_idx + kStrata, main_idx + 2 * kStrata, main_idx + 3 * kStrata};
+  int rot = (int)rng.bounded(4);
+  int t = idx[0]; idx[0] = idx[rot]; idx[rot] = t;
+  const float lambda0 = 370.f + 10.f * idx[0] + 10.f * rng.uniform();
+  rng.uniform(); rng.uniform(); rng.uniform();  // the other three jittered wavelengths (unused)
+  uint32_t bins = (uint32_t)idx[0] | ((uint32_t)idx[1] << 8) | ((uint32_t)idx[2] << 16) | ((uint32_t)idx[3] << 24);
+  const int px = pixel % sc.cam.width, py = pixel / sc.cam.width;
+  float r0 = rng.uniform(), r1 = rng.uniform();
+  float3 dir;
+  bool ok = camera_ray(sc.cam, px, py, r0, r1, dir);
+  q.org[i] = make_float4(sc.cam.pos.x, sc.cam.pos.y, sc.cam.pos.z, lambda0);
+  q.dir[i] = make_float4(dir.x, dir.y, dir.z, 0.f);
+  q.thru[i] = ok ? f4((float)kStrata) : f4(0.f);
+  q.rad[i] = f4(0.f);
+  q.bins[i] = bins;
+  q.pixel[i] = pixel;
+  q.rng_state[i] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+  q.rng_inc[i] = make_uint2((uint32_t)rng.inc, (uint32_t)(rng.inc >> 32));
+  q.flags[i] = 2u;  // node count 2: camera position and direction nodes (pt2.cxx:388-406)
+  q.last_pdf[i] = 0.f;
+  q.roughness[i] = 0.f;
+  q.tracker[i] = make_uint2((uint32_t)sc.cam_tracker, (uint32_t)(sc.cam_tracker >> 32));
+  if (q.wt_r) {
+    q.wt_r[i] = f4(1.f);
+    q.wt_c[i] = f4(1.f);
+  }
+}
+
+// The first `count` camera paths of a pass fill slots [0, count).
+__global__ void __launch_bounds__(256) k_raygen(DeviceScene sc, PathQueues q, uint32_t count, uint32_t npix, uint32_t spp_base,
+                                                uint64_t seed) {
   for (uint32_t i = gtid(); i < count; i += grid_stride()) {
-    const uint64_t g = base + i;
-    const uint32_t pixel = (uint32_t)(g % npix);
-    const uint64_t sample = spp_base + g / npix;
-    Pcg rng;
-    rng.seed(seed, sample * npix + pixel);
-    int main_idx = (int)rng.bounded(kStrata);
-    int idx[4] = {main_idx, main_idx + kStrata, main_idx + 2 * kStrata, main_idx + 3 * kStrata};
-    int rot = (int)rng.bounded(4);
-    int t = idx[0]; idx[0] = idx[rot]; idx[rot] = t;
-    const float lambda0 = 370.f + 10.f * idx[0] + 10.f * rng.uniform();
-    rng.uniform(); rng.uniform(); rng.uniform();  // the other three jittered wavelengths (unused)
-    uint32_t bins = (uint32_t)idx[0] | ((uint32_t)idx[1] << 8) | ((uint32_t)idx[2] << 16) | ((uint32_t)idx[3] << 24);
-    const int px = pixel % sc.cam.width, py = pixel / sc.cam.width;
-    float r0 = rng.uniform(), r1 = rng.uniform();
-    float3 dir;
-    bool ok = camera_ray(sc.cam, px, py, r0, r1, dir);
-    q.org[i] = make_float4(sc.cam.pos.x, sc.cam.pos.y, sc.cam.pos.z, lambda0);
-    q.dir[i] = make_float4(dir.x, dir.y, dir.z, 0.f);
-    q.thru[i] = ok ? f4((float)kStrata) : f4(0.f);
-    q.rad[i] = f4(0.f);
-    q.bins[i] = bins;
-    q.pixel[i] = pixel;
-    q.rng_state[i] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
-    q.rng_inc[i] = make_uint2((uint32_t)rng.inc, (uint32_t)(rng.inc >> 32));
-    q.flags[i] = 2u;  // node count 2: camera position and direction nodes (pt2.cxx:388-406)
-    q.last_pdf[i] = 0.f;
-    q.roughness[i] = 0.f;
-    q.tracker[i] = make_uint2((uint32_t)sc.cam_tracker, (uint32_t)(sc.cam_tracker >> 32));
-    if (q.wt_r) {
-      q.wt_r[i] = f4(1.f);
-      q.wt_c[i] = f4(1.f);
-    }
+    init_camera_path(sc, q, i, i, npix, spp_base, seed);
     q.queue_a[i] = i;
+  }
+}
+
+// Sets up the counters of a pass whose first `count` paths are in queue A.
+__global__ void k_begin_pass(PathQueues q, uint32_t count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    q.counters[kCountA] = count;
+    q.counters[kCountB] = 0;
+    q.counters[kCountShadow] = 0;
+    q.counters[kCountFree] = 0;
+    q.progress[0] = count;
+    q.progress[1] = count;
+  }
+}
+
+// Path regeneration: the slots freed in this iteration take the next camera paths of the pass (up
+// to `limit` paths in total), queued after the surviving paths for the next trace.
+__device__ __forceinline__ uint32_t regen_count(const PathQueues &q, uint64_t limit, uint64_t issued) {
+  const uint64_t left = limit > issued ? limit - issued : 0;
+  return (uint32_t)min((uint64_t)q.counters[kCountFree], left);
+}
+__global__ void __launch_bounds__(256) k_regen(DeviceScene sc, PathQueues q, uint32_t *__restrict__ queue_out,
+                                               const uint32_t *__restrict__ count_out, uint64_t limit, uint32_t npix,
+                                               uint32_t spp_base, uint64_t seed) {
+  const uint64_t issued = q.progress[0];
+  const uint32_t n = regen_count(q, limit, issued);
+  const uint32_t out0 = *count_out;
+  for (uint32_t k = gtid(); k < n; k += grid_stride()) {
+    const uint32_t slot = q.free_slots[k];
+    init_camera_path(sc, q, slot, issued + k, npix, spp_base, seed);
+    queue_out[out0 + k] = slot;
+  }
+}
+// Runs after k_regen (same stream): commits the issued paths and the queue length.
+__global__ void k_regen_commit(PathQueues q, uint32_t *count_out, uint64_t limit) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const uint64_t issued = q.progress[0];
+    const uint32_t n = regen_count(q, limit, issued);
+    const uint32_t total = *count_out + n;
+    *count_out = total;
+    q.progress[0] = issued + n;
+    q.progress[1] = total;
+    q.counters[kCountFree] = 0;
   }
 }
 
@@ -329,7 +370,9 @@ __device__ __forceinline__ SurfHit make_hit(const DeviceScene &sc, int prim, flo
   return h;
 }
 
+// Adds a finished path's radiance to its pixel and hands its slot to path regeneration.
 __device__ __forceinline__ void finalize_path(const PathQueues &q, double *fb, double *fb_sq, uint32_t i, float4 rad) {
+  q.free_slots[atomicAdd(q.counters + kCountFree, 1u)] = i;
   const uint32_t p = q.pixel[i];
   atomicAdd(fb + 3 * (size_t)p + 0, (double)rad.x);
   atomicAdd(fb + 3 * (size_t)p + 1, (double)rad.y);
@@ -1137,18 +1180,22 @@ static int Blocks(uint64_t n, int block, int cap) {
   return (int)(b > (uint64_t)cap ? cap : b);
 }
 
-void LaunchSetCounters(uint32_t *counters, uint32_t a, uint32_t b, uint32_t s, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, counters, a, b, s);
-}
 void LaunchResetCounter(uint32_t *counters, int which, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counters, which);
 }
 void LaunchFill(double *p, size_t n, double v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_f64, dim3(Blocks(n, 256, 4096)), dim3(256), 0, st, p, n, v);
 }
-void LaunchRaygen(const DeviceScene &sc, const PathQueues &q, uint64_t base, uint32_t count, uint32_t npix, uint32_t spp_base,
-                  uint64_t seed, hipStream_t st) {
-  hipLaunchKernelGGL(k_raygen, dim3(Blocks(count, 256, 65535)), dim3(256), 0, st, sc, q, base, count, npix, spp_base, seed);
+void LaunchRaygen(const DeviceScene &sc, const PathQueues &q, uint32_t count, uint32_t npix, uint32_t spp_base, uint64_t seed,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_raygen, dim3(Blocks(count, 256, 65535)), dim3(256), 0, st, sc, q, count, npix, spp_base, seed);
+  hipLaunchKernelGGL(k_begin_pass, dim3(1), dim3(64), 0, st, q, count);
+}
+void LaunchRegen(const DeviceScene &sc, const PathQueues &q, uint32_t *queue_out, uint32_t *count_out, uint32_t max_count,
+                 uint64_t limit, uint32_t npix, uint32_t spp_base, uint64_t seed, hipStream_t st) {
+  hipLaunchKernelGGL(k_regen, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_out, count_out, limit, npix, spp_base,
+                     seed);
+  hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, st, q, count_out, limit);
 }
 // Workgroups that fit on the device at once for a persistent kernel (computed on first use).
 template <class K>

@@ -7,11 +7,15 @@
 namespace ttgpu {
 
 hipError_t UploadSpectralTables(const float *to_rgb, const float *smits);
-void LaunchSetCounters(uint32_t *counters, uint32_t a, uint32_t b, uint32_t s, hipStream_t st);
 void LaunchResetCounter(uint32_t *counters, int which, hipStream_t st);
 void LaunchFill(double *p, size_t n, double v, hipStream_t st);
-void LaunchRaygen(const DeviceScene &sc, const PathQueues &q, uint64_t base, uint32_t count, uint32_t npix, uint32_t spp_base,
-                  uint64_t seed, hipStream_t st);
+// First `count` camera paths of a pass into slots [0, count) of queue A; resets the pass counters.
+void LaunchRaygen(const DeviceScene &sc, const PathQueues &q, uint32_t count, uint32_t npix, uint32_t spp_base, uint64_t seed,
+                  hipStream_t st);
+// Path regeneration after an iteration's shadow rays: freed slots take the pass's next camera paths
+// (at most `limit` issued in total) and are queued after the survivors in queue_out.
+void LaunchRegen(const DeviceScene &sc, const PathQueues &q, uint32_t *queue_out, uint32_t *count_out, uint32_t max_count,
+                 uint64_t limit, uint32_t npix, uint32_t spp_base, uint64_t seed, hipStream_t st);
 void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue, const uint32_t *count, uint32_t max_count,
                  hipStream_t st);
 void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,

@@ -88,6 +88,9 @@ struct ttgpu_handle {
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_events;
   std::vector<hipEvent_t> event_pool;
   size_t event_next = 0;
+  // path-regeneration progress read-back: two pinned slots of {issued, queued} and their events
+  unsigned long long *poll = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   int npix = 0;
   uint32_t capacity = 0;
   // host copies needed later
@@ -150,6 +153,9 @@ struct ttgpu_handle {
   ~ttgpu_handle() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto e : event_pool) (void)hipEventDestroy(e);
+    for (auto e : poll_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (poll) (void)hipHostFree(poll);
     for (void *p : allocs) (void)hipFree(p);
     if (d_guide_nodes) (void)hipFree(d_guide_nodes);
     if (d_guide_cells) (void)hipFree(d_guide_cells);
@@ -610,6 +616,11 @@ void AllocateQueues(ttgpu_handle *h) {
   q.queue_a = h->Alloc<uint32_t>(N); q.queue_b = h->Alloc<uint32_t>(N); q.shadow_queue = h->Alloc<uint32_t>(N);
   q.counters = h->Alloc<uint32_t>(kNumCounters);
   Check(hipMemset(q.counters, 0, kNumCounters * sizeof(uint32_t)), "hipMemset counters");
+  q.free_slots = h->Alloc<uint32_t>(N);
+  q.progress = h->Alloc<unsigned long long>(2);
+  Check(hipMemset(q.progress, 0, 2 * sizeof(unsigned long long)), "hipMemset progress");
+  Check(hipHostMalloc((void **)&h->poll, 4 * sizeof(unsigned long long)), "hipHostMalloc");
+  for (auto &e : h->poll_ev) Check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
   if (h->dsc.has_media) {
     q.wt_r = h->Alloc<float4>(N);
     q.wt_c = h->Alloc<float4>(N);
@@ -756,58 +767,84 @@ void ResetFramebuffer(ttgpu_handle *h) {
   std::fill(h->pixel_spp.begin(), h->pixel_spp.end(), 0);
 }
 
-// Renders `spp` samples per pixel; returns false if a stop cut the pass short.
+// Renders `spp` samples per pixel as one stream of npix * spp camera paths through the wavefront:
+// the first `capacity` paths start at once, and after every iteration (trace, shade, shadow rays)
+// the slots of finished paths take the next camera paths, so launches stay full until the pass's
+// last paths drain.  The host runs one iteration ahead of a 16-byte progress read-back.  A stop
+// request ends the issuing of new paths; the paths in flight finish.  Returns false after a stop.
 bool RenderPass(ttgpu_handle *h, int spp) {
+  if (h->stop.load()) return false;
   const uint64_t npix = (uint64_t)h->npix;
   const uint64_t total = npix * (uint64_t)spp;
   const uint32_t cap = h->capacity;
-  const int bounces = h->params.max_ray_depth - 1;
   PathQueues &q = h->q;
-  auto t0 = std::chrono::steady_clock::now();
   hipEvent_t ev0 = h->NextEvent(), ev1 = h->NextEvent();
   Check(hipEventRecord(ev0, h->stream), "hipEventRecord");
-  bool complete = true;
   if (h->dsc.guide_record) {
     LaunchResetCounter(q.counters, kCountRecords, h->stream);
     LaunchResetCounter(q.counters, kStatRecordsDropped, h->stream);
   }
-  for (uint64_t base = 0; base < total; base += cap) {
-    if (h->stop.load()) { complete = false; break; }
+  const uint32_t n0 = (uint32_t)std::min<uint64_t>(cap, total);
+  const uint32_t spp_base = (uint32_t)h->total_spp;
+  h->Timed("raygen", [&] { LaunchRaygen(h->dsc, q, n0, (uint32_t)npix, spp_base, h->params.seed, h->stream); });
+  // Every path finishes within max_depth - 1 iterations, and then its slot takes a new path: at
+  // most ceil(total / capacity) + 1 rounds of max_depth - 1 iterations.
+  const uint64_t max_iter = (total / cap + 2) * (uint64_t)std::max(1, h->params.max_ray_depth) + 8;
+  uint64_t limit = total;
+  uint32_t *cur_q = q.queue_a, *nxt_q = q.queue_b;
+  uint32_t *cur_c = q.counters + kCountA, *nxt_c = q.counters + kCountB;
+  int nxt_idx = kCountB;
+  bool stopped = false;
+  for (uint64_t it = 0;; ++it) {
+    if (it > max_iter) throw std::runtime_error("path regeneration did not drain (internal error)");
+    if (!stopped && h->stop.load()) {
+      stopped = true;
+      limit = 0;  // issue no further camera paths; the paths in flight finish
+    }
+    h->Timed("trace", [&] { LaunchTrace(h->dsc, q, cur_q, cur_c, cap, h->stream); });
+    if (h->learned) h->Timed("guide", [&] { LaunchGuideInfer(h->dsc, q, cur_q, cur_c, cap, h->net->View(), h->stream); });
+    LaunchResetCounter(q.counters, nxt_idx, h->stream);
+    LaunchResetCounter(q.counters, kCountShadow, h->stream);
+    h->Timed("shade", [&] { LaunchShade(h->dsc, q, cur_q, cur_c, nxt_q, nxt_c, cap, h->fb, h->fb_sq, h->stream); });
+    h->Timed("shadow", [&] { LaunchShadow(h->dsc, q, cap, h->fb, h->fb_sq, h->stream); });
+    h->Timed("raygen", [&] {
+      LaunchRegen(h->dsc, q, nxt_q, nxt_c, cap, limit, (uint32_t)npix, spp_base, h->params.seed, h->stream);
+    });
+    Check(hipGetLastError(), "kernel launch");
+    const int slot = (int)(it & 1);
+    Check(hipMemcpyAsync(h->poll + 2 * slot, q.progress, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream),
+          "progress read-back");
+    Check(hipEventRecord(h->poll_ev[slot], h->stream), "hipEventRecord");
+    std::swap(cur_q, nxt_q);
+    std::swap(cur_c, nxt_c);
+    nxt_idx = nxt_idx == kCountA ? kCountB : kCountA;
     if (h->interrupt.exchange(false)) {
       Check(hipStreamSynchronize(h->stream), "sync");
       if (h->cb) h->cb(h->cb_user, 0);
     }
-    const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - base);
-    h->Timed("raygen", [&] { LaunchRaygen(h->dsc, q, base, n, (uint32_t)npix, (uint32_t)h->total_spp, h->params.seed, h->stream); });
-    LaunchSetCounters(q.counters, n, 0, 0, h->stream);
-    uint32_t *cur_q = q.queue_a, *nxt_q = q.queue_b;
-    uint32_t *cur_c = q.counters + kCountA, *nxt_c = q.counters + kCountB;
-    int nxt_idx = kCountB, cur_idx = kCountA;
-    for (int b = 0; b < bounces; ++b) {
-      h->Timed("trace", [&] { LaunchTrace(h->dsc, q, cur_q, cur_c, n, h->stream); });
-      if (h->learned) h->Timed("guide", [&] { LaunchGuideInfer(h->dsc, q, cur_q, cur_c, n, h->net->View(), h->stream); });
-      LaunchResetCounter(q.counters, nxt_idx, h->stream);
-      LaunchResetCounter(q.counters, kCountShadow, h->stream);
-      h->Timed("shade", [&] { LaunchShade(h->dsc, q, cur_q, cur_c, nxt_q, nxt_c, n, h->fb, h->fb_sq, h->stream); });
-      h->Timed("shadow", [&] { LaunchShadow(h->dsc, q, n, h->fb, h->fb_sq, h->stream); });
-      std::swap(cur_q, nxt_q);
-      std::swap(cur_c, nxt_c);
-      std::swap(cur_idx, nxt_idx);
+    // Wait for the previous iteration (this one is already queued behind it): an empty queue after
+    // regeneration means every issued path has finished and none is left to issue.
+    if (it > 0) {
+      const int prev = 1 - slot;
+      Check(hipEventSynchronize(h->poll_ev[prev]), "hipEventSynchronize");
+      if (h->poll[2 * prev + 1] == 0) break;
     }
-    Check(hipGetLastError(), "kernel launch");
-    // Per-pixel sample counts of this chunk (global index g -> pixel g % npix).
-    uint64_t full = n / npix, rem = n % npix, start = base % npix;
-    if (full)
-      for (auto &c : h->pixel_spp) c += full;
-    for (uint64_t k = 0; k < rem; ++k) h->pixel_spp[(start + k) % npix] += 1;
-    h->stats.paths_traced += n;
   }
   Check(hipEventRecord(ev1, h->stream), "hipEventRecord");
   Check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  const uint64_t issued = std::max(h->poll[0], h->poll[2]);
+  const bool complete = issued == total;
+  // Per-pixel sample counts: path g of the pass belongs to pixel g % npix.
+  {
+    const uint64_t full = issued / npix, rem = issued % npix;
+    if (full)
+      for (auto &c : h->pixel_spp) c += full;
+    for (uint64_t k = 0; k < rem; ++k) h->pixel_spp[k] += 1;
+  }
+  h->stats.paths_traced += issued;
   float ms = 0.f;
   Check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
   h->stats.render_ms = ms;
-  (void)t0;
   h->CollectTimers();
   h->event_next = 0;
   uint32_t counters[kNumCounters];

@@ -12,7 +12,8 @@ import os
 
 NBINS = 36
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libttgpu.so")
+# TTGPU_LIBRARY names another build of the library (for example one compiled with other tuning flags).
+LIB_PATH = os.environ.get("TTGPU_LIBRARY") or os.path.join(_HERE, "libttgpu.so")
 
 # enums (mirror include/ttgpu.h)
 SHADER_DIFFUSE, SHADER_MIRROR, SHADER_DIELECTRIC, SHADER_BECKMANN_METAL, SHADER_GGX_METAL, \

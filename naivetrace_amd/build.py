@@ -27,14 +27,16 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, extra_flags=()):
+    """Compiles every source and links libttgpu.so (or `out`); `extra_flags` go to every hipcc compile."""
+    out = out or OUT
     cc = hipcc()
     tmp = tempfile.mkdtemp(prefix="ttgpu_build_")
     objs = []
     try:
         for src in HIP_SOURCES:
             obj = os.path.join(tmp, src + ".o")
-            cmd = [cc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + HIP_FLAGS
+            cmd = [cc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + HIP_FLAGS + list(extra_flags)
             if verbose:
                 print(" ".join(cmd))
             subprocess.run(cmd, check=True)
@@ -44,13 +46,13 @@ def build(verbose=False):
             cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj]
             subprocess.run(cmd, check=True)
             objs.append(obj)
-        out_tmp = OUT + ".tmp"
+        out_tmp = out + ".tmp"
         cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", out_tmp] + objs + ["-lpthread"]
         subprocess.run(cmd, check=True)
-        os.replace(out_tmp, OUT)
+        os.replace(out_tmp, out)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    return OUT
+    return out
 
 
 if __name__ == "__main__":

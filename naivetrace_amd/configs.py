@@ -152,7 +152,7 @@ def sky_texture(w=1024, h=512, sun_dir=(0.3, 0.6, 0.2)):
     return tex.astype(np.float32)
 
 
-def terrain_instances(width=1920, height=1080, seed=1, cells=512, n_spheres=200, freq=11):
+def terrain_instances(width=1920, height=1080, seed=1, cells=512, n_spheres=200, freq=11, sky=(1024, 512)):
     rng = np.random.default_rng(seed)
     s = Scene()
     size = 100.0
@@ -180,7 +180,7 @@ def terrain_instances(width=1920, height=1080, seed=1, cells=512, n_spheres=200,
         gj = int((z + size / 2) / size * cells)
         y = hmap[min(gj, n - 1), min(gi, n - 1)] + r * rng.uniform(0.2, 1.0)
         s.add_mesh(sv * r + np.array([x, y, z]), sf, random_material(), normals=sv)
-    s.add_env_map([0, 1, 0], sky_texture())
+    s.add_env_map([0, 1, 0], sky_texture(*sky))
     s.set_camera([0.0, 30.0, 70.0], [0.0, 5.0, 0.0], [0.0, 1.0, 0.0], 60.0)
     s.width, s.height = width, height
     return s

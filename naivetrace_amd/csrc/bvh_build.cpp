@@ -206,4 +206,81 @@ BuiltBvh BuildBvh(const std::vector<BuildPrim> &prims, int max_leaf, int num_thr
   return out;
 }
 
+namespace {
+
+// One child entry of a BVH2 node in the device layout (16 words: lxy, rxy, lrz, ch).
+struct Child2 {
+  float lo[3], hi[3];
+  int ref;    // >= 0: BVH2 inner node, < 0: leaf ~first primitive
+  int count;  // leaf primitives
+  float Area() const {
+    float d[3];
+    for (int k = 0; k < 3; ++k) d[k] = std::max(0.f, hi[k] - lo[k]);
+    return d[0] * d[1] + d[1] * d[2] + d[2] * d[0];
+  }
+};
+
+Child2 ChildOf(const BuiltBvh &b, int node, int side) {
+  const float *n = &b.nodes[16 * (size_t)node];
+  const float *xy = n + 4 * side;
+  const float *z = n + 8 + 2 * side;
+  int ch[4];
+  std::memcpy(ch, n + 12, sizeof(ch));
+  Child2 c;
+  c.lo[0] = xy[0]; c.hi[0] = xy[1]; c.lo[1] = xy[2]; c.hi[1] = xy[3]; c.lo[2] = z[0]; c.hi[2] = z[1];
+  c.ref = ch[side];
+  c.count = c.ref < 0 ? ch[2 + side] : 0;
+  return c;
+}
+
+struct Collapser {
+  const BuiltBvh &b2;
+  BuiltBvh4 out;
+  int Emit(int n2, int depth) {
+    out.max_depth = std::max(out.max_depth, depth);
+    std::vector<Child2> ch{ChildOf(b2, n2, 0), ChildOf(b2, n2, 1)};
+    while (ch.size() < 4) {
+      int best = -1;
+      float best_area = -1.f;
+      for (int i = 0; i < (int)ch.size(); ++i)
+        if (ch[i].ref >= 0 && ch[i].Area() > best_area) { best = i; best_area = ch[i].Area(); }
+      if (best < 0) break;
+      const int r = ch[best].ref;
+      ch.erase(ch.begin() + best);
+      ch.push_back(ChildOf(b2, r, 0));
+      ch.push_back(ChildOf(b2, r, 1));
+    }
+    const int id = out.num_nodes++;
+    out.nodes.resize(32 * (size_t)out.num_nodes, 0.f);
+    int child[4], count[4];
+    float box[6][4];
+    for (int s = 0; s < 4; ++s) {
+      if (s < (int)ch.size()) {
+        for (int k = 0; k < 3; ++k) { box[2 * k][s] = ch[s].lo[k]; box[2 * k + 1][s] = ch[s].hi[k]; }
+        child[s] = ch[s].ref >= 0 ? Emit(ch[s].ref, depth + 1) : ch[s].ref;
+        count[s] = ch[s].count;
+      } else {
+        // Empty slot: a box at +infinity misses every ray.
+        for (int k = 0; k < 6; ++k) box[k][s] = kInf;
+        child[s] = ~0;
+        count[s] = 0;
+      }
+    }
+    float *n = &out.nodes[32 * (size_t)id];
+    // Rows: lo.x, hi.x, lo.y, hi.y, lo.z, hi.z (4 children each), child indices, leaf counts.
+    for (int k = 0; k < 6; ++k) std::memcpy(n + 4 * k, box[k], sizeof(box[k]));
+    std::memcpy(n + 24, child, sizeof(child));
+    std::memcpy(n + 28, count, sizeof(count));
+    return id;
+  }
+};
+
+}  // namespace
+
+BuiltBvh4 CollapseToBvh4(const BuiltBvh &b2) {
+  Collapser c{b2, {}};
+  if (b2.num_nodes > 0) c.Emit(0, 0);
+  return c.out;
+}
+
 }  // namespace ttgpu

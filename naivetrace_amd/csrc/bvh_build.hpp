@@ -23,4 +23,14 @@ struct BuiltBvh {
 // SAH with 32 bins per axis, leaves of at most `max_leaf` primitives.
 BuiltBvh BuildBvh(const std::vector<BuildPrim> &prims, int max_leaf = 4, int num_threads = 0);
 
+// 4-wide BVH collapsed from a BVH2 (device layout Bvh4Node, 32 words per node): every node takes
+// its BVH2 node's two children and then opens the inner child of largest surface area until it has
+// four.  Leaves keep the BVH2's primitive ranges (same leaf order).
+struct BuiltBvh4 {
+  std::vector<float> nodes;      // 32 * num_nodes
+  int num_nodes = 0;
+  int max_depth = 0;             // depth of the deepest node (root = 0)
+};
+BuiltBvh4 CollapseToBvh4(const BuiltBvh &b2);
+
 }  // namespace ttgpu

@@ -28,6 +28,19 @@ struct alignas(16) BvhNode {
   int4 ch;      // left, right: >= 0 inner node, < 0 leaf ~first_prim; z/w: prim counts of leaves
 };
 
+// BVH4 node of the surface BVH, 128 bytes: the bounds of all four children in the parent
+// (structure of arrays, one float4 row per bound), so one fetch decides all four.  Empty slots
+// have bounds at +infinity.
+struct alignas(16) Bvh4Node {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  int4 child;   // >= 0: inner node; < 0: leaf ~first primitive
+  int4 count;   // primitives of a leaf child
+};
+
+constexpr int kStackEntries = 96;                   // traversal stack entries per lane
+// A BVH4 node pushes at most three children, so a tree of depth D needs 3 (D + 1) entries.
+constexpr int kMaxBvh4Depth = kStackEntries / 3 - 1;
+
 // Primitive record in leaf order, 48 bytes.
 //   triangle: a = (v0, orig_id bits), b = (v1, kind=0), c = (v2, 0)
 //   sphere  : a = (centre, orig_id bits), b = (radius, 0, 0, kind=1), c = 0
@@ -68,9 +81,11 @@ struct DevEnvLight {
   float spectrum[kNBins];            // sun (power * pdf * solar), dome, directional
   float3 dir;                        // dome: up; directional: dir_out
   int tex_w, tex_h;
-  const float *tex;                  // RGB float
-  const float *cdf_rows;             // h
-  const float *cdf_cols;             // h * w (row-major, normalised per row)
+  // map: texels and alias tables (Walker/Vose) over rows and, per row, over columns.  Alias entry:
+  // {keep threshold, alias index (int bits), probability of the entry, probability of its alias}.
+  const float4 *texel;               // h * w: R, G, B, probability of the column within its row
+  const float4 *alias_rows;          // h
+  const float4 *alias_cols;          // h * w (row-major)
 };
 
 struct DevPointLight {
@@ -102,7 +117,7 @@ struct GuideCell {
 };
 
 struct DeviceScene {
-  const BvhNode *nodes;
+  const Bvh4Node *nodes;             // surface BVH
   const PrimRecord *prims;         // leaf order
   const int *prim_material;        // leaf order
   const float4 *prim_normals;      // leaf order, 3 per triangle (vertex normals), or null
@@ -195,7 +210,11 @@ enum FlagBits : uint32_t {
 
 enum Counter {
   kCountA = 0, kCountB = 1, kCountShadow = 2, kCountRecords = 3, kStatClosest = 4, kStatShadow = 5, kStatPaths = 6,
-  kStatRecordsDropped = 7, kWorkTrace = 8, kCountFree = 9, kNumCounters = 16
+  kStatRecordsDropped = 7, kWorkTrace = 8, kCountFree = 9, kIdentA = 10, kIdentB = 11, kNumCounters = 16
 };
+// Queues A and B carry a flag beside their length (counters[kIdentA], counters[kIdentB]): 1 when the
+// queue holds every slot of the wavefront, so that entry k may be taken as slot k.
+constexpr int kIdentOffset = kIdentA - kCountA;
+static_assert(kIdentB - kCountB == kIdentOffset, "identity flags follow the queue lengths");
 
 }  // namespace ttgpu

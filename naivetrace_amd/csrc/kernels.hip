@@ -14,6 +14,14 @@
 #include "nn_device.hpp"
 #include "kernels.hpp"
 
+// Minimum waves per SIMD asked of the register allocator for the shade and traversal kernels.
+#ifndef TTGPU_SHADE_WAVES
+#define TTGPU_SHADE_WAVES 1
+#endif
+#ifndef TTGPU_TRACE_WAVES
+#define TTGPU_TRACE_WAVES 1
+#endif
+
 namespace ttgpu {
 
 // ---- wave-aggregated queue append ----------------------------------------
@@ -110,6 +118,8 @@ __global__ void k_begin_pass(PathQueues q, uint32_t count) {
     q.counters[kCountB] = 0;
     q.counters[kCountShadow] = 0;
     q.counters[kCountFree] = 0;
+    q.counters[kIdentA] = 1;  // queue A is 0, 1, .., count - 1
+    q.counters[kIdentB] = 0;
     q.progress[0] = count;
     q.progress[1] = count;
   }
@@ -140,6 +150,8 @@ __global__ void k_regen_commit(PathQueues q, uint32_t *count_out, uint64_t limit
     const uint32_t n = regen_count(q, limit, issued);
     const uint32_t total = *count_out + n;
     *count_out = total;
+    // A queue of every slot is read as slot k = entry k (coalesced path-state accesses).
+    count_out[kIdentOffset] = total == (uint32_t)q.capacity ? 1u : 0u;
     q.progress[0] = issued + n;
     q.progress[1] = total;
     q.counters[kCountFree] = 0;
@@ -147,12 +159,13 @@ __global__ void k_regen_commit(PathQueues q, uint32_t *count_out, uint64_t limit
 }
 
 // ---- closest hit --------------------------------------------------------------
-__global__ void __launch_bounds__(kTraceBlock) k_trace(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue,
+__global__ void __launch_bounds__(kTraceBlock, TTGPU_TRACE_WAVES) k_trace(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue,
                                                        const uint32_t *__restrict__ count_ptr, uint32_t *stats) {
   __shared__ int lds_stack[kLdsStack * kTraceBlock];
   const uint32_t count = *count_ptr;
+  const bool ident = count_ptr[kIdentOffset] != 0;
   for (uint32_t k = gtid(); k < count; k += grid_stride()) {
-    const uint32_t i = queue[k];
+    const uint32_t i = ident ? k : queue[k];
     Ray r;
     r.org = xyz(q.org[i]);
     r.dir = xyz(q.dir[i]);
@@ -165,15 +178,19 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(DeviceScene sc, PathQueue
 }
 
 // ---- persistent closest-hit traversal with dynamic ray fetch -------------------------------
-constexpr int kStepsPerFetch = 16;  // traversal steps between refills of idle lanes
+#ifndef TTGPU_STEPS_PER_FETCH
+#define TTGPU_STEPS_PER_FETCH 16
+#endif
+constexpr int kStepsPerFetch = TTGPU_STEPS_PER_FETCH;  // traversal steps between refills of idle lanes
 
-__global__ void __launch_bounds__(kTraceBlock) k_trace_dyn(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue,
+__global__ void __launch_bounds__(kTraceBlock, TTGPU_TRACE_WAVES) k_trace_dyn(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue,
                                                            const uint32_t *__restrict__ count_ptr, uint32_t *work,
                                                            uint32_t *stats) {
   __shared__ int lds_stack[kLdsStack * kTraceBlock];
   int *stack = lds_stack + threadIdx.x;
   int spill[kSpillStack];
   const uint32_t count = *count_ptr;
+  const bool ident = count_ptr[kIdentOffset] != 0;
   Trav t;
   bool active = false;
   uint32_t i = 0;
@@ -181,7 +198,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_dyn(DeviceScene sc, PathQ
     const bool need = !active;
     const uint32_t got = wave_append(work, need);
     if (need && got < count) {
-      i = queue[got];
+      i = ident ? got : queue[got];
       Ray r;
       r.org = xyz(q.org[i]);
       r.dir = xyz(q.dir[i]);
@@ -202,15 +219,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_dyn(DeviceScene sc, PathQ
 }
 
 // ---- environment lights (src/light.hxx:148-288, src/light.cxx:29-189) -------------
-__device__ __forceinline__ uint32_t upper_bound(const float *cdf, uint32_t n, float r) {
-  uint32_t lo = 0, hi = n;  // first index with r < cdf[i]
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (r < cdf[mid]) hi = mid; else lo = mid + 1;
-  }
-  return lo >= n ? n - 1 : lo;
+// Alias-table draw (Walker 1977, Vose 1991): bucket from u, kept when coin < threshold, else its
+// alias; one load gives the index and its probability.
+__device__ __forceinline__ uint32_t alias_pick(const float4 *t, uint32_t n, float u, float coin, float &p) {
+  const uint32_t b = min((uint32_t)(u * (float)n), n - 1u);
+  const float4 e = t[b];
+  const bool keep = coin < e.x;
+  p = keep ? e.z : e.w;
+  return keep ? b : (uint32_t)as_int(e.y);
 }
-__device__ __forceinline__ float cmf_prob(const float *cdf, uint32_t i) { return cdf[i] - (i > 0 ? cdf[i - 1] : 0.f); }
 
 __device__ __forceinline__ void env_map_pixel(const DevEnvLight &e, float3 dir_out, int &x, int &y) {
   float3 l = mk(dot(e.frame_x, dir_out), dot(e.frame_y, dir_out), dot(e.frame_z, dir_out));
@@ -259,11 +276,12 @@ __device__ float4 env_eval_one(const DevEnvLight &e, float3 dir_out, uint32_t bi
     case TTGPU_ENV_MAP: {
       int x, y;
       env_map_pixel(e, dir_out, x, y);
-      const float *px = e.tex + 3 * ((size_t)y * e.tex_w + x);
+      const float4 tx = e.texel[(size_t)y * e.tex_w + x];
+      const float prow = e.alias_rows[y].z;
       float p0, z0, p1, z1;
       env_pixel_bounds(e, x, y, p0, z0, p1, z1);
-      *pdf = cmf_prob(e.cdf_rows, y) * cmf_prob(e.cdf_cols + (size_t)y * e.tex_w, x) / ((z1 - z0) * (p1 - p0));
-      return rgb_to_spectral(mk(px[0], px[1], px[2]), bins);
+      *pdf = prow * tx.w / ((z1 - z0) * (p1 - p0));
+      return rgb_to_spectral(mk(tx.x, tx.y, tx.z), bins);
     }
   }
   *pdf = 0.f;
@@ -310,18 +328,20 @@ __device__ void env_sample(const DeviceScene &sc, uint32_t bins, Pcg &rng, float
       break;
     }
     default: {
-      float r0 = rng.uniform(), r1 = rng.uniform();
-      uint32_t y = upper_bound(e.cdf_rows, e.tex_h, r1);
-      const float *row = e.cdf_cols + (size_t)y * e.tex_w;
-      uint32_t x = upper_bound(row, e.tex_w, r0);
+      // Pixel with probability p_row * p_col (the sin-theta weighted CMFs of src/light.cxx:88-116,
+      // drawn from alias tables), then a uniform point of its solid angle.
+      const float r0 = rng.uniform(), r1 = rng.uniform(), c0 = rng.uniform(), c1 = rng.uniform();
+      float prow, pcol;
+      const uint32_t y = alias_pick(e.alias_rows, e.tex_h, r1, c1, prow);
+      const uint32_t x = alias_pick(e.alias_cols + (size_t)y * e.tex_w, e.tex_w, r0, c0, pcol);
+      const float4 tx = e.texel[(size_t)y * e.tex_w + x];
       float p0, z0, p1, z1;
       env_pixel_bounds(e, x, y, p0, z0, p1, z1);
       float s0 = rng.uniform(), s1 = rng.uniform();
       float3 l = to_sphere_section(s0, s1, p0, z0, p1, z1);
       dir_out = e.frame_x * l.x + e.frame_y * l.y + e.frame_z * l.z;
-      const float *px = e.tex + 3 * ((size_t)y * e.tex_w + x);
-      val = rgb_to_spectral(mk(px[0], px[1], px[2]), bins);
-      pdf = cmf_prob(e.cdf_rows, y) * cmf_prob(row, x) / ((z1 - z0) * (p1 - p0));
+      val = rgb_to_spectral(mk(tx.x, tx.y, tx.z), bins);
+      pdf = prow * pcol / ((z1 - z0) * (p1 - p0));
       break;
     }
   }
@@ -473,10 +493,11 @@ __device__ __forceinline__ LightSample sample_light(const DeviceScene &sc, float
 }
 
 // ---- shade: emission, NEE, scattering (pt2.cxx:410-443, 479-556, 605-684) --------
-__global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
+__global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
                                                const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                uint32_t *count_out, double *fb, double *fb_sq) {
   const uint32_t count = *count_in;
+  const bool ident = count_in[kIdentOffset] != 0;
   const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
   const bool use_emission = sc.sample_mode != TTGPU_SAMPLE_LIGHTS;
   const bool mis = use_nee && use_emission;
@@ -487,7 +508,7 @@ __global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, con
     bool push = false, push_shadow = false;
     uint32_t i = 0;
     if (active) {
-      i = queue_in[k];
+      i = ident ? k : queue_in[k];
       const float4 hitv = q.hit[i];
       const float3 dir = xyz(q.dir[i]);
       const float4 org4 = q.org[i];
@@ -586,7 +607,7 @@ __global__ void __launch_bounds__(256) k_shade(DeviceScene sc, PathQueues q, con
 }
 
 // ---- shadow rays: any hit + accumulate + finalize -----------------------------------
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(DeviceScene sc, PathQueues q, double *fb, double *fb_sq, uint32_t *stats) {
+__global__ void __launch_bounds__(kTraceBlock, TTGPU_TRACE_WAVES) k_shadow(DeviceScene sc, PathQueues q, double *fb, double *fb_sq, uint32_t *stats) {
   __shared__ int lds_stack[kLdsStack * kTraceBlock];
   const uint32_t count = q.counters[kCountShadow];
   for (uint32_t k = gtid(); k < count; k += grid_stride()) {
@@ -614,6 +635,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shade_media(DeviceScene sc, Pat
                                                              const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                              uint32_t *count_out, double *fb, double *fb_sq) {
   const uint32_t count = *count_in;
+  const bool ident = count_in[kIdentOffset] != 0;
   const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
   const bool use_emission = sc.sample_mode != TTGPU_SAMPLE_LIGHTS;
   const bool mis = use_nee && use_emission;
@@ -624,7 +646,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shade_media(DeviceScene sc, Pat
     bool push = false, push_shadow = false;
     uint32_t i = 0;
     if (active) {
-      i = queue_in[k];
+      i = ident ? k : queue_in[k];
       const float4 hitv = q.hit[i];
       const float3 dir = xyz(q.dir[i]);
       const float4 org4 = q.org[i];
@@ -856,6 +878,7 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
                                                       const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                       uint32_t *count_out, double *fb, double *fb_sq) {
   const uint32_t count = *count_in;
+  const bool ident = count_in[kIdentOffset] != 0;
   const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
   const bool use_emission = sc.sample_mode != TTGPU_SAMPLE_LIGHTS;
   const bool mis = use_nee && use_emission;
@@ -868,7 +891,7 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
     bool push = false, push_shadow = false;
     uint32_t i = 0;
     if (active) {
-      i = queue_in[k];
+      i = ident ? k : queue_in[k];
       const float4 hitv = q.hit[i];
       const float3 dir = xyz(q.dir[i]);
       const float4 org4 = q.org[i];
@@ -1038,13 +1061,14 @@ __global__ void __launch_bounds__(nn::kTile) k_guide_infer(DeviceScene sc, PathQ
   __shared__ __attribute__((aligned(16))) __bf16 x[nn::kTile * nn::kPitch], h0[nn::kTile * nn::kPitch], h1[nn::kTile * nn::kPitch];
   __shared__ __attribute__((aligned(16))) float o[nn::kTile * nn::kPitchOut32];
   const uint32_t count = *count_ptr;
+  const bool ident = count_ptr[kIdentOffset] != 0;
   const int l = threadIdx.x;
   for (uint32_t tile = blockIdx.x; tile * nn::kTile < count; tile += gridDim.x) {
     const uint32_t k = tile * nn::kTile + l;
     uint32_t i = 0;
     bool hit = false;
     if (k < count) {
-      i = queue[k];
+      i = ident ? k : queue[k];
       const float4 hv = q.hit[i];
       const int prim = as_int(hv.w);
       if (prim >= 0) {

@@ -165,7 +165,11 @@ TT_HD float4 metal_core(const DevShader &s, float alpha, float3 ng_l, float3 wi,
   // GGX
   float3 h = wh.z < 0.f ? -wh : wh;
   float lw = ggx_lambda(alpha, wi);
-  float pv = wi.z > 0.f ? (1.f / (1.f + lw)) * clampdot(wi, h) * ggx_d(alpha, h) / wi.z : 0.f;
+  // Density of the visible normal for the direction the sampler used: wi, or wi lifted just above
+  // the shading hemisphere when it lies below (then the value is zero and the path ends).
+  const float3 wv = wi.z > 0.f ? wi : normalize(mk(wi.x, wi.y, 1e-7f));
+  const float lv = wi.z > 0.f ? lw : ggx_lambda(alpha, wv);
+  float pv = (1.f / (1.f + lv)) * clampdot(wv, h) * ggx_d(alpha, h) / wv.z;
   *pdf = pv * 0.25f / (fabsf(dot(wi, h)) + kDblEps);
   if (dot(ng_l, wo) <= 0.f || wi.z <= 0.f || wo.z <= 0.f) return f4(0.f);
   float D = ggx_d(alpha, h);

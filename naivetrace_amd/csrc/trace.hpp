@@ -9,9 +9,12 @@
 
 namespace ttgpu {
 
-constexpr int kTraceBlock = 128;    // threads per block of the trace kernels
-constexpr int kLdsStack = 24;       // LDS stack entries per lane
-constexpr int kSpillStack = 40;     // private overflow entries per lane
+#ifndef TTGPU_LDS_STACK
+#define TTGPU_LDS_STACK 24
+#endif
+constexpr int kTraceBlock = 128;                    // threads per block of the trace kernels
+constexpr int kLdsStack = TTGPU_LDS_STACK;          // traversal stack entries per lane in LDS
+constexpr int kSpillStack = kStackEntries - kLdsStack;  // and in a private overflow area
 
 struct Ray {
   float3 org, dir;
@@ -146,72 +149,9 @@ TT_HD void box2(const BvhNode &n, const RayPre &p, float tmax, float &tl, float 
   tr = rmin <= rmax ? rmin : as_float(0x7f800000);
 }
 
-// Generic traversal.  `stack` points at this lane's LDS column (stride kTraceBlock).
-template <bool kAnyHit>
-TT_HD bool traverse(const BvhNode *__restrict__ nodes, const PrimRecord *__restrict__ prims, const Ray &r,
-                                         int *stack, HitRec &hit) {
-  RayPre p = precompute(r);
-  float tmax = r.tmax;
-  hit.prim = -1;
-  int spill[kSpillStack];
-  int sp = 0;
-  int node = 0;
-  for (;;) {
-    const BvhNode n = nodes[node];
-    float tl, tr;
-    box2(n, p, tmax, tl, tr);
-    int next = -1;
-    // Leaves are intersected as soon as their parent is visited.
-    int children[2] = {n.ch.x, n.ch.y};
-    int counts[2] = {n.ch.z, n.ch.w};
-    float tc[2] = {tl, tr};
-    int inner[2];
-    float tin[2];
-    int ninner = 0;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      if (!(tc[c] <= tmax)) continue;
-      if (children[c] < 0) {
-        int first = ~children[c];
-        for (int k = 0; k < counts[c]; ++k) {
-          float t, u, v;
-          if (intersect_prim(prims[first + k], p, r, tmax, t, u, v)) {
-            tmax = t;
-            hit.t = t; hit.u = u; hit.v = v; hit.prim = first + k;
-            if (kAnyHit) return true;
-          }
-        }
-      } else {
-        inner[ninner] = children[c];
-        tin[ninner] = tc[c];
-        ++ninner;
-      }
-    }
-    if (ninner == 2) {
-      bool swap = tin[1] < tin[0];
-      next = swap ? inner[1] : inner[0];
-      int far = swap ? inner[0] : inner[1];
-      // The host rejects trees deeper than kLdsStack + kSpillStack, so this bound never drops a node.
-      if (sp < kLdsStack) stack[sp * kTraceBlock] = far;
-      else if (sp < kLdsStack + kSpillStack) spill[sp - kLdsStack] = far;
-      ++sp;
-      if (sp > kLdsStack + kSpillStack) sp = kLdsStack + kSpillStack;
-    } else if (ninner == 1) {
-      next = inner[0];
-    }
-    if (next < 0) {
-      if (sp == 0) break;
-      --sp;
-      next = sp < kLdsStack ? stack[sp * kTraceBlock] : spill[sp - kLdsStack];
-    }
-    node = next;
-  }
-  return hit.prim >= 0;
-}
-
-// Resumable traversal for persistent kernels with dynamic ray fetch: a lane advances its ray one
-// node at a time and, when the ray is finished, takes the next one while the rest of the wave keeps
-// traversing (persistent while-while with dynamic fetch, Aila & Laine 2009).
+// ---- BVH4 traversal ----------------------------------------------------------------------
+// A ray's traversal state; one step processes one node, so the loop below and the persistent
+// kernels with dynamic ray fetch (Aila & Laine 2009) share the same code.
 struct Trav {
   RayPre p;
   float3 org, dir;
@@ -230,28 +170,49 @@ TT_HD void trav_init(Trav &t, const Ray &r) {
   t.sp = 0;
 }
 
-// Processes the current node (testing leaf primitives right away); returns false when the
-// traversal is finished (stack empty, or an occluder found for any-hit queries).
+// Slab test of the four children; entry distances, +inf when missed.
+TT_HD void box4(const Bvh4Node &n, const RayPre &p, float tmax, float t[4]) {
+  const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w}, hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
+  const float ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w}, hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w};
+  const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w}, hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
+  // Conservative far bound (1 + 2 gamma(3)) so grazing hits are not lost (Ize 2013).
+  const float pad = 1.0000004f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float x0 = lx[c] * p.inv.x - p.oinv.x, x1 = hx[c] * p.inv.x - p.oinv.x;
+    const float y0 = ly[c] * p.inv.y - p.oinv.y, y1 = hy[c] * p.inv.y - p.oinv.y;
+    const float z0 = lz[c] * p.inv.z - p.oinv.z, z1 = hz[c] * p.inv.z - p.oinv.z;
+    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.f));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * pad;
+    t[c] = tn <= tf ? tn : as_float(0x7f800000);
+  }
+}
+
+// Processes the current node: leaf children are intersected at once, inner children are visited
+// nearest first with the others pushed far to near.  Returns false when the traversal is finished
+// (stack empty, or an occluder found for any-hit queries).  `stack` is this lane's LDS column
+// (stride kTraceBlock), `spill` its private overflow; the host bounds the tree depth so that the
+// kStackEntries entries always suffice.
 template <bool kAnyHit>
-TT_HD bool trav_step(Trav &t, const BvhNode *__restrict__ nodes, const PrimRecord *__restrict__ prims, int *stack, int *spill) {
-  const BvhNode n = nodes[t.node];
-  float tl, tr;
-  box2(n, t.p, t.tmax, tl, tr);
-  const int children[2] = {n.ch.x, n.ch.y};
-  const int counts[2] = {n.ch.z, n.ch.w};
-  const float tc[2] = {tl, tr};
-  int inner[2];
-  float tin[2];
-  int ninner = 0;
+TT_HD bool trav_step(Trav &t, const Bvh4Node *__restrict__ nodes, const PrimRecord *__restrict__ prims, int *stack, int *spill) {
+  const Bvh4Node n = nodes[t.node];
+  float tc[4];
+  box4(n, t.p, t.tmax, tc);
+  const int child[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+  const int count[4] = {n.count.x, n.count.y, n.count.z, n.count.w};
   Ray r;
   r.org = t.org;
   r.dir = t.dir;
+  float key[4];
+  int id[4];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < 4; ++c) {
+    key[c] = as_float(0x7f800000);
+    id[c] = -1;
     if (!(tc[c] <= t.tmax)) continue;
-    if (children[c] < 0) {
-      const int first = ~children[c];
-      for (int k = 0; k < counts[c]; ++k) {
+    if (child[c] < 0) {
+      const int first = ~child[c];
+      for (int k = 0; k < count[c]; ++k) {
         float th, u, v;
         if (intersect_prim(prims[first + k], t.p, r, t.tmax, th, u, v)) {
           t.tmax = th;
@@ -260,23 +221,30 @@ TT_HD bool trav_step(Trav &t, const BvhNode *__restrict__ nodes, const PrimRecor
         }
       }
     } else {
-      inner[ninner] = children[c];
-      tin[ninner] = tc[c];
-      ++ninner;
+      key[c] = tc[c];
+      id[c] = child[c];
     }
   }
+  // Sort the (at most four) inner children by entry distance: five compare-exchanges.
+#define TT_CSWAP(a, b)                                                                  \
+  if (key[b] < key[a]) {                                                               \
+    const float kt = key[a]; key[a] = key[b]; key[b] = kt;                             \
+    const int it = id[a]; id[a] = id[b]; id[b] = it;                                   \
+  }
+  TT_CSWAP(0, 1) TT_CSWAP(2, 3) TT_CSWAP(0, 2) TT_CSWAP(1, 3) TT_CSWAP(1, 2)
+#undef TT_CSWAP
   int next = -1;
-  if (ninner == 2) {
-    const bool swap = tin[1] < tin[0];
-    next = swap ? inner[1] : inner[0];
-    const int far = swap ? inner[0] : inner[1];
-    // The host rejects trees deeper than kLdsStack + kSpillStack, so this bound never drops a node.
-    if (t.sp < kLdsStack) stack[t.sp * kTraceBlock] = far;
-    else if (t.sp < kLdsStack + kSpillStack) spill[t.sp - kLdsStack] = far;
-    ++t.sp;
-    if (t.sp > kLdsStack + kSpillStack) t.sp = kLdsStack + kSpillStack;
-  } else if (ninner == 1) {
-    next = inner[0];
+#pragma unroll
+  for (int c = 3; c >= 0; --c) {
+    // Children entered beyond a hit found in this node's leaves are dropped.
+    if (id[c] < 0 || !(key[c] <= t.tmax)) continue;
+    if (next >= 0) {
+      if (t.sp < kLdsStack) stack[t.sp * kTraceBlock] = next;
+      else if (t.sp < kStackEntries) spill[t.sp - kLdsStack] = next;
+      ++t.sp;
+      if (t.sp > kStackEntries) t.sp = kStackEntries;
+    }
+    next = id[c];
   }
   if (next < 0) {
     if (t.sp == 0) return false;
@@ -285,6 +253,18 @@ TT_HD bool trav_step(Trav &t, const BvhNode *__restrict__ nodes, const PrimRecor
   }
   t.node = next;
   return true;
+}
+
+// Whole traversal of one ray (closest hit, or any hit with kAnyHit); true if something was hit.
+template <bool kAnyHit>
+TT_HD bool traverse(const Bvh4Node *__restrict__ nodes, const PrimRecord *__restrict__ prims, const Ray &r, int *stack,
+                    HitRec &hit) {
+  int spill[kSpillStack];
+  Trav t;
+  trav_init(t, r);
+  while (trav_step<kAnyHit>(t, nodes, prims, stack, spill)) {}
+  hit = t.hit;
+  return t.hit.prim >= 0;
 }
 
 }  // namespace ttgpu

@@ -271,7 +271,10 @@ void BuildGeometry(ttgpu_handle *h, const ttgpu_scene &s, const std::vector<int6
     }
   }
   auto t0 = std::chrono::steady_clock::now();
-  BuiltBvh bvh = BuildBvh(bp, 4, h->params.num_threads);
+#ifndef TTGPU_MAX_LEAF
+#define TTGPU_MAX_LEAF 4
+#endif
+  BuiltBvh bvh = BuildBvh(bp, TTGPU_MAX_LEAF, h->params.num_threads);
   auto t1 = std::chrono::steady_clock::now();
   h->stats.bvh_build_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (bvh.max_depth >= 63) throw std::runtime_error("BVH deeper than the traversal stack (63 levels)");
@@ -322,17 +325,20 @@ void BuildGeometry(ttgpu_handle *h, const ttgpu_scene &s, const std::vector<int6
     std::memcpy(bvh.nodes.data() + 12, ch, sizeof(ch));
     bvh.num_nodes = 1;
   }
-  const BvhNode *nodes = h->Upload((const BvhNode *)bvh.nodes.data(), bvh.num_nodes);
   if (surfaces) {
-    h->dsc.nodes = nodes;
+    // Surfaces: the BVH2 collapsed to a 4-wide BVH for traversal.
+    const BuiltBvh4 b4 = CollapseToBvh4(bvh);
+    if (b4.max_depth > kMaxBvh4Depth)
+      throw std::runtime_error("BVH4 deeper than the traversal stack (" + std::to_string(kMaxBvh4Depth) + " levels)");
+    h->dsc.nodes = h->Upload((const Bvh4Node *)b4.nodes.data(), b4.num_nodes);
+    h->stats.bvh_nodes = b4.num_nodes;
+    h->stats.scene_bytes += b4.nodes.size() * 4;
     h->dsc.prims = h->Upload(recs.data(), recs.size());
     h->dsc.prim_material = h->Upload(mats.data(), mats.size());
     h->dsc.prim_area = h->Upload(areas.data(), areas.size());
     h->dsc.prim_normals = smooth ? h->Upload(normals.data(), normals.size()) : nullptr;
     h->dsc.num_prims = (int)n;
-    h->stats.bvh_nodes = bvh.num_nodes;
-    h->stats.scene_bytes += bvh.nodes.size() * 4 + recs.size() * sizeof(PrimRecord) + mats.size() * 4 + areas.size() * 4 +
-                            normals.size() * sizeof(float4);
+    h->stats.scene_bytes += recs.size() * sizeof(PrimRecord) + mats.size() * 4 + areas.size() * 4 + normals.size() * sizeof(float4);
     // Area lights: emissive surface primitives.
     std::vector<int> lights;
     for (size_t k = 0; k < n; ++k)
@@ -340,7 +346,7 @@ void BuildGeometry(ttgpu_handle *h, const ttgpu_scene &s, const std::vector<int6
     h->dsc.area_lights = h->Upload(lights.data(), lights.size());
     h->dsc.num_area_lights = (int)lights.size();
   } else {
-    h->dsc.vol_nodes = nodes;
+    h->dsc.vol_nodes = h->Upload((const BvhNode *)bvh.nodes.data(), bvh.num_nodes);
     h->dsc.vol_prims = h->Upload(recs.data(), recs.size());
     h->dsc.vol_material = h->Upload(mats.data(), mats.size());
     h->dsc.num_vol_prims = (int)n;
@@ -455,6 +461,40 @@ uint64_t CameraTracker(const ttgpu_scene &s, int empty) {
   return PackTracker(t);
 }
 
+// Alias table of a discrete distribution p[0..n) (sums to 1), Vose's construction in FP64: entry
+// b keeps bucket b with probability thr_b and otherwise yields alias_b, so that every index i is
+// drawn with probability p_i.  Entries store {thr, alias (int bits), p_b, p_alias}.
+void BuildAlias(const double *p, int n, float4 *out) {
+  std::vector<double> scaled(n);
+  std::vector<int> small, large, alias(n);
+  std::vector<double> thr(n, 1.0);
+  for (int i = 0; i < n; ++i) {
+    alias[i] = i;
+    scaled[i] = p[i] * n;
+    (scaled[i] < 1.0 ? small : large).push_back(i);
+  }
+  while (!small.empty() && !large.empty()) {
+    const int s = small.back(), l = large.back();
+    small.pop_back();
+    thr[s] = scaled[s];
+    alias[s] = l;
+    scaled[l] = (scaled[l] + scaled[s]) - 1.0;
+    if (scaled[l] < 1.0) {
+      large.pop_back();
+      small.push_back(l);
+    }
+  }
+  // Leftovers carry rounding only: they keep themselves.
+  for (int i : small) thr[i] = 1.0;
+  for (int i : large) thr[i] = 1.0;
+  for (int i = 0; i < n; ++i) {
+    int32_t a = alias[i];
+    float af;
+    std::memcpy(&af, &a, 4);
+    out[i] = make_float4((float)thr[i], af, (float)p[i], (float)p[alias[i]]);
+  }
+}
+
 void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
   std::vector<DevShader> shaders(std::max(1, s.num_shaders));
   for (int i = 0; i < s.num_shaders; ++i) {
@@ -514,26 +554,35 @@ void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
       d.frame_x = F3(f.x); d.frame_y = F3(f.y); d.frame_z = F3(f.z);
       const int w = e.tex_width, hh = e.tex_height;
       d.tex_w = w; d.tex_h = hh;
-      // Row/column CMFs weighted by sin(theta) (src/light.cxx:88-116).
-      std::vector<double> rows(hh), cols((size_t)w * hh);
+      // Row and per-row column probabilities weighted by sin(theta) (the CMFs of
+      // src/light.cxx:88-116), sampled through alias tables.
+      std::vector<double> prow(hh), pcol((size_t)w * hh);
+      double total = 0.;
       for (int y = 0; y < hh; ++y) {
         double row = 0.;
-        double theta = kPiD * (((double)(hh - 1 - y) + 0.5) / hh);
+        const double theta = kPiD * (((double)(hh - 1 - y) + 0.5) / hh);
         for (int x = 0; x < w; ++x) {
           const float *px = e.texture_rgb + 3 * ((size_t)y * w + x);
-          row += std::sin(theta) * ((double)px[0] + px[1] + px[2]) / 3.;
-          cols[(size_t)y * w + x] = row;
+          const double v = std::sin(theta) * ((double)px[0] + px[1] + px[2]) / 3.;
+          pcol[(size_t)y * w + x] = v;
+          row += v;
         }
-        for (int x = 0; x < w; ++x) cols[(size_t)y * w + x] = row > 0 ? cols[(size_t)y * w + x] / row : (double)(x + 1) / w;
-        rows[y] = row;
+        for (int x = 0; x < w; ++x) pcol[(size_t)y * w + x] = row > 0 ? pcol[(size_t)y * w + x] / row : 1. / w;
+        prow[y] = row;
+        total += row;
       }
-      for (int y = 1; y < hh; ++y) rows[y] += rows[y - 1];
-      for (int y = 0; y < hh; ++y) rows[y] /= rows[hh - 1];
-      std::vector<float> rf(rows.begin(), rows.end()), cf(cols.begin(), cols.end());
-      d.cdf_rows = h->Upload(rf.data(), rf.size());
-      d.cdf_cols = h->Upload(cf.data(), cf.size());
-      d.tex = h->Upload(e.texture_rgb, 3 * (size_t)w * hh);
-      h->stats.scene_bytes += (rf.size() + cf.size() + 3 * (size_t)w * hh) * 4;
+      for (int y = 0; y < hh; ++y) prow[y] = total > 0. ? prow[y] / total : 1. / hh;
+      std::vector<float4> texel((size_t)w * hh), arows(hh), acols((size_t)w * hh);
+      for (size_t k = 0; k < texel.size(); ++k) {
+        const float *px = e.texture_rgb + 3 * k;
+        texel[k] = make_float4(px[0], px[1], px[2], (float)pcol[k]);
+      }
+      BuildAlias(prow.data(), hh, arows.data());
+      for (int y = 0; y < hh; ++y) BuildAlias(pcol.data() + (size_t)y * w, w, acols.data() + (size_t)y * w);
+      d.texel = h->Upload(texel.data(), texel.size());
+      d.alias_rows = h->Upload(arows.data(), arows.size());
+      d.alias_cols = h->Upload(acols.data(), acols.size());
+      h->stats.scene_bytes += (texel.size() + arows.size() + acols.size()) * sizeof(float4);
     }
   }
   h->dsc.envs = h->Upload(envs.data(), envs.size());

@@ -421,7 +421,10 @@ template <int NW> struct Shading {
     V3 wh = HalfVector(wi, wo);
     if (wh.z < 0) wh = -wh;
     if (pdf) {
-      double pv = g.PdfVisible(wi, wh);
+      // Visible-normal density for the direction the sampler used: wi, or wi lifted just above the
+      // shading hemisphere when it lies below (the value is then zero and the path ends).
+      V3 wv = wi.z > 0. ? wi : Normalized(V3{wi.x, wi.y, 1e-7});
+      double pv = g.PdfVisible(wv, wh);
       *pdf = pv * 0.25 / (std::abs(Dot(wi, wh)) + Epsilon);
     }
     if (Dot(ng_local, wo) <= 0. || wi.z <= 0. || wo.z <= 0.) return SpecConst<NW>(0.);

@@ -30,6 +30,10 @@ FIXTURES = {
     "medium_sphere32": (lambda w, h: configs.medium_sphere(w, h), 32, 32, 8192, 16, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
     "fog_box32": (lambda w, h: configs.fog_box(w, h), 32, 32, 8192, 10, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
     "atmosphere32": (lambda w, h: configs.atmosphere(w, h), 32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+    # the benchmark scene at small size: HDR sky map with a sun disc (env-map NEE and MIS), GGX,
+    # glass, mirror and Lambert materials on a heightfield and icospheres
+    "terrain32": (lambda w, h: configs.terrain_instances(w, h, cells=32, n_spheres=24, freq=3, sky=(256, 128)), 32, 32, 8192, 8,
+                  abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
 }
 
 

@@ -42,10 +42,14 @@ def test_bvh_matches_gpu_bruteforce(gpu_lib):
         tb, pb, _, _ = r.trace_closest(org, d, tmax, bruteforce=True)
     print("bvh hit rate", np.mean(prim >= 0), "brute-force hit rate", np.mean(pb >= 0), flush=True)
     assert np.array_equal(prim >= 0, pb >= 0)
-    # Same triangle test on both sides; where a ray crosses a shared edge the two orders may keep
-    # either neighbour, whose distances differ in the last bit.
+    # Same triangle test on both sides, so the closest distance agrees (to the last bit where a ray
+    # crosses a shared edge).  Where two primitives lie at the same distance -- the boxes' bottom
+    # faces are coplanar with the floor -- the BVH's nearest-first order and the brute force's index
+    # order may keep different ones.
     np.testing.assert_allclose(t, tb, rtol=3e-7, atol=0)
-    assert np.mean(prim == pb) > 0.999
+    differ = prim != pb
+    np.testing.assert_allclose(t[differ], tb[differ], rtol=3e-7, atol=0)
+    assert np.mean(differ) < 0.005
 
 
 def test_closest_hit_matches_oracle(gpu_lib, oracle):
@@ -140,6 +144,12 @@ def test_cornell_parity_consistent_mis(gpu_lib):
 
 def test_cornell_parity_bsdf_only(gpu_lib):
     render_parity("cornell32_bsdf", mode=abi.SAMPLE_BSDF)
+
+
+def test_terrain_env_map_parity(gpu_lib):
+    # The benchmark scene at 32x32: HDR sky map with a sun disc (alias-table NEE on the GPU, CDF
+    # inversion in the oracle), GGX / glass / mirror / Lambert on a heightfield and icospheres.
+    render_parity("terrain32", builder=lambda w, h: configs.terrain_instances(w, h, cells=32, n_spheres=24, freq=3, sky=(256, 128)))
 
 
 def test_run_schedule_and_image(gpu_lib):

@@ -392,6 +392,22 @@ int ttgpu_nn_gradient(const ttgpu_nn_config *cfg, const float *params, int64_t n
 int ttgpu_nn_train(const ttgpu_nn_config *cfg, float *params, int64_t n, const float *pos, const float *nrm, const float *wo,
                    const float *wi, const float *weight, int32_t passes, float *loss_per_pass);
 
+/* ---- scene files ----------------------------------------------------------
+ * Scene::ParseNFF / ParseNFFString (src/parsenff.cxx:992-1028): ToyTrace's NFF dialect
+ * (scopes, transforms, yaml{ }yaml blocks, shaders, media, lights, `include`) with the OBJ
+ * and COLLADA meshes named by `m` (assimp::Read, src/parse_common.cxx:225-249).  The file
+ * object owns every array of the scene description it hands out.  Errors (status
+ * TTGPU_ERR_INVALID_ARGUMENT for malformed input or unsupported directives, TTGPU_ERR_RUNTIME
+ * for unreadable files, as the reference's exceptions) are described by ttgpu_create_error(). */
+typedef struct ttgpu_scene_file ttgpu_scene_file;
+int ttgpu_scene_file_load(const char *path, ttgpu_scene_file **out);
+/* NFF text; relative file names resolve against the directory of `path_hint` (may be NULL). */
+int ttgpu_scene_file_parse(const char *nff_text, const char *path_hint, ttgpu_scene_file **out);
+const ttgpu_scene *ttgpu_scene_file_scene(const ttgpu_scene_file *f);
+/* The resolution of the file's camera (0 x 0 without a camera). */
+int ttgpu_scene_file_resolution(const ttgpu_scene_file *f, int32_t *width, int32_t *height);
+void ttgpu_scene_file_free(ttgpu_scene_file *f);
+
 /* Device / build information string ("gfx950, 256 CUs, ..."). */
 const char *ttgpu_device_info(void);
 int ttgpu_abi_version(void);

@@ -27,6 +27,7 @@ ALGO_PT2, ALGO_PTG = range(2)
 SAMPLE_BOTH, SAMPLE_BSDF, SAMPLE_LIGHTS = range(3)
 MIS_REFERENCE, MIS_CONSISTENT = range(2)
 GUIDE_NONE, GUIDE_VMF_TREE, GUIDE_LEARNED = range(3)
+OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_DEVICE, ERR_INTERRUPTED, ERR_OUT_OF_MEMORY = range(6)
 
 Spectrum = C.c_float * NBINS
 
@@ -135,6 +136,8 @@ EXPORTS = [
     "ttgpu_abi_version", "ttgpu_guide_info", "ttgpu_guide_init_vmf", "ttgpu_guide_fit_vmf",
     "ttgpu_guide_build_kdtree", "ttgpu_guide_sample_vmf", "ttgpu_guide_pdf_vmf", "ttgpu_nn_param_count",
     "ttgpu_nn_init_params", "ttgpu_nn_forward", "ttgpu_nn_gradient", "ttgpu_nn_train",
+    "ttgpu_scene_file_load", "ttgpu_scene_file_parse", "ttgpu_scene_file_scene", "ttgpu_scene_file_resolution",
+    "ttgpu_scene_file_free",
 ]
 
 _lib = None
@@ -181,6 +184,11 @@ def _declare(lib):
         "ttgpu_nn_forward": (C.c_int, [P(NnConfig), f32p, C.c_int64, f32p, f32p, f32p, f32p]),
         "ttgpu_nn_gradient": (C.c_int, [P(NnConfig), f32p, C.c_int64, f32p, f32p, f32p, f32p, f32p, f32p, f32p]),
         "ttgpu_nn_train": (C.c_int, [P(NnConfig), f32p, C.c_int64, f32p, f32p, f32p, f32p, f32p, C.c_int32, f32p]),
+        "ttgpu_scene_file_load": (C.c_int, [C.c_char_p, P(C.c_void_p)]),
+        "ttgpu_scene_file_parse": (C.c_int, [C.c_char_p, C.c_char_p, P(C.c_void_p)]),
+        "ttgpu_scene_file_scene": (P(SceneDesc), [C.c_void_p]),
+        "ttgpu_scene_file_resolution": (C.c_int, [C.c_void_p, i32p, i32p]),
+        "ttgpu_scene_file_free": (None, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

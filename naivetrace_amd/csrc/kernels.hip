@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(kTraceBlock, TTGPU_TRACE_WAVES) k_trace(Device
 
 // ---- persistent closest-hit traversal with dynamic ray fetch -------------------------------
 #ifndef TTGPU_STEPS_PER_FETCH
-#define TTGPU_STEPS_PER_FETCH 16
+#define TTGPU_STEPS_PER_FETCH 32
 #endif
 constexpr int kStepsPerFetch = TTGPU_STEPS_PER_FETCH;  // traversal steps between refills of idle lanes
 

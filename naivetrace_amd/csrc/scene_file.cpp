@@ -1,0 +1,1162 @@
+// NFF scene files with the reference's extensions (src/parsenff.cxx) and the OBJ / COLLADA meshes
+// they name (assimp in the reference, src/parse_common.cxx:106-249), read into the flat
+// ttgpu_scene description.
+//
+// Semantics kept from the reference:
+//  * scopes `{ }` inherit the active shader, medium, area emitter, named materials and transform;
+//    changes inside a scope do not leak out (src/parse_common.hxx:115-136);
+//  * the default symbols: shaders "default" (diffuse, RGB 0.8), "black", "invisible", "none";
+//    medium "default" (none); area emitter "none" (src/parse_common.cxx:15-24, src/scene.cxx:10-31);
+//  * a primitive gets the material made of the active shader, medium and area emitter;
+//  * `transform tx ty tz [h p b [sx sy sz]]`: translation, then heading (about y), pitch (about x)
+//    and bank (about z) in radians, then scaling; YAML `transform:` also takes rotaxis, degrees;
+//  * polygons are fans; a polygon with any vertex lacking a normal, and meshes without normals,
+//    get flat normals (src/primitive.cxx:18-45); degenerate mesh triangles are dropped;
+//  * a medium's priority is the number of media known in the scope when it is defined;
+//  * sun / directional / dome vectors are not transformed; lsun's default opening angle 0.26 deg;
+//  * COLLADA: node transforms, the asset's unit (meters per unit) and up axis are applied to the
+//    vertices, materials are named by their <material name>, like assimp's importer.
+// Not supported (the GPU path has no textures, and no tabulated atmosphere): diffusetexture,
+// exponenttexture, alpha_texture, tabulatedatmosphere; lenv reads PFM images only.  These raise
+// std::invalid_argument naming the directive.
+#include "scene_file.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <sstream>
+#include <stdexcept>
+#include <tuple>
+
+#include "spectral_tables.inc"
+
+namespace ttgpu {
+namespace scenefile {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---- small linear algebra ----------------------------------------------------------------
+struct V3 {
+  double x = 0, y = 0, z = 0;
+};
+V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 Cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double Len(V3 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+V3 Normalized(V3 a) {
+  const double l = Len(a);
+  return l > 0 ? V3{a.x / l, a.y / l, a.z / l} : a;
+}
+
+// Affine transform: 3x4 matrix (linear part and translation).
+struct Affine {
+  double m[3][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}};
+  static Affine Translation(V3 t) {
+    Affine a;
+    a.m[0][3] = t.x; a.m[1][3] = t.y; a.m[2][3] = t.z;
+    return a;
+  }
+  static Affine Scaling(V3 s) {
+    Affine a;
+    a.m[0][0] = s.x; a.m[1][1] = s.y; a.m[2][2] = s.z;
+    return a;
+  }
+  // Rotation by `angle` radians about the unit axis (Rodrigues).
+  static Affine Rotation(V3 axis, double angle) {
+    axis = Normalized(axis);
+    const double c = std::cos(angle), s = std::sin(angle), t = 1 - c;
+    const double x = axis.x, y = axis.y, z = axis.z;
+    Affine a;
+    a.m[0][0] = t * x * x + c;     a.m[0][1] = t * x * y - s * z; a.m[0][2] = t * x * z + s * y;
+    a.m[1][0] = t * x * y + s * z; a.m[1][1] = t * y * y + c;     a.m[1][2] = t * y * z - s * x;
+    a.m[2][0] = t * x * z - s * y; a.m[2][1] = t * y * z + s * x; a.m[2][2] = t * z * z + c;
+    return a;
+  }
+  static Affine FromRows(const double r[12]) {
+    Affine a;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 4; ++j) a.m[i][j] = r[4 * i + j];
+    return a;
+  }
+  Affine operator*(const Affine &b) const {
+    Affine r;
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 4; ++j) {
+        double v = m[i][0] * b.m[0][j] + m[i][1] * b.m[1][j] + m[i][2] * b.m[2][j];
+        if (j == 3) v += m[i][3];
+        r.m[i][j] = v;
+      }
+    }
+    return r;
+  }
+  V3 Point(V3 p) const {
+    return {m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3], m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3],
+            m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3]};
+  }
+  // Normals transform with the inverse transpose of the linear part.
+  V3 Normal(V3 n) const {
+    const double a = m[0][0], b = m[0][1], c = m[0][2], d = m[1][0], e = m[1][1], f = m[1][2], g = m[2][0], h = m[2][1],
+                 i = m[2][2];
+    // cofactor matrix = det * inverse transpose
+    const double C[3][3] = {{e * i - f * h, -(d * i - f * g), d * h - e * g},
+                            {-(b * i - c * h), a * i - c * g, -(a * h - b * g)},
+                            {b * f - c * e, -(a * f - c * d), a * e - b * d}};
+    const double det = a * C[0][0] + b * C[0][1] + c * C[0][2];
+    const double s = det < 0 ? -1.0 : 1.0;
+    V3 r{C[0][0] * n.x + C[0][1] * n.y + C[0][2] * n.z, C[1][0] * n.x + C[1][1] * n.y + C[1][2] * n.z,
+         C[2][0] * n.x + C[2][1] * n.y + C[2][2] * n.z};
+    return Normalized(V3{r.x * s, r.y * s, r.z * s});
+  }
+};
+
+// Heading (about y), pitch (about x), bank (about z): y up, z forward.
+Affine HeadingPitchBank(V3 r) {
+  return Affine::Rotation({0, 1, 0}, r.x) * Affine::Rotation({1, 0, 0}, r.y) * Affine::Rotation({0, 0, 1}, r.z);
+}
+
+// ---- text helpers ------------------------------------------------------------------------
+std::vector<std::string> Split(const std::string &s) {
+  std::istringstream is(s);
+  std::vector<std::string> out;
+  std::string w;
+  while (is >> w) out.push_back(w);
+  return out;
+}
+
+// Parses leading numbers after `skip` words; stops at the first word that is not a number.
+std::vector<double> Numbers(const std::vector<std::string> &w, size_t skip) {
+  std::vector<double> out;
+  for (size_t k = skip; k < w.size(); ++k) {
+    char *end = nullptr;
+    const double v = std::strtod(w[k].c_str(), &end);
+    if (end == w[k].c_str() || *end != '\0') break;
+    out.push_back(v);
+  }
+  return out;
+}
+
+bool StartsWith(const std::string &s, const std::string &p) { return s.compare(0, p.size(), p) == 0; }
+
+std::string ReadFile(const std::string &path) {
+  std::ifstream is(path, std::ios::binary);
+  if (!is.good()) throw std::runtime_error("could not open " + path);
+  std::stringstream ss;
+  ss << is.rdbuf();
+  return ss.str();
+}
+
+std::string DirOf(const std::string &path) {
+  const size_t p = path.find_last_of('/');
+  return p == std::string::npos ? std::string() : path.substr(0, p + 1);
+}
+
+std::string Lower(std::string s) {
+  for (char &c : s) c = (char)std::tolower((unsigned char)c);
+  return s;
+}
+
+// ---- minimal YAML (the subset the NFF's yaml{ }yaml blocks use) ---------------------------
+// Top-level keys with a block of "key: value" lines; values are scalars or [a, b, c] lists.
+using YamlMap = std::map<std::string, std::string>;
+std::vector<std::pair<std::string, YamlMap>> ParseYaml(const std::string &text) {
+  std::vector<std::pair<std::string, YamlMap>> out;
+  std::istringstream is(text);
+  std::string line;
+  while (std::getline(is, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line = line.substr(0, hash);
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    const size_t indent = line.find_first_not_of(" \t");
+    const size_t colon = line.find(':');
+    if (colon == std::string::npos) throw std::invalid_argument("yaml: expected 'key: value' in [" + line + "]");
+    auto trim = [](std::string s) {
+      const size_t a = s.find_first_not_of(" \t\r"), b = s.find_last_not_of(" \t\r");
+      return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+    };
+    const std::string key = trim(line.substr(0, colon)), value = trim(line.substr(colon + 1));
+    if (indent == 0) {
+      out.push_back({key, {}});
+      if (!value.empty()) out.back().second[""] = value;
+    } else {
+      if (out.empty()) throw std::invalid_argument("yaml: indented line without a parent key");
+      out.back().second[key] = value;
+    }
+  }
+  return out;
+}
+
+std::vector<double> YamlList(const std::string &v) {
+  std::string s = v;
+  for (char &c : s)
+    if (c == '[' || c == ']' || c == ',') c = ' ';
+  return Numbers(Split(s), 0);
+}
+
+bool YamlBool(const std::string &v) {
+  const std::string l = Lower(v);
+  return l == "true" || l == "yes" || l == "on" || l == "1";
+}
+
+// ---- scope --------------------------------------------------------------------------------
+struct NamedMaterial {
+  int shader, medium, emitter;
+};
+
+struct Scope {
+  Affine xf;
+  std::map<std::string, int> shaders, media, emitters;  // -1: none
+  std::map<std::string, NamedMaterial> materials;
+  int shader = -1, medium = -1, emitter = -1;  // active
+};
+
+// ---- minimal XML for COLLADA ---------------------------------------------------------------
+struct Xml {
+  std::string tag, text;
+  std::map<std::string, std::string> attr;
+  std::vector<std::unique_ptr<Xml>> kids;
+  const Xml *Child(const std::string &t) const {
+    for (auto &k : kids)
+      if (k->tag == t) return k.get();
+    return nullptr;
+  }
+  std::vector<const Xml *> Children(const std::string &t) const {
+    std::vector<const Xml *> out;
+    for (auto &k : kids)
+      if (k->tag == t) out.push_back(k.get());
+    return out;
+  }
+  std::string Attr(const std::string &a, const std::string &def = "") const {
+    auto it = attr.find(a);
+    return it == attr.end() ? def : it->second;
+  }
+};
+
+std::unique_ptr<Xml> ParseXml(const std::string &s) {
+  auto root = std::make_unique<Xml>();
+  std::vector<Xml *> stack{root.get()};
+  size_t i = 0;
+  const size_t n = s.size();
+  while (i < n) {
+    if (s[i] != '<') {
+      const size_t j = s.find('<', i);
+      stack.back()->text += s.substr(i, (j == std::string::npos ? n : j) - i);
+      i = j == std::string::npos ? n : j;
+      continue;
+    }
+    if (s.compare(i, 4, "<!--") == 0) {
+      const size_t j = s.find("-->", i);
+      i = j == std::string::npos ? n : j + 3;
+      continue;
+    }
+    if (s.compare(i, 2, "<?") == 0 || s.compare(i, 2, "<!") == 0) {
+      const size_t j = s.find('>', i);
+      i = j == std::string::npos ? n : j + 1;
+      continue;
+    }
+    const size_t j = s.find('>', i);
+    if (j == std::string::npos) throw std::invalid_argument("xml: unterminated tag");
+    std::string body = s.substr(i + 1, j - i - 1);
+    i = j + 1;
+    if (!body.empty() && body[0] == '/') {
+      if (stack.size() > 1) stack.pop_back();
+      continue;
+    }
+    const bool self_close = !body.empty() && body.back() == '/';
+    if (self_close) body.pop_back();
+    auto node = std::make_unique<Xml>();
+    size_t k = 0;
+    while (k < body.size() && !std::isspace((unsigned char)body[k])) ++k;
+    node->tag = body.substr(0, k);
+    // attributes: name="value"
+    while (k < body.size()) {
+      while (k < body.size() && std::isspace((unsigned char)body[k])) ++k;
+      const size_t eq = body.find('=', k);
+      if (eq == std::string::npos) break;
+      std::string name = body.substr(k, eq - k);
+      name.erase(std::remove_if(name.begin(), name.end(), [](char c) { return std::isspace((unsigned char)c); }), name.end());
+      const size_t q0 = body.find_first_of("\"'", eq);
+      if (q0 == std::string::npos) break;
+      const size_t q1 = body.find(body[q0], q0 + 1);
+      if (q1 == std::string::npos) break;
+      node->attr[name] = body.substr(q0 + 1, q1 - q0 - 1);
+      k = q1 + 1;
+    }
+    Xml *raw = node.get();
+    stack.back()->kids.push_back(std::move(node));
+    if (!self_close) stack.push_back(raw);
+  }
+  return root;
+}
+
+std::vector<double> XmlNumbers(const Xml *x) { return x ? Numbers(Split(x->text), 0) : std::vector<double>{}; }
+
+// ---- the parser ----------------------------------------------------------------------------
+class Parser {
+ public:
+  Parser(SceneData &sd, const std::string &text, const std::string &path) : sd_(sd), path_(path) {
+    std::istringstream is(text);
+    std::string l;
+    while (std::getline(is, l)) {
+      if (!l.empty() && l.back() == '\r') l.pop_back();
+      lines_.push_back(l);
+    }
+  }
+
+  void Parse(Scope &scope);
+
+ private:
+  SceneData &sd_;
+  std::string path_;
+  std::vector<std::string> lines_;
+  size_t pos_ = 0;  // next line
+  std::string line_;
+
+  bool NextLine() {
+    if (pos_ >= lines_.size()) return false;
+    line_ = lines_[pos_++];
+    return true;
+  }
+  const std::string *Peek() const { return pos_ < lines_.size() ? &lines_[pos_] : nullptr; }
+  [[noreturn]] void Fail(const std::string &msg) const {
+    std::ostringstream os;
+    if (!path_.empty()) os << path_ << ":";
+    os << pos_ << ": " << msg << " [" << line_ << "]";
+    throw std::invalid_argument(os.str());
+  }
+  std::string FullPath(const std::string &name) const {
+    if (!name.empty() && name[0] == '/') return name;
+    return DirOf(path_) + name;
+  }
+
+  int AddShader(const std::string &name, Scope &scope, const ttgpu_shader &s) {
+    sd_.shaders.push_back(s);
+    const int id = (int)sd_.shaders.size() - 1;
+    scope.shaders[name] = id;
+    scope.shader = id;
+    return id;
+  }
+  int AddMedium(const std::string &name, Scope &scope, const ttgpu_medium &m) {
+    sd_.media.push_back(m);
+    const int id = (int)sd_.media.size() - 1;
+    scope.media[name] = id;
+    scope.medium = id;
+    return id;
+  }
+  void Activate(std::map<std::string, int> &table, int &active, const std::string &name, const char *what) {
+    auto it = table.find(name);
+    if (it == table.end()) Fail(std::string("Error: ") + what + " " + name + " not defined. Define it in the NFF file prior to referencing it.");
+    active = it->second;
+  }
+  int ActiveMaterial(const Scope &s) { return sd_.Material(s.shader, s.medium, s.emitter); }
+  // GetMaterial (src/parsenff.cxx:47-68): a named material, else the shader of that name made active.
+  int MaterialByName(Scope &scope, const std::string &name) {
+    if (!name.empty() && name != "DefaultMaterial") {
+      auto it = scope.materials.find(name);
+      if (it != scope.materials.end()) return sd_.Material(it->second.shader, it->second.medium, it->second.emitter);
+      Activate(scope.shaders, scope.shader, name, "Shader");
+    }
+    return ActiveMaterial(scope);
+  }
+  void RejectTexture(const char *identifier) {
+    const std::string *p = Peek();
+    if (p && StartsWith(*p, identifier)) {
+      NextLine();
+      Fail(std::string(identifier) + ": textures are not supported by this renderer");
+    }
+  }
+
+  void Camera(bool fisheye);
+  void Polygon(const Scope &scope, int n);
+  void Mesh(Scope &scope, const std::string &file);
+  void Obj(Scope &scope, const std::string &file);
+  void Dae(Scope &scope, const std::string &file);
+  void AddTriangles(const std::vector<V3> &pos, const std::vector<V3> *nrm, const std::vector<std::array<uint32_t, 3>> &tris,
+                    int material);
+  void Yaml(const std::string &text, Scope &scope);
+  ttgpu_shader Shader(int kind, const double rgb[3]) const {
+    ttgpu_shader s;
+    std::memset(&s, 0, sizeof(s));
+    s.kind = kind;
+    s.ior_ratio = 1.f;
+    if (rgb) RgbToSpectrum(rgb, s.reflectance);
+    return s;
+  }
+};
+
+void Parser::Camera(bool fisheye) {
+  V3 from, at, up;
+  int rx = 0, ry = 0;
+  auto expect = [&](const char *key, int n) {
+    if (!NextLine()) Fail("camera: missing line");
+    auto w = Split(line_);
+    auto v = Numbers(w, 1);
+    if (w.empty() || w[0] != key || (int)v.size() < n) Fail(std::string("camera: expected '") + key + "'");
+    return v;
+  };
+  auto f = expect("from", 3), a = expect("at", 3), u = expect("up", 3), r = expect("resolution", 2);
+  from = {f[0], f[1], f[2]};
+  at = {a[0], a[1], a[2]};
+  up = {u[0], u[1], u[2]};
+  rx = (int)r[0];
+  ry = (int)r[1];
+  double angle = 180.0;
+  if (!fisheye) angle = expect("angle", 1)[0];
+  ttgpu_camera &c = sd_.camera;
+  c.kind = fisheye ? TTGPU_CAMERA_FISHEYE : TTGPU_CAMERA_PINHOLE;
+  const V3 d = at - from;
+  c.pos[0] = from.x; c.pos[1] = from.y; c.pos[2] = from.z;
+  c.dir[0] = d.x; c.dir[1] = d.y; c.dir[2] = d.z;
+  c.up[0] = up.x; c.up[1] = up.y; c.up[2] = up.z;
+  c.fov_deg = angle;
+  sd_.has_camera = true;
+  sd_.width = rx;
+  sd_.height = ry;
+}
+
+// Triangles with per-vertex normals (flat normals when `nrm` is null: vertices are duplicated per
+// triangle with the face normal).  Zero-area triangles are dropped.
+void Parser::AddTriangles(const std::vector<V3> &pos, const std::vector<V3> *nrm, const std::vector<std::array<uint32_t, 3>> &tris,
+                          int material) {
+  for (const auto &t : tris) {
+    const V3 a = pos[t[0]], b = pos[t[1]], c = pos[t[2]];
+    const V3 fn = Cross(b - a, c - a);
+    if (!(Len(fn) > 0)) continue;
+    const V3 face = Normalized(fn);
+    const uint32_t base = (uint32_t)(sd_.vertices.size() / 3);
+    for (int j = 0; j < 3; ++j) {
+      const V3 p = pos[t[j]];
+      const V3 n = nrm ? (*nrm)[t[j]] : face;
+      sd_.vertices.insert(sd_.vertices.end(), {(float)p.x, (float)p.y, (float)p.z});
+      sd_.normals.insert(sd_.normals.end(), {(float)n.x, (float)n.y, (float)n.z});
+      sd_.indices.push_back(base + j);
+    }
+    sd_.tri_material.push_back(material);
+  }
+}
+
+void Parser::Polygon(const Scope &scope, int n) {
+  if (n < 3) Fail("Polygon must be specified with at least 3 vertices");
+  std::vector<V3> pos(n), nrm(n);
+  bool flat = false;
+  for (int i = 0; i < n; ++i) {
+    if (!NextLine()) Fail("Cannot read specified number of vertices");
+    auto v = Numbers(Split(line_), 0);
+    if (v.size() < 3) Fail("Cannot read vertex coordinates");
+    flat |= v.size() < 6;
+    pos[i] = scope.xf.Point({v[0], v[1], v[2]});
+    if (v.size() >= 6) nrm[i] = scope.xf.Normal({v[3], v[4], v[5]});
+  }
+  std::vector<std::array<uint32_t, 3>> tris;
+  for (int i = 0; i < n - 2; ++i) tris.push_back({0u, (uint32_t)i + 1, (uint32_t)i + 2});
+  // Polygon triangles keep zero-area ones out as well (the GPU cannot shade them).
+  AddTriangles(pos, flat ? nullptr : &nrm, tris, ActiveMaterial(scope));
+}
+
+void Parser::Mesh(Scope &original, const std::string &file) {
+  Scope scope = original;  // activations made by material look-ups stay local (src/parsenff.cxx:922-930)
+  const std::string ext = Lower(file.substr(file.find_last_of('.') == std::string::npos ? file.size() : file.find_last_of('.')));
+  if (ext == ".obj") Obj(scope, file);
+  else if (ext == ".dae") Dae(scope, file);
+  else Fail("mesh format not supported (OBJ and COLLADA are): " + file);
+}
+
+// Wavefront OBJ: v, vn, f (fans, negative indices), usemtl selects the material by name.
+void Parser::Obj(Scope &scope, const std::string &file) {
+  const std::string text = ReadFile(FullPath(file));
+  std::vector<V3> v, vn;
+  struct Group {
+    std::vector<V3> pos, nrm;
+    std::vector<std::array<uint32_t, 3>> tris;
+    bool normals = true;
+  };
+  std::map<std::string, Group> groups;  // per material name
+  std::vector<std::string> order;
+  std::string mtl = "DefaultMaterial";
+  std::istringstream is(text);
+  std::string line;
+  while (std::getline(is, line)) {
+    auto w = Split(line);
+    if (w.empty() || w[0][0] == '#') continue;
+    if (w[0] == "v") {
+      auto x = Numbers(w, 1);
+      if (x.size() < 3) throw std::invalid_argument(file + ": bad vertex");
+      v.push_back({x[0], x[1], x[2]});
+    } else if (w[0] == "vn") {
+      auto x = Numbers(w, 1);
+      if (x.size() < 3) throw std::invalid_argument(file + ": bad normal");
+      vn.push_back({x[0], x[1], x[2]});
+    } else if (w[0] == "usemtl" && w.size() > 1) {
+      mtl = w[1];
+    } else if (w[0] == "f") {
+      if (!groups.count(mtl)) order.push_back(mtl);
+      Group &g = groups[mtl];
+      std::vector<uint32_t> corner;
+      for (size_t k = 1; k < w.size(); ++k) {
+        int iv = 0, it = 0, in = 0;
+        const std::string &c = w[k];
+        const size_t s1 = c.find('/');
+        iv = std::atoi(c.substr(0, s1).c_str());
+        if (s1 != std::string::npos) {
+          const size_t s2 = c.find('/', s1 + 1);
+          if (s2 != std::string::npos) in = std::atoi(c.substr(s2 + 1).c_str());
+          (void)it;
+        }
+        if (iv < 0) iv = (int)v.size() + iv + 1;
+        if (in < 0) in = (int)vn.size() + in + 1;
+        if (iv < 1 || iv > (int)v.size()) throw std::invalid_argument(file + ": vertex index out of range");
+        g.pos.push_back(scope.xf.Point(v[iv - 1]));
+        if (in >= 1 && in <= (int)vn.size()) g.nrm.push_back(scope.xf.Normal(vn[in - 1]));
+        else {
+          g.nrm.push_back({0, 0, 1});
+          g.normals = false;
+        }
+        corner.push_back((uint32_t)g.pos.size() - 1);
+      }
+      for (size_t k = 1; k + 1 < corner.size(); ++k) g.tris.push_back({corner[0], corner[k], corner[k + 1]});
+    }
+  }
+  for (const auto &name : order) {
+    Group &g = groups[name];
+    const int mat = MaterialByName(scope, name);
+    AddTriangles(g.pos, g.normals ? &g.nrm : nullptr, g.tris, mat);
+  }
+}
+
+// COLLADA: geometries with triangles / polylist / polygons, instanced by the visual scene's
+// nodes (matrix / translate / rotate / scale), scaled by the asset's unit, Z_UP turned to Y_UP.
+void Parser::Dae(Scope &scope, const std::string &file) {
+  auto doc = ParseXml(ReadFile(FullPath(file)));
+  const Xml *col = doc->Child("COLLADA");
+  if (!col) throw std::invalid_argument(file + ": not a COLLADA document");
+  double unit = 1.0;
+  std::string up = "Y_UP";
+  if (const Xml *asset = col->Child("asset")) {
+    if (const Xml *u = asset->Child("unit")) unit = std::atof(u->Attr("meter", "1").c_str());
+    if (const Xml *ua = asset->Child("up_axis")) up = Split(ua->text).empty() ? up : Split(ua->text)[0];
+  }
+  Affine root = Affine::Scaling({unit, unit, unit});
+  if (up == "Z_UP") {
+    const double r[12] = {1, 0, 0, 0, 0, 0, 1, 0, 0, -1, 0, 0};
+    root = root * Affine::FromRows(r);
+  } else if (up == "X_UP") {
+    const double r[12] = {0, -1, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0};
+    root = root * Affine::FromRows(r);
+  }
+  std::map<std::string, std::string> material_name;  // id -> name
+  if (const Xml *lm = col->Child("library_materials"))
+    for (const Xml *m : lm->Children("material")) material_name[m->Attr("id")] = m->Attr("name", m->Attr("id"));
+  std::map<std::string, const Xml *> geometries;
+  if (const Xml *lg = col->Child("library_geometries"))
+    for (const Xml *g : lg->Children("geometry")) geometries[g->Attr("id")] = g;
+  auto strip = [](const std::string &u) { return !u.empty() && u[0] == '#' ? u.substr(1) : u; };
+
+  auto instance = [&](const Xml *geom, const Affine &xf, const std::map<std::string, std::string> &bind) {
+    const Xml *mesh = geom->Child("mesh");
+    if (!mesh) return;
+    std::map<std::string, std::pair<std::vector<double>, int>> sources;  // id -> values, stride
+    for (const Xml *s : mesh->Children("source")) {
+      int stride = 1;
+      if (const Xml *tc = s->Child("technique_common"))
+        if (const Xml *acc = tc->Child("accessor")) stride = std::max(1, std::atoi(acc->Attr("stride", "1").c_str()));
+      sources[s->Attr("id")] = {XmlNumbers(s->Child("float_array")), stride};
+    }
+    std::map<std::string, std::string> vertices_pos, vertices_nrm;  // <vertices> id -> source
+    for (const Xml *vx : mesh->Children("vertices"))
+      for (const Xml *in : vx->Children("input")) {
+        if (in->Attr("semantic") == "POSITION") vertices_pos[vx->Attr("id")] = strip(in->Attr("source"));
+        if (in->Attr("semantic") == "NORMAL") vertices_nrm[vx->Attr("id")] = strip(in->Attr("source"));
+      }
+    for (const auto &kind : {"triangles", "polylist", "polygons"}) {
+      for (const Xml *pl : mesh->Children(kind)) {
+        int off_v = -1, off_n = -1, nin = 0;
+        std::string src_v, src_n;
+        for (const Xml *in : pl->Children("input")) {
+          const int off = std::atoi(in->Attr("offset", "0").c_str());
+          nin = std::max(nin, off + 1);
+          const std::string sem = in->Attr("semantic");
+          if (sem == "VERTEX") {
+            off_v = off;
+            src_v = vertices_pos[strip(in->Attr("source"))];
+            if (vertices_nrm.count(strip(in->Attr("source")))) {
+              off_n = off;
+              src_n = vertices_nrm[strip(in->Attr("source"))];
+            }
+          } else if (sem == "NORMAL") {
+            off_n = off;
+            src_n = strip(in->Attr("source"));
+          }
+        }
+        if (off_v < 0 || !sources.count(src_v)) throw std::invalid_argument(file + ": primitive without positions");
+        const auto &P = sources[src_v];
+        const std::pair<std::vector<double>, int> *N = off_n >= 0 && sources.count(src_n) ? &sources[src_n] : nullptr;
+        // polygon sizes
+        std::vector<int> sizes;
+        std::vector<double> idx;
+        if (std::string(kind) == "triangles") {
+          idx = XmlNumbers(pl->Child("p"));
+          sizes.assign(idx.size() / (3 * (size_t)nin), 3);
+        } else if (std::string(kind) == "polylist") {
+          for (double c : XmlNumbers(pl->Child("vcount"))) sizes.push_back((int)c);
+          idx = XmlNumbers(pl->Child("p"));
+        } else {
+          for (const Xml *p : pl->Children("p")) {
+            auto v = XmlNumbers(p);
+            sizes.push_back((int)(v.size() / nin));
+            idx.insert(idx.end(), v.begin(), v.end());
+          }
+        }
+        std::vector<V3> pos, nrm;
+        std::vector<std::array<uint32_t, 3>> tris;
+        size_t at = 0;
+        for (int sz : sizes) {
+          const uint32_t first = (uint32_t)pos.size();
+          for (int c = 0; c < sz; ++c, at += nin) {
+            if (at + nin > idx.size()) throw std::invalid_argument(file + ": index list too short");
+            const size_t vi = (size_t)idx[at + off_v];
+            if ((vi + 1) * P.second > P.first.size() || P.second < 3) throw std::invalid_argument(file + ": position index out of range");
+            pos.push_back(xf.Point({P.first[vi * P.second], P.first[vi * P.second + 1], P.first[vi * P.second + 2]}));
+            if (N) {
+              const size_t ni = (size_t)idx[at + off_n];
+              if ((ni + 1) * N->second > N->first.size() || N->second < 3) throw std::invalid_argument(file + ": normal index out of range");
+              nrm.push_back(xf.Normal({N->first[ni * N->second], N->first[ni * N->second + 1], N->first[ni * N->second + 2]}));
+            }
+          }
+          for (int c = 1; c + 1 < sz; ++c) tris.push_back({first, first + c, first + c + 1});
+        }
+        std::string mname = "DefaultMaterial";
+        const std::string sym = pl->Attr("material");
+        if (!sym.empty()) {
+          auto b = bind.find(sym);
+          const std::string target = b != bind.end() ? b->second : sym;
+          auto mn = material_name.find(target);
+          mname = mn != material_name.end() ? mn->second : target;
+        }
+        AddTriangles(pos, N ? &nrm : nullptr, tris, MaterialByName(scope, mname));
+      }
+    }
+  };
+
+  std::function<void(const Xml *, const Affine &)> node = [&](const Xml *nd, const Affine &parent) {
+    Affine local;
+    for (auto &k : nd->kids) {
+      const auto v = XmlNumbers(k.get());
+      if (k->tag == "matrix" && v.size() >= 12) local = local * Affine::FromRows(v.data());
+      else if (k->tag == "translate" && v.size() >= 3) local = local * Affine::Translation({v[0], v[1], v[2]});
+      else if (k->tag == "rotate" && v.size() >= 4) local = local * Affine::Rotation({v[0], v[1], v[2]}, v[3] * kPi / 180.);
+      else if (k->tag == "scale" && v.size() >= 3) local = local * Affine::Scaling({v[0], v[1], v[2]});
+    }
+    const Affine world = parent * local;
+    for (const Xml *ig : nd->Children("instance_geometry")) {
+      auto g = geometries.find(strip(ig->Attr("url")));
+      if (g == geometries.end()) continue;
+      std::map<std::string, std::string> bind;
+      if (const Xml *bm = ig->Child("bind_material"))
+        if (const Xml *tc = bm->Child("technique_common"))
+          for (const Xml *im : tc->Children("instance_material")) bind[im->Attr("symbol")] = strip(im->Attr("target"));
+      instance(g->second, world, bind);
+    }
+    for (const Xml *c : nd->Children("node")) node(c, world);
+  };
+  const Affine top = scope.xf * root;
+  if (const Xml *lvs = col->Child("library_visual_scenes"))
+    for (const Xml *vs : lvs->Children("visual_scene"))
+      for (const Xml *nd : vs->Children("node")) node(nd, top);
+}
+
+void Parser::Yaml(const std::string &text, Scope &scope) {
+  for (auto &entry : ParseYaml(text)) {
+    const std::string &key = entry.first;
+    YamlMap &m = entry.second;
+    if (key == "transform") {
+      const bool deg = m.count("angle_in_degree") && YamlBool(m["angle_in_degree"]);
+      const double k = deg ? kPi / 180. : 1.;
+      Affine xf;
+      if (m.count("pos")) {
+        auto p = YamlList(m["pos"]);
+        if (p.size() != 3) Fail("yaml transform: pos needs 3 numbers");
+        xf = Affine::Translation({p[0], p[1], p[2]});
+      }
+      if (m.count("hpb")) {
+        auto r = YamlList(m["hpb"]);
+        if (r.size() != 3) Fail("yaml transform: hpb needs 3 numbers");
+        xf = xf * HeadingPitchBank({r[0] * k, r[1] * k, r[2] * k});
+      }
+      if (m.count("rotaxis")) {
+        auto r = YamlList(m["rotaxis"]);
+        if (r.size() != 3) Fail("yaml transform: rotaxis needs 3 numbers");
+        const V3 a{r[0] * k, r[1] * k, r[2] * k};
+        if (Len(a) > 0) xf = xf * Affine::Rotation(a, Len(a));
+      }
+      if (m.count("scale")) {
+        auto s = YamlList(m["scale"]);
+        if (s.size() != 3) Fail("yaml transform: scale needs 3 numbers");
+        xf = xf * Affine::Scaling({s[0], s[1], s[2]});
+      }
+      scope.xf = xf;
+    } else if (key == "shader") {
+      const std::string cls = m["class"], name = m["name"];
+      if (name.empty()) Fail("yaml shader: missing name");
+      if (m.count("alpha_texture")) Fail("yaml shader: alpha_texture: textures are not supported by this renderer");
+      ttgpu_shader s = Shader(TTGPU_SHADER_DIFFUSE, nullptr);
+      if (cls == "speculartransmissivedielectric") {
+        s.kind = TTGPU_SHADER_DIELECTRIC;
+        s.ior_ratio = (float)std::atof(m["ior_ratio"].c_str());
+        if (m.count("abbe_number")) {
+          // Abbe number v = (n(589) - 1) / (n(486) - n(656)), ior_ratio taken at 589 nm.
+          const double v = std::atof(m["abbe_number"].c_str());
+          s.ior_lambda_coeff = (float)((s.ior_ratio - 1) / v / (656. - 486.));
+        }
+      } else if (cls == "glossytransmissivedielectric") {
+        s.kind = TTGPU_SHADER_GLOSSY_DIELECTRIC;
+        s.ior_ratio = (float)std::atof(m["ior_ratio"].c_str());
+        s.alpha = (float)std::atof(m["alpha"].c_str());
+        s.alpha_min = s.alpha;
+      } else if (cls == "glossy") {
+        s.kind = TTGPU_SHADER_BECKMANN_METAL;
+        s.alpha = (float)std::atof(m["alpha"].c_str());
+        double rgb[3] = {1, 1, 1};
+        if (m.count("rgb")) {
+          auto c = YamlList(m["rgb"]);
+          if (c.size() != 3) Fail("yaml shader: rgb needs 3 numbers");
+          rgb[0] = c[0]; rgb[1] = c[1]; rgb[2] = c[2];
+        }
+        const double x = m.count("x") ? std::atof(m["x"].c_str()) : 1.0;
+        for (double &c : rgb) c *= x;
+        RgbToSpectrum(rgb, s.reflectance);
+      } else {
+        Fail("Unknown shader class in yaml: " + cls);
+      }
+      AddShader(name, scope, s);
+    } else if (key == "material") {
+      const std::string name = m["name"];
+      if (name.empty()) Fail("yaml material: missing name");
+      scope.materials[name] = {scope.shader, scope.medium, scope.emitter};
+    } else {
+      Fail("unknown yaml key " + key);
+    }
+  }
+}
+
+void Parser::Parse(Scope &scope) {
+  while (NextLine()) {
+    if (line_.empty() || line_[0] == '#') continue;
+    const auto w = Split(line_);
+    if (w.empty()) continue;
+    const std::string &tok = w[0];
+    if (tok == "{") {
+      Scope child = scope;
+      Parse(child);
+      continue;
+    }
+    if (tok == "}") break;
+    if (tok == "yaml{") {
+      std::string text;
+      bool closed = false;
+      while (NextLine()) {
+        if (StartsWith(line_, "}yaml")) {
+          closed = true;
+          break;
+        }
+        text += line_ + "\n";
+      }
+      if (!closed) Fail("Error. Yaml section was not terminated with }yaml token.");
+      Yaml(text, scope);
+      continue;
+    }
+    const auto num = Numbers(w, 1);
+    if (tok == "transform") {
+      const size_t n = num.size();
+      if (n != 0 && n != 3 && n != 6 && n != 9) Fail("Bad number of arguments to transform directive");
+      Affine xf;
+      if (n >= 3) xf = Affine::Translation({num[0], num[1], num[2]});
+      if (n >= 6) xf = xf * HeadingPitchBank({num[3], num[4], num[5]});
+      if (n >= 9) xf = xf * Affine::Scaling({num[6], num[7], num[8]});
+      scope.xf = xf;
+    } else if (tok == "v" || tok == "vfisheye") {
+      Camera(tok == "vfisheye");
+    } else if (tok == "s") {
+      if (num.size() != 4) Fail("Error");
+      const V3 c = scope.xf.Point({num[0], num[1], num[2]});
+      sd_.spheres.insert(sd_.spheres.end(), {(float)c.x, (float)c.y, (float)c.z, (float)num[3]});
+      sd_.sphere_material.push_back(ActiveMaterial(scope));
+    } else if (tok == "p") {
+      if (num.empty()) Fail("Error");
+      Polygon(scope, (int)num[0]);
+    } else if (tok == "shader") {
+      if (w.size() < 2) Fail("shader directive needs name of the shader.");
+      Activate(scope.shaders, scope.shader, w[1], "Shader");
+    } else if (tok == "diffuse") {
+      const auto n = Numbers(w, 2);
+      if (w.size() < 2 || n.size() != 4) Fail("Error");
+      const double rgb[3] = {n[3] * n[0], n[3] * n[1], n[3] * n[2]};
+      RejectTexture("diffusetexture");
+      AddShader(w[1], scope, Shader(TTGPU_SHADER_DIFFUSE, rgb));
+    } else if (tok == "specularreflective") {
+      const auto n = Numbers(w, 2);
+      if (w.size() < 2 || n.size() != 4) Fail("Error");
+      const double rgb[3] = {n[3] * n[0], n[3] * n[1], n[3] * n[2]};
+      AddShader(w[1], scope, Shader(TTGPU_SHADER_MIRROR, rgb));
+    } else if (tok == "speculartransmissivedielectric") {
+      const auto n = Numbers(w, 2);
+      if (w.size() < 2 || n.size() != 1) Fail("Error");
+      ttgpu_shader s = Shader(TTGPU_SHADER_DIELECTRIC, nullptr);
+      s.ior_ratio = (float)n[0];
+      AddShader(w[1], scope, s);
+    } else if (tok == "speculardensedielectric") {
+      const auto n = Numbers(w, 2);
+      if (w.size() < 2 || n.size() != 4) Fail("Error");
+      const double rgb[3] = {n[0], n[1], n[2]};
+      ttgpu_shader s = Shader(TTGPU_SHADER_DENSE_DIELECTRIC, rgb);
+      s.specular = (float)n[3];
+      RejectTexture("diffusetexture");
+      AddShader(w[1], scope, s);
+    } else if (tok == "glossy") {
+      const auto n = Numbers(w, 2);
+      if (w.size() < 2 || n.size() != 5) Fail("Error");
+      const double rgb[3] = {n[3] * n[0], n[3] * n[1], n[3] * n[2]};
+      ttgpu_shader s = Shader(TTGPU_SHADER_BECKMANN_METAL, rgb);
+      s.alpha = (float)n[4];
+      RejectTexture("exponenttexture");
+      AddShader(w[1], scope, s);
+    } else if (tok == "medium") {
+      const auto n = Numbers(w, 2);
+      if (w.size() < 2) Fail("Error");
+      if (n.empty() && w.size() == 2) {
+        Activate(scope.media, scope.medium, w[1], "Medium");
+        continue;
+      }
+      if (n.size() != 2 && n.size() != 6) Fail("Error");
+      ttgpu_medium m;
+      std::memset(&m, 0, sizeof(m));
+      m.kind = TTGPU_MEDIUM_HOMOGENEOUS;
+      m.priority = (int)scope.media.size();
+      m.phase_kind = TTGPU_PHASE_UNIFORM;
+      const double ss[3] = {n[0], n.size() == 6 ? n[1] : n[0], n.size() == 6 ? n[2] : n[0]};
+      const double sa[3] = {n.size() == 6 ? n[3] : n[1], n.size() == 6 ? n[4] : n[1], n.size() == 6 ? n[5] : n[1]};
+      RgbToSpectrum(ss, m.sigma_s);
+      RgbToSpectrum(sa, m.sigma_a);
+      const std::string *pk = Peek();
+      if (pk && StartsWith(*pk, "pf ")) {
+        NextLine();
+        const auto pw = Split(line_);
+        const auto pn = Numbers(pw, 2);
+        if (pw.size() >= 2 && pw[1] == "rayleigh") m.phase_kind = TTGPU_PHASE_RAYLEIGH;
+        else if (pw.size() >= 2 && pw[1] == "henleygreenstein" && !pn.empty()) {
+          m.phase_kind = TTGPU_PHASE_HENYEY_GREENSTEIN;
+          m.phase_g = (float)pn[0];
+        } else Fail("Error");
+      }
+      AddMedium(w[1], scope, m);
+    } else if (tok == "vacuummedium") {
+      if (w.size() < 2) Fail("Error");
+      ttgpu_medium m;
+      std::memset(&m, 0, sizeof(m));
+      m.kind = TTGPU_MEDIUM_VACUUM;
+      m.priority = (int)scope.media.size();
+      AddMedium(w[1], scope, m);
+    } else if (tok == "simpleatmosphere") {
+      const auto n = Numbers(w, 2);
+      if (w.size() < 2) Fail("Error");
+      if (n.empty()) {
+        Activate(scope.media, scope.medium, w[1], "Medium");
+        continue;
+      }
+      if (n.size() != 4) Fail("Error");
+      // Atmosphere::MakeSimple (src/atmosphere.cxx:132-160, 355-358): lengths in km, exponential
+      // Rayleigh (scale height 8) and aerosol (1.2, sigma_s 20e-3, sigma_a 2.22e-3 per km).
+      ttgpu_medium m;
+      std::memset(&m, 0, sizeof(m));
+      m.kind = TTGPU_MEDIUM_ATMOSPHERE;
+      m.priority = (int)scope.media.size();
+      m.phase_kind = TTGPU_PHASE_RAYLEIGH;
+      m.planet_center[0] = n[0]; m.planet_center[1] = n[1]; m.planet_center[2] = n[2];
+      m.planet_radius = n[3];
+      m.atmosphere_height = 0;
+      m.rayleigh_scale_height = 8.0;
+      m.mie_scale_height = 1.2;
+      m.mie_g = 0.76f;
+      const double Na = 6.022e23, rho = 1.225, mol = 29e-3, ior = 1.00028;
+      const double N = rho * Na / mol, ior_term = (ior * ior - 1) * (ior * ior - 1);
+      const double pref = 8. * kPi * kPi * kPi / 3. * ior_term / N * 1.e27 * 1.e12;
+      for (int b = 0; b < TTGPU_NBINS; ++b) {
+        const double l = ttgpu_tables::kBinCenters[b];
+        m.rayleigh_sigma_s[b] = (float)(pref / (l * l * l * l));
+        m.mie_sigma_s[b] = 20.e-3f;
+        m.mie_sigma_a[b] = 2.22e-3f;
+      }
+      AddMedium(w[1], scope, m);
+    } else if (tok == "tabulatedatmosphere") {
+      if (w.size() == 2) {
+        Activate(scope.media, scope.medium, w[1], "Medium");
+        continue;
+      }
+      Fail("tabulatedatmosphere: tabulated atmospheres are not supported by this renderer");
+    } else if (tok == "lsun") {
+      if (num.size() != 4 && num.size() != 5) Fail("Error");
+      const V3 d = Normalized({num[0], num[1], num[2]});
+      ttgpu_env_light e;
+      std::memset(&e, 0, sizeof(e));
+      e.kind = TTGPU_ENV_SUN;
+      e.dir[0] = -d.x; e.dir[1] = -d.y; e.dir[2] = -d.z;  // the NFF gives the direction light travels
+      e.total_power = num[3];
+      e.opening_angle_deg = num.size() == 5 ? num[4] : 0.26;
+      sd_.envs.push_back(e);
+    } else if (tok == "lddirection" || tok == "lddome") {
+      if (num.size() != 6) Fail("Error");
+      const V3 d = Normalized({num[0], num[1], num[2]});
+      ttgpu_env_light e;
+      std::memset(&e, 0, sizeof(e));
+      const double rgb[3] = {num[3], num[4], num[5]};
+      RgbToSpectrum(rgb, e.spectrum);
+      if (tok == "lddirection") {
+        e.kind = TTGPU_ENV_DIRECTIONAL;
+        e.dir[0] = -d.x; e.dir[1] = -d.y; e.dir[2] = -d.z;
+      } else {
+        e.kind = TTGPU_ENV_DOME;
+        e.dir[0] = d.x; e.dir[1] = d.y; e.dir[2] = d.z;
+      }
+      sd_.envs.push_back(e);
+    } else if (tok == "lenv") {
+      if (num.size() != 3 || w.size() < 5) Fail("Error");
+      const std::string file = FullPath(w[4]);
+      if (Lower(file.substr(file.size() >= 4 ? file.size() - 4 : 0)) != ".pfm") Fail("lenv: only PFM images are supported: " + file);
+      auto tex = std::make_unique<std::vector<float>>();
+      int tw = 0, th = 0;
+      {
+        const std::string data = ReadFile(file);
+        std::istringstream is(data);
+        std::string magic;
+        double scale = 0;
+        is >> magic >> tw >> th >> scale;
+        is.get();
+        if (magic != "PF" || tw <= 0 || th <= 0) Fail("lenv: not an RGB PFM image: " + file);
+        const size_t nbytes = (size_t)tw * th * 3 * 4;
+        const size_t off = (size_t)is.tellg();
+        if (data.size() < off + nbytes) Fail("lenv: truncated PFM image: " + file);
+        std::vector<float> raw((size_t)tw * th * 3);
+        std::memcpy(raw.data(), data.data() + off, nbytes);
+        if (scale > 0) {  // big-endian
+          for (float &f : raw) {
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            u = __builtin_bswap32(u);
+            std::memcpy(&f, &u, 4);
+          }
+        }
+        // PFM stores the bottom row first; the texture rows run top to bottom.
+        tex->resize(raw.size());
+        for (int y = 0; y < th; ++y) std::memcpy(tex->data() + (size_t)y * tw * 3, raw.data() + (size_t)(th - 1 - y) * tw * 3, (size_t)tw * 3 * 4);
+      }
+      ttgpu_env_light e;
+      std::memset(&e, 0, sizeof(e));
+      e.kind = TTGPU_ENV_MAP;
+      const V3 d = Normalized({num[0], num[1], num[2]});
+      e.dir[0] = d.x; e.dir[1] = d.y; e.dir[2] = d.z;
+      e.tex_width = tw;
+      e.tex_height = th;
+      sd_.textures.push_back(std::move(tex));
+      sd_.envs.push_back(e);
+    } else if (tok == "larea") {
+      if (w.size() == 2) {
+        Activate(scope.emitters, scope.emitter, w[1], "AreaEmitter");
+        continue;
+      }
+      const auto n = Numbers(w, 3);
+      if (w.size() < 3 || (w[2] != "uniform" && w[2] != "parallel") || n.size() != 4) Fail("Error");
+      ttgpu_area_emitter a;
+      std::memset(&a, 0, sizeof(a));
+      a.kind = w[2] == "uniform" ? TTGPU_AREA_UNIFORM : TTGPU_AREA_PARALLEL;
+      const double rgb[3] = {n[0], n[1], n[2]};
+      RgbToSpectrum(rgb, a.spectrum);
+      for (float &f : a.spectrum) f = (float)(f * n[3]);
+      sd_.emitters.push_back(a);
+      scope.emitters[w[1]] = (int)sd_.emitters.size() - 1;
+      scope.emitter = (int)sd_.emitters.size() - 1;
+    } else if (tok == "l") {
+      if (num.size() != 3 && num.size() != 7) Fail("Error");
+      ttgpu_point_light p;
+      std::memset(&p, 0, sizeof(p));
+      p.pos[0] = num[0]; p.pos[1] = num[1]; p.pos[2] = num[2];
+      double rgb[3] = {1, 1, 1};
+      if (num.size() == 7) for (int k = 0; k < 3; ++k) rgb[k] = num[6] * num[3 + k];
+      RgbToSpectrum(rgb, p.spectrum);
+      sd_.points.push_back(p);
+    } else if (tok == "include") {
+      if (w.size() < 2) Fail("Unable to parse include line");
+      const std::string file = FullPath(w[1]);
+      Parser sub(sd_, ReadFile(file), file);
+      sub.Parse(scope);  // the included file works in this scope
+    } else if (tok == "m") {
+      if (w.size() < 2) Fail("Error");
+      Mesh(scope, w[1]);
+    } else {
+      Fail("Unknown directive");
+    }
+  }
+}
+
+}  // namespace
+
+void RgbToSpectrum(const double rgb[3], float out[TTGPU_NBINS]) {
+  const double r = rgb[0], g = rgb[1], b = rgb[2];
+  enum { W, R, G, B, Y, C, M };
+  auto lin = [&](double a, int ia, double bb, int ib, double c, int ic) {
+    for (int k = 0; k < TTGPU_NBINS; ++k)
+      out[k] = (float)(a * ttgpu_tables::kSmits[ia][k] + bb * ttgpu_tables::kSmits[ib][k] + c * ttgpu_tables::kSmits[ic][k]);
+  };
+  if (r <= g && r <= b) {
+    if (g <= b) lin(r, W, g - r, C, b - g, B);
+    else lin(r, W, b - r, C, g - b, G);
+  } else if (g <= r && g <= b) {
+    if (r <= b) lin(g, W, r - g, M, b - r, B);
+    else lin(g, W, b - g, M, r - b, R);
+  } else {
+    if (r <= g) lin(b, W, r - b, Y, g - r, G);
+    else lin(b, W, g - b, Y, r - g, R);
+  }
+}
+
+int SceneData::Material(int shader, int medium, int emitter) {
+  const std::vector<int> key{shader, medium, emitter};
+  auto it = material_index.find(key);
+  if (it != material_index.end()) return it->second;
+  ttgpu_material m{shader, medium, -1, emitter};
+  materials.push_back(m);
+  const int id = (int)materials.size() - 1;
+  material_index[key] = id;
+  return id;
+}
+
+void SceneData::Finish() {
+  std::memset(&view, 0, sizeof(view));
+  view.num_vertices = (int64_t)(vertices.size() / 3);
+  view.vertices = vertices.data();
+  view.normals = normals.empty() ? nullptr : normals.data();
+  view.num_triangles = (int64_t)tri_material.size();
+  view.indices = indices.data();
+  view.tri_material = tri_material.data();
+  view.num_spheres = (int64_t)sphere_material.size();
+  view.spheres = spheres.data();
+  view.sphere_material = sphere_material.data();
+  view.num_shaders = (int32_t)shaders.size();
+  view.shaders = shaders.data();
+  view.num_media = (int32_t)media.size();
+  view.media = media.data();
+  view.empty_space_medium = -1;
+  view.num_materials = (int32_t)materials.size();
+  view.materials = materials.data();
+  view.num_area_emitters = (int32_t)emitters.size();
+  view.area_emitters = emitters.data();
+  size_t t = 0;
+  for (auto &e : envs)
+    if (e.kind == TTGPU_ENV_MAP) e.texture_rgb = textures[t++]->data();
+  view.num_env_lights = (int32_t)envs.size();
+  view.env_lights = envs.data();
+  view.num_point_lights = (int32_t)points.size();
+  view.point_lights = points.data();
+  view.camera = camera;
+}
+
+std::unique_ptr<SceneData> LoadNff(const std::string &text, const std::string &path) {
+  auto sd = std::make_unique<SceneData>();
+  // Default shaders in the reference's order (src/scene.cxx:10-31): invisible, default, black.
+  ttgpu_shader inv;
+  std::memset(&inv, 0, sizeof(inv));
+  inv.kind = TTGPU_SHADER_INVISIBLE;
+  inv.ior_ratio = 1.f;
+  ttgpu_shader def = inv, black = inv;
+  def.kind = black.kind = TTGPU_SHADER_DIFFUSE;
+  const double gray[3] = {0.8, 0.8, 0.8};
+  RgbToSpectrum(gray, def.reflectance);
+  sd->shaders = {inv, def, black};
+  // Default symbols (src/parse_common.cxx:15-24).
+  Scope scope;
+  scope.media["default"] = -1;
+  scope.shaders["black"] = 2;
+  scope.shaders["invisible"] = 0;
+  scope.shaders["none"] = -1;
+  scope.shaders["default"] = 1;
+  scope.shader = 1;
+  scope.emitters["none"] = -1;
+  scope.materials["default"] = {1, -1, -1};
+  // A default camera looking down -z, as the reference's scenes set their own.
+  sd->camera.kind = TTGPU_CAMERA_PINHOLE;
+  sd->camera.dir[2] = -1;
+  sd->camera.up[1] = 1;
+  sd->camera.fov_deg = 45;
+  Parser p(*sd, text, path);
+  p.Parse(scope);
+  sd->Finish();
+  return sd;
+}
+
+std::unique_ptr<SceneData> LoadNffFile(const std::string &path) { return LoadNff(ReadFile(path), path); }
+
+}  // namespace scenefile
+}  // namespace ttgpu
+
+// ---- C ABI ----------------------------------------------------------------------------------
+struct ttgpu_scene_file {
+  std::unique_ptr<ttgpu::scenefile::SceneData> data;
+};
+
+namespace {
+template <class F>
+int SceneFileGuard(ttgpu_scene_file **out, F &&load) {
+  if (!out) {
+    ttgpu::SetCreateError("null argument");
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  try {
+    auto f = std::make_unique<ttgpu_scene_file>();
+    f->data = load();
+    *out = f.release();
+    return TTGPU_OK;
+  } catch (const std::invalid_argument &e) {
+    ttgpu::SetCreateError(e.what());
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  } catch (const std::bad_alloc &) {
+    ttgpu::SetCreateError("host allocation failed");
+    return TTGPU_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    ttgpu::SetCreateError(e.what());
+    return TTGPU_ERR_RUNTIME;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int ttgpu_scene_file_load(const char *path, ttgpu_scene_file **out) {
+  if (!path) {
+    ttgpu::SetCreateError("null path");
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  }
+  return SceneFileGuard(out, [&] { return ttgpu::scenefile::LoadNffFile(path); });
+}
+
+int ttgpu_scene_file_parse(const char *nff_text, const char *path_hint, ttgpu_scene_file **out) {
+  if (!nff_text) {
+    ttgpu::SetCreateError("null text");
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  }
+  return SceneFileGuard(out, [&] { return ttgpu::scenefile::LoadNff(nff_text, path_hint ? path_hint : ""); });
+}
+
+const ttgpu_scene *ttgpu_scene_file_scene(const ttgpu_scene_file *f) { return f ? &f->data->view : nullptr; }
+
+int ttgpu_scene_file_resolution(const ttgpu_scene_file *f, int32_t *width, int32_t *height) {
+  if (!f || !width || !height) return TTGPU_ERR_INVALID_ARGUMENT;
+  *width = f->data->width;
+  *height = f->data->height;
+  return TTGPU_OK;
+}
+
+void ttgpu_scene_file_free(ttgpu_scene_file *f) { delete f; }
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// Scene files: ToyTrace's NFF dialect (src/parsenff.cxx) with the meshes it references (OBJ and
+// COLLADA, read by assimp in the reference: src/parse_common.cxx:106-249).  The loader builds the
+// flat ttgpu_scene description and owns every array it points to.
+#pragma once
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ttgpu.h"
+
+namespace ttgpu {
+
+// Sets the message ttgpu_create_error() returns (calls that have no handle).
+void SetCreateError(const std::string &msg);
+
+namespace scenefile {
+
+struct SceneData {
+  std::vector<float> vertices, normals;  // 3 per vertex; every vertex has a normal
+  std::vector<uint32_t> indices;
+  std::vector<int32_t> tri_material;
+  std::vector<float> spheres;            // centre xyz, radius
+  std::vector<int32_t> sphere_material;
+  std::vector<ttgpu_shader> shaders;
+  std::vector<ttgpu_medium> media;
+  std::vector<ttgpu_material> materials;
+  std::vector<ttgpu_area_emitter> emitters;
+  std::vector<ttgpu_env_light> envs;
+  std::vector<std::unique_ptr<std::vector<float>>> textures;  // env-map texels (RGB)
+  std::vector<ttgpu_point_light> points;
+  ttgpu_camera camera{};
+  bool has_camera = false;
+  int width = 0, height = 0;             // the file's camera resolution
+  std::map<std::vector<int>, int> material_index;  // (shader, medium, emitter) -> materials
+  ttgpu_scene view{};                    // the description handed out (points into the vectors above)
+
+  int Material(int shader, int medium, int emitter);
+  void Finish();
+};
+
+// Parses NFF text; `path` names the file for messages and relative file names (may be empty:
+// then relative names resolve against the working directory).  Throws std::invalid_argument for
+// malformed input and std::runtime_error for files that cannot be read.
+std::unique_ptr<SceneData> LoadNff(const std::string &text, const std::string &path);
+std::unique_ptr<SceneData> LoadNffFile(const std::string &path);
+
+// RGB -> 36-bin spectrum (Smits 2000 with the reference's basis spectra, src/spectral.cxx:81-116).
+void RgbToSpectrum(const double rgb[3], float out[TTGPU_NBINS]);
+
+}  // namespace scenefile
+}  // namespace ttgpu

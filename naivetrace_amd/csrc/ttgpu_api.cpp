@@ -22,6 +22,7 @@
 #include "device_scene.hpp"
 #include "guiding.hpp"
 #include "kernels.hpp"
+#include "scene_file.hpp"
 #include "nn_host.hpp"
 
 #include "spectral_tables.inc"
@@ -788,6 +789,12 @@ void LearnFromPass(ttgpu_handle *h) {
   h->Timed("train", [&] { h->net->TrainOnRecords(h->q.rec_pos_w, h->q.rec_dir, h->q.rec_nrm, h->q.rec_wo, n, h->stream); });
   h->CollectTimers();
 }
+
+}  // namespace
+
+void ttgpu::SetCreateError(const std::string &msg) { g_create_error = msg; }
+
+namespace {
 
 int Fail(ttgpu_handle *h, int code, const std::string &msg) {
   if (h) h->err = msg;

@@ -1,0 +1,72 @@
+"""Scene files: ToyTrace's NFF format with OBJ / COLLADA meshes, read by libttgpu.so.
+
+Mirrors Scene::ParseNFF (src/parsenff.cxx:992-1006): ``SceneFile(path)`` parses the file (and
+the files it includes or names as meshes); ``SceneFile.from_text(text, path_hint)`` parses a
+string like Scene::ParseNFFString.  The object can be passed wherever a ``scene.Scene`` is
+accepted (``GpuRenderer``, the oracle): ``to_abi()`` returns the flat scene description, whose
+arrays the library owns as long as this object lives.  Malformed input raises ValueError (the
+reference's std::invalid_argument), unreadable files RuntimeError.
+"""
+import ctypes as C
+
+from . import abi
+
+
+class SceneFileError(Exception):
+    pass
+
+
+class SceneFile:
+    def __init__(self, path=None, _handle=None):
+        self._lib = abi.load_library()
+        if _handle is None:
+            h = C.c_void_p()
+            rc = self._lib.ttgpu_scene_file_load(str(path).encode(), C.byref(h))
+            self._raise(rc)
+            _handle = h
+        self._h = _handle
+        w, hh = C.c_int32(), C.c_int32()
+        self._lib.ttgpu_scene_file_resolution(self._h, C.byref(w), C.byref(hh))
+        self.width, self.height = w.value, hh.value
+
+    @classmethod
+    def from_text(cls, text, path_hint=None):
+        lib = abi.load_library()
+        h = C.c_void_p()
+        rc = lib.ttgpu_scene_file_parse(text.encode(), path_hint.encode() if path_hint else None, C.byref(h))
+        cls._raise_with(lib, rc)
+        return cls(_handle=h)
+
+    @staticmethod
+    def _raise_with(lib, rc):
+        if rc == abi.OK:
+            return
+        msg = lib.ttgpu_create_error().decode()
+        if rc == abi.ERR_INVALID_ARGUMENT:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+    def _raise(self, rc):
+        self._raise_with(self._lib, rc)
+
+    def to_abi(self):
+        """The scene description (a copy of the struct; its arrays belong to this object)."""
+        return self._lib.ttgpu_scene_file_scene(self._h).contents
+
+    @property
+    def num_triangles(self):
+        return int(self.to_abi().num_triangles)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ttgpu_scene_file_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -16,7 +16,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle_lib  # noqa: E402
-from naivetrace_amd import abi, configs  # noqa: E402
+from naivetrace_amd import abi, configs, scene_file  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -34,6 +34,12 @@ FIXTURES = {
     # glass, mirror and Lambert materials on a heightfield and icospheres
     "terrain32": (lambda w, h: configs.terrain_instances(w, h, cells=32, n_spheres=24, freq=3, sky=(256, 128)), 32, 32, 8192, 8,
                   abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+    # scenes of the reference read from NFF (tests/golden/scenes): COLLADA Cornell box, glass
+    # shells, area light under a YAML transform, dome light; media nested in media
+    "nff_cornell_spheres32": (lambda w, h: scene_file.SceneFile(os.path.join(HERE, "scenes", "cornelbox_translucentsphere.nff")),
+                              32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+    "nff_medium_in_medium32": (lambda w, h: scene_file.SceneFile(os.path.join(HERE, "scenes", "medium_in_medium.nff")),
+                               32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
 }
 
 

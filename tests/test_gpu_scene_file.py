@@ -1,0 +1,34 @@
+"""Scenes read from NFF files rendered on the GPU: the closed-form plane under a dome, and the
+reference's own test scenes (tests/golden/scenes) against the oracle's renders of the same files."""
+import os
+
+import numpy as np
+import pytest
+
+from naivetrace_amd import abi
+from naivetrace_amd.renderer import GpuRenderer
+from naivetrace_amd.scene_file import SceneFile
+from test_gpu_parity import render_parity
+from test_scene_file import PLANE_UNDER_DOME, plane_under_dome_expected
+
+pytestmark = pytest.mark.gpu
+SCENES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
+
+
+def test_gpu_nff_plane_under_dome_closed_form(gpu_lib):
+    sf = SceneFile.from_text(PLANE_UNDER_DOME)
+    p = abi.default_params(sf.width, sf.height, max_depth=3, seed=5)
+    with GpuRenderer(sf, p) as r:
+        r.render_spp(256)
+        rgb, _, n = r.framebuffer()
+    mean = (rgb / n[..., None]).mean(axis=(0, 1))
+    print("plane under dome", mean, "expected", plane_under_dome_expected(), flush=True)
+    np.testing.assert_allclose(mean, plane_under_dome_expected(), rtol=3e-3)
+
+
+def test_gpu_nff_cornell_glass_spheres_parity(gpu_lib):
+    render_parity("nff_cornell_spheres32", builder=lambda w, h: SceneFile(os.path.join(SCENES, "cornelbox_translucentsphere.nff")))
+
+
+def test_gpu_nff_medium_in_medium_parity(gpu_lib):
+    render_parity("nff_medium_in_medium32", builder=lambda w, h: SceneFile(os.path.join(SCENES, "medium_in_medium.nff")))

@@ -392,6 +392,18 @@ int ttgpu_nn_gradient(const ttgpu_nn_config *cfg, const float *params, int64_t n
 int ttgpu_nn_train(const ttgpu_nn_config *cfg, float *params, int64_t n, const float *pos, const float *nrm, const float *wo,
                    const float *wi, const float *weight, int32_t passes, float *loss_per_pass);
 
+/* ---- HDR image files ------------------------------------------------------
+ * Linear counterparts of GenerateImage's 8-bit sRGB output (src/renderbuffer.hxx:99-125):
+ * PFM (little-endian float RGB, bottom row first) or OpenEXR (single-part scanline file,
+ * uncompressed 32-bit float B, G, R channels).  No counterpart in the reference, which writes
+ * PNG only. */
+enum ttgpu_image_format { TTGPU_IMAGE_PFM = 0, TTGPU_IMAGE_EXR = 1 };
+/* `rgb`: width x height x 3 floats, top row first. */
+int ttgpu_write_rgb_image(const float *rgb, int32_t width, int32_t height, const char *path, int32_t format);
+/* The framebuffer's mean radiance per pixel (linear RGB sums / samples per pixel; pixels
+ * without samples are 0). */
+int ttgpu_write_image(ttgpu_handle *h, const char *path, int32_t format);
+
 /* ---- scene files ----------------------------------------------------------
  * Scene::ParseNFF / ParseNFFString (src/parsenff.cxx:992-1028): ToyTrace's NFF dialect
  * (scopes, transforms, yaml{ }yaml blocks, shaders, media, lights, `include`) with the OBJ

@@ -28,6 +28,7 @@ SAMPLE_BOTH, SAMPLE_BSDF, SAMPLE_LIGHTS = range(3)
 MIS_REFERENCE, MIS_CONSISTENT = range(2)
 GUIDE_NONE, GUIDE_VMF_TREE, GUIDE_LEARNED = range(3)
 OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_DEVICE, ERR_INTERRUPTED, ERR_OUT_OF_MEMORY = range(6)
+IMAGE_PFM, IMAGE_EXR = range(2)
 
 Spectrum = C.c_float * NBINS
 
@@ -137,7 +138,7 @@ EXPORTS = [
     "ttgpu_guide_build_kdtree", "ttgpu_guide_sample_vmf", "ttgpu_guide_pdf_vmf", "ttgpu_nn_param_count",
     "ttgpu_nn_init_params", "ttgpu_nn_forward", "ttgpu_nn_gradient", "ttgpu_nn_train",
     "ttgpu_scene_file_load", "ttgpu_scene_file_parse", "ttgpu_scene_file_scene", "ttgpu_scene_file_resolution",
-    "ttgpu_scene_file_free",
+    "ttgpu_scene_file_free", "ttgpu_write_rgb_image", "ttgpu_write_image",
 ]
 
 _lib = None
@@ -189,6 +190,8 @@ def _declare(lib):
         "ttgpu_scene_file_scene": (P(SceneDesc), [C.c_void_p]),
         "ttgpu_scene_file_resolution": (C.c_int, [C.c_void_p, i32p, i32p]),
         "ttgpu_scene_file_free": (None, [C.c_void_p]),
+        "ttgpu_write_rgb_image": (C.c_int, [f32p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]),
+        "ttgpu_write_image": (C.c_int, [h, C.c_char_p, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

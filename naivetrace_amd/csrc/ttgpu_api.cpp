@@ -1100,6 +1100,23 @@ int ttgpu_run(ttgpu_handle *h, int32_t max_passes) {
   })
 }
 
+int ttgpu_write_image(ttgpu_handle *h, const char *path, int32_t format) {
+  if (!h || !path) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(h, {
+    const size_t n = (size_t)h->npix;
+    std::vector<double> rgb(3 * n);
+    Check(hipMemcpy(rgb.data(), h->fb, rgb.size() * sizeof(double), hipMemcpyDeviceToHost), "read framebuffer");
+    std::vector<float> img(3 * n);
+    for (size_t p = 0; p < n; ++p) {
+      const double s = h->pixel_spp[p] ? 1.0 / (double)h->pixel_spp[p] : 0.0;
+      for (int c = 0; c < 3; ++c) img[3 * p + c] = (float)(rgb[3 * p + c] * s);
+    }
+    const int rc = ttgpu_write_rgb_image(img.data(), h->params.width, h->params.height, path, format);
+    if (rc != TTGPU_OK) return Fail(h, rc, std::string("could not write ") + path);
+    return TTGPU_OK;
+  })
+}
+
 int ttgpu_reset_framebuffer(ttgpu_handle *h) {
   if (!h) return TTGPU_ERR_INVALID_ARGUMENT;
   TTGPU_GUARD(h, {

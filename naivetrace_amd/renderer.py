@@ -98,6 +98,10 @@ class GpuRenderer:
         rgb, _, spp = self.framebuffer()
         return rgb / np.maximum(spp, 1)[..., None]
 
+    def write_image(self, path, fmt=abi.IMAGE_EXR):
+        """Mean radiance per pixel (linear RGB) as OpenEXR (uncompressed float) or PFM."""
+        self._check(self._lib.ttgpu_write_image(self._h, str(path).encode(), fmt))
+
     def generate_image(self):
         img = np.zeros((self.params.height, self.params.width, 3), dtype=np.uint8)
         self._check(self._lib.ttgpu_generate_image(self._h, img.ctypes.data_as(C.POINTER(C.c_uint8))))
@@ -336,3 +340,14 @@ def RenderAlgorithmFactory(scene, params, name=None):
     else:
         raise ValueError(f"unknown or unsupported algorithm {name!r}; this library provides 'pt2' and 'ptg'")
     return RenderingAlgo(scene, params)
+
+
+def write_rgb_image(rgb, path, fmt=abi.IMAGE_EXR):
+    """Writes a (h, w, 3) linear RGB array (top row first) as OpenEXR or PFM."""
+    img = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w, c = img.shape
+    if c != 3:
+        raise ValueError("expected an (h, w, 3) array")
+    rc = abi.load_library().ttgpu_write_rgb_image(img.ctypes.data_as(C.POINTER(C.c_float)), w, h, str(path).encode(), fmt)
+    if rc != 0:
+        raise TTGpuError(rc, f"could not write {path}")

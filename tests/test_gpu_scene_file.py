@@ -32,3 +32,18 @@ def test_gpu_nff_cornell_glass_spheres_parity(gpu_lib):
 
 def test_gpu_nff_medium_in_medium_parity(gpu_lib):
     render_parity("nff_medium_in_medium32", builder=lambda w, h: SceneFile(os.path.join(SCENES, "medium_in_medium.nff")))
+
+
+def test_gpu_write_hdr_images(gpu_lib, tmp_path):
+    from test_image_io import read_exr, read_pfm
+    sf = SceneFile.from_text(PLANE_UNDER_DOME)
+    p = abi.default_params(sf.width, sf.height, max_depth=3, seed=6)
+    with GpuRenderer(sf, p) as r:
+        r.render_spp(16)
+        rgb, _, n = r.framebuffer()
+        r.write_image(tmp_path / "a.exr", abi.IMAGE_EXR)
+        r.write_image(tmp_path / "a.pfm", abi.IMAGE_PFM)
+    mean = (rgb / n[..., None]).astype(np.float32)
+    ch = read_exr(tmp_path / "a.exr")
+    np.testing.assert_array_equal(np.stack([ch["R"], ch["G"], ch["B"]], axis=-1), mean)
+    np.testing.assert_array_equal(read_pfm(tmp_path / "a.pfm"), mean)

@@ -77,7 +77,8 @@ enum ttgpu_phase_kind {
 enum ttgpu_medium_kind {
   TTGPU_MEDIUM_VACUUM = 0,
   TTGPU_MEDIUM_HOMOGENEOUS = 1,
-  TTGPU_MEDIUM_ATMOSPHERE = 2       /* exponential Rayleigh + Mie shells, FP64 geometry */
+  TTGPU_MEDIUM_ATMOSPHERE = 2,      /* exponential Rayleigh + Mie shells, FP64 geometry */
+  TTGPU_MEDIUM_ATMOSPHERE_TABULATED = 3 /* coefficient tables over altitude (TabulatedConstituents) */
 };
 
 typedef struct ttgpu_medium {
@@ -100,6 +101,16 @@ typedef struct ttgpu_medium {
   float rayleigh_sigma_s[TTGPU_NBINS];
   float mie_sigma_s[TTGPU_NBINS];
   float mie_sigma_a[TTGPU_NBINS];
+  /* Tabulated atmosphere (kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED), TabulatedConstituents
+   * (src/atmosphere.cxx:219-353): spectra at altitudes table_h0 + i * table_dh, i < table_size,
+   * each table_size x 36 values per scene unit; coefficients are interpolated linearly in
+   * altitude, zero above the table, the lowest entry below it.  Geometry (planet_center,
+   * planet_radius) and mie_g (HG of the aerosols) as for the exponential atmosphere. */
+  int32_t table_size;
+  double table_h0, table_dh;
+  const float *table_sigma_t;            /* extinction */
+  const float *table_sigma_s_molecules;  /* Rayleigh scattering */
+  const float *table_sigma_s_aerosols;   /* HG scattering */
 } ttgpu_medium;
 
 /* ---- materials (src/scene.hxx:46-72) ----------------------------------- */

@@ -19,7 +19,7 @@ LIB_PATH = os.environ.get("TTGPU_LIBRARY") or os.path.join(_HERE, "libttgpu.so")
 SHADER_DIFFUSE, SHADER_MIRROR, SHADER_DIELECTRIC, SHADER_BECKMANN_METAL, SHADER_GGX_METAL, \
     SHADER_INVISIBLE, SHADER_GLOSSY_DIELECTRIC, SHADER_DENSE_DIELECTRIC = range(8)
 PHASE_UNIFORM, PHASE_HENYEY_GREENSTEIN, PHASE_RAYLEIGH = range(3)
-MEDIUM_VACUUM, MEDIUM_HOMOGENEOUS, MEDIUM_ATMOSPHERE = range(3)
+MEDIUM_VACUUM, MEDIUM_HOMOGENEOUS, MEDIUM_ATMOSPHERE, MEDIUM_ATMOSPHERE_TABULATED = range(4)
 AREA_UNIFORM, AREA_PARALLEL = range(2)
 ENV_SUN, ENV_DOME, ENV_DIRECTIONAL, ENV_MAP = range(4)
 CAMERA_PINHOLE, CAMERA_FISHEYE = range(2)
@@ -45,7 +45,10 @@ class Medium(C.Structure):
                 ("planet_center", C.c_double * 3), ("planet_radius", C.c_double),
                 ("atmosphere_height", C.c_double), ("rayleigh_scale_height", C.c_double),
                 ("mie_scale_height", C.c_double), ("mie_g", C.c_float),
-                ("rayleigh_sigma_s", Spectrum), ("mie_sigma_s", Spectrum), ("mie_sigma_a", Spectrum)]
+                ("rayleigh_sigma_s", Spectrum), ("mie_sigma_s", Spectrum), ("mie_sigma_a", Spectrum),
+                ("table_size", C.c_int32), ("table_h0", C.c_double), ("table_dh", C.c_double),
+                ("table_sigma_t", C.POINTER(C.c_float)), ("table_sigma_s_molecules", C.POINTER(C.c_float)),
+                ("table_sigma_s_aerosols", C.POINTER(C.c_float))]
 
 
 class Material(C.Structure):

@@ -259,6 +259,28 @@ def absorber_slab(width=16, height=16, seed=3, thickness=0.5, dome=1.0, fov_deg=
     return s
 
 
+def tabulated_absorber_shell(width=8, height=8, seed=6, radius=10.0, fov_deg=20.0):
+    """A tabulated atmosphere (TabulatedConstituents) with altitude-constant, purely absorbing
+    chromatic tables inside an invisible sphere of `radius` around the camera, which looks up at a
+    uniform dome.  Every ray leaves through the sphere after `radius`, so each pixel's expected
+    radiance is sum_b M[:, b] exp(-sigma_a[b] * radius)."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    sig_a = rng.uniform(0.02, 0.2, NBINS)
+    n = 21
+    st = np.tile(sig_a, (n, 1))
+    zero = np.zeros((n, NBINS))
+    # Planet radius 5: altitudes inside the sphere span [-5, 5], within the table's [-10, 10].
+    med = s.add_tabulated_atmosphere([0.0, 0.0, 0.0], 5.0, -10.0, 1.0, st, zero, zero)
+    s.add_sphere([0.0, 0.0, 0.0], radius, s.add_material(medium=med))
+    s.add_dome([0.0, 1.0, 0.0], flat(1.0))
+    s.set_camera([0.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], fov_deg)
+    s.width, s.height = width, height
+    s.absorber_sigma_a = sig_a
+    s.shell_radius = radius
+    return s
+
+
 def point_lit_plane_behind_absorber(width=16, height=16, seed=4, thickness=0.4, power=50.0, albedo=0.5):
     """Lambertian plane z = 0 lit by a point light at (0, 0, 2) through a purely absorbing slab
     0.8 < z < 0.8 + thickness (an invisible volume boundary).  The camera sits below the slab at

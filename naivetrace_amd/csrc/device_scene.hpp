@@ -63,7 +63,14 @@ struct DevMedium {
   double planet_radius, rayleigh_inv_h, mie_inv_h, lower_cutoff;
   float mie_g;
   float rayleigh_sigma_s[kNBins], mie_sigma_s[kNBins], mie_sigma_a[kNBins];
+  // tabulated atmosphere: rows of 3 x 36 (extinction, molecular and aerosol scattering) per
+  // altitude table_h0 + i / table_inv_dh, and the extinction majorant per altitude
+  int table_n;
+  double table_h0, table_inv_dh;
+  const float *table;
+  const float *table_majorant;
 };
+
 
 struct DevMaterial {
   int shader, medium, outer_medium, emitter;

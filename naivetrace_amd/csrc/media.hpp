@@ -124,12 +124,47 @@ TT_HD D3 d3(float3 a) { return {(double)a.x, (double)a.y, (double)a.z}; }
 TT_HD D3 d3_along(D3 o, float3 d, double t) { return {o.x + (double)d.x * t, o.y + (double)d.y * t, o.z + (double)d.z * t}; }
 TT_HD float3 f3(D3 a) { return mk((float)a.x, (float)a.y, (float)a.z); }
 
+TT_HD bool is_atmosphere(const DevMedium &m) { return m.kind == TTGPU_MEDIUM_ATMOSPHERE || m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED; }
 TT_HD double atmo_altitude(const DevMedium &m, D3 p) {
   const double x = p.x - m.planet_center[0], y = p.y - m.planet_center[1], z = p.z - m.planet_center[2];
   return sqrt(x * x + y * y + z * z) - m.planet_radius;
 }
-// ExponentialConstituentDistribution::ComputeCollisionCoefficients (atmosphere.cxx:130-200).
+// Row index and interpolation weight of a tabulated atmosphere at an altitude, as
+// TabulatedConstituents does (atmosphere.cxx:265-330): the index truncates towards zero, so
+// altitudes less than one step below the table extrapolate the first interval; further below,
+// the first row; at or above the last altitude, nothing.  Returns false above the table.
+TT_HD bool atmo_table_index(const DevMedium &m, double alt, int &i0, int &i1, float &f) {
+  const double real = (alt - m.table_h0) * m.table_inv_dh;
+  if (real >= (double)(m.table_n - 1)) return false;
+  const int idx = (int)real;
+  if (idx >= 0) {
+    i0 = idx;
+    i1 = idx + 1;
+    f = (float)(real - idx);
+  } else {
+    i0 = i1 = 0;
+    f = 0.f;
+  }
+  return true;
+}
+// Collision coefficients per constituent: ExponentialConstituentDistribution (atmosphere.cxx:
+// 130-200), or TabulatedConstituents' linear interpolation (absorption = extinction - scattering).
 TT_HD void atmo_coeffs(const DevMedium &m, double alt, uint32_t bins, float4 &ss_ray, float4 &ss_mie, float4 &sa) {
+  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) {
+    int i0, i1;
+    float f;
+    if (!atmo_table_index(m, alt, i0, i1, f)) {
+      ss_ray = ss_mie = sa = f4(0.f);
+      return;
+    }
+    const float *r0 = m.table + (size_t)i0 * 3 * kNBins, *r1 = m.table + (size_t)i1 * 3 * kNBins;
+    auto lerp = [&](int block) { return take(r0 + block * kNBins, bins) * (1.f - f) + take(r1 + block * kNBins, bins) * f; };
+    const float4 st = lerp(0);
+    ss_ray = lerp(1);
+    ss_mie = lerp(2);
+    sa = st - (ss_ray + ss_mie);
+    return;
+  }
   alt = alt > m.lower_cutoff ? alt : m.lower_cutoff;
   const float rr = expf((float)(-alt * m.rayleigh_inv_h)), rm = expf((float)(-alt * m.mie_inv_h));
   ss_ray = take(m.rayleigh_sigma_s, bins) * rr;
@@ -138,7 +173,7 @@ TT_HD void atmo_coeffs(const DevMedium &m, double alt, uint32_t bins, float4 &ss
 }
 TT_HD void medium_coeffs(const DeviceScene &sc, int mid, D3 pos, uint32_t bins, float4 &ss, float4 &st) {
   const DevMedium &m = sc.media[mid];
-  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+  if (is_atmosphere(m)) {
     float4 a, b, c;
     atmo_coeffs(m, atmo_altitude(m, pos), bins, a, b, c);
     ss = a + b;
@@ -152,12 +187,20 @@ TT_HD void medium_coeffs(const DeviceScene &sc, int mid, D3 pos, uint32_t bins, 
 // (SphereGeometry::ComputeLowestPointAlong, atmosphere.hxx:95-123).
 TT_HD float4 medium_majorant(const DeviceScene &sc, int mid, D3 org, float3 dir, double len, uint32_t bins) {
   const DevMedium &m = sc.media[mid];
-  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+  if (is_atmosphere(m)) {
     const double ox = org.x - m.planet_center[0], oy = org.y - m.planet_center[1], oz = org.z - m.planet_center[2];
     double tl = -(ox * dir.x + oy * dir.y + oz * dir.z);
     tl = tl > len ? len : (tl < 0. ? 0. : tl);
+    const double alt = atmo_altitude(m, d3_along(org, dir, tl));
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) {
+      // ComputeSigmaTMajorante (atmosphere.cxx:333-352) at max(lowest table altitude, lowest point).
+      int i0, i1;
+      float f;
+      if (!atmo_table_index(m, alt > m.table_h0 ? alt : m.table_h0, i0, i1, f)) return f4(0.f);
+      return f4(m.table_majorant[i0] * (1.f - f) + m.table_majorant[i1] * f);
+    }
     float4 a, b, c;
-    atmo_coeffs(m, atmo_altitude(m, d3_along(org, dir, tl)), bins, a, b, c);
+    atmo_coeffs(m, alt, bins, a, b, c);
     return a + b + c;
   }
   return take(m.sigma_t, bins);
@@ -189,7 +232,7 @@ TT_HD Scatter medium_phase_sample(const DeviceScene &sc, int mid, float3 wi, D3 
   const DevMedium &m = sc.media[mid];
   Scatter s;
   s.delta = false;
-  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+  if (is_atmosphere(m)) {
     float4 w[2];
     float sel[2];
     atmo_phase_weights(m, pos, bins, w, sel);
@@ -212,7 +255,7 @@ TT_HD Scatter medium_phase_sample(const DeviceScene &sc, int mid, float3 wi, D3 
 }
 TT_HD float4 medium_phase_eval(const DeviceScene &sc, int mid, float3 wi, D3 pos, uint32_t bins, float3 wo, float *pdf) {
   const DevMedium &m = sc.media[mid];
-  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+  if (is_atmosphere(m)) {
     float4 w[2];
     float sel[2];
     atmo_phase_weights(m, pos, bins, w, sel);
@@ -223,6 +266,14 @@ TT_HD float4 medium_phase_eval(const DeviceScene &sc, int mid, float3 wi, D3 pos
   const float v = phase_eval(m.phase_kind, m.phase_g, wi, wo);
   *pdf = v;
   return f4(v);
+}
+
+// Null-collision coefficient majorant - extinction, never negative: the majorant bounds every
+// wavelength's extinction, but in FP32 an interpolated extinction can exceed the separately
+// interpolated majorant by a rounding step, and a negative null weight would mix -inf and +inf
+// entries into the path-space weights.
+TT_HD float4 null_coeff(float4 maj, float4 st) {
+  return make_float4(fmaxf(maj.x - st.x, 0.f), fmaxf(maj.y - st.y, 0.f), fmaxf(maj.z - st.z, 0.f), fmaxf(maj.w - st.w, 0.f));
 }
 
 TT_HD float4 exp_neg(double seg, float4 maj) {
@@ -251,7 +302,7 @@ TT_HD bool track_medium(const DeviceScene &sc, int mid, D3 org, float3 dir, doub
     }
     float4 ss, st;
     medium_coeffs(sc, mid, d3_along(org, dir, t), bins, ss, st);
-    const float4 sn = maj - st;
+    const float4 sn = null_coeff(maj, st);
     const float prob_n = (float)((double)sn.x * mfp);
     if (rng.uniform() < prob_n) {
       nm_acc(w_track, tr * sn, tr * sn);
@@ -288,7 +339,7 @@ TT_HD void transmit_medium(const DeviceScene &sc, int mid, D3 org, float3 dir, d
     }
     float4 ss, st;
     medium_coeffs(sc, mid, d3_along(org, dir, t), bins, ss, st);
-    const float4 sn = maj - st;
+    const float4 sn = null_coeff(maj, st);
     nm_acc(w_track, tr * sn, tr * sn);
     nm_acc(w_nulls, tr * sn, tr * maj);
     bool all_large = true;

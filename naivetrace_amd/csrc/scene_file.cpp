@@ -16,9 +16,8 @@
 //  * sun / directional / dome vectors are not transformed; lsun's default opening angle 0.26 deg;
 //  * COLLADA: node transforms, the asset's unit (meters per unit) and up axis are applied to the
 //    vertices, materials are named by their <material name>, like assimp's importer.
-// Not supported (the GPU path has no textures, and no tabulated atmosphere): diffusetexture,
-// exponenttexture, alpha_texture, tabulatedatmosphere; lenv reads PFM images only.  These raise
-// std::invalid_argument naming the directive.
+// Not supported (the GPU path has no textures): diffusetexture, exponenttexture, alpha_texture;
+// lenv reads PFM images only.  These raise std::invalid_argument naming the directive.
 #include "scene_file.hpp"
 
 #include <algorithm>
@@ -293,6 +292,97 @@ std::unique_ptr<Xml> ParseXml(const std::string &s) {
 }
 
 std::vector<double> XmlNumbers(const Xml *x) { return x ? Numbers(Split(x->text), 0) : std::vector<double>{}; }
+
+// ---- minimal JSON (objects, arrays, numbers, strings, literals) ---------------------------------
+struct Json {
+  enum Kind { kNull, kNumber, kString, kArray, kObject, kBool } kind = kNull;
+  double num = 0;
+  std::string str;
+  std::vector<Json> arr;
+  std::map<std::string, Json> obj;
+};
+
+class JsonReader {
+ public:
+  explicit JsonReader(const std::string &s) : s_(s) {}
+  Json Parse() {
+    Json v = Value();
+    Ws();
+    if (i_ != s_.size()) Err("trailing characters");
+    return v;
+  }
+
+ private:
+  const std::string &s_;
+  size_t i_ = 0;
+  [[noreturn]] void Err(const char *m) const { throw std::invalid_argument(std::string("json: ") + m + " at byte " + std::to_string(i_)); }
+  void Ws() {
+    while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
+  }
+  Json Value() {
+    Ws();
+    if (i_ >= s_.size()) Err("unexpected end");
+    const char c = s_[i_];
+    Json v;
+    if (c == '{') {
+      v.kind = Json::kObject;
+      ++i_;
+      Ws();
+      if (i_ < s_.size() && s_[i_] == '}') { ++i_; return v; }
+      for (;;) {
+        Ws();
+        Json k = Value();
+        if (k.kind != Json::kString) Err("object key must be a string");
+        Ws();
+        if (i_ >= s_.size() || s_[i_] != ':') Err("expected ':'");
+        ++i_;
+        v.obj[k.str] = Value();
+        Ws();
+        if (i_ < s_.size() && s_[i_] == ',') { ++i_; continue; }
+        if (i_ < s_.size() && s_[i_] == '}') { ++i_; return v; }
+        Err("expected ',' or '}'");
+      }
+    }
+    if (c == '[') {
+      v.kind = Json::kArray;
+      ++i_;
+      Ws();
+      if (i_ < s_.size() && s_[i_] == ']') { ++i_; return v; }
+      for (;;) {
+        v.arr.push_back(Value());
+        Ws();
+        if (i_ < s_.size() && s_[i_] == ',') { ++i_; continue; }
+        if (i_ < s_.size() && s_[i_] == ']') { ++i_; return v; }
+        Err("expected ',' or ']'");
+      }
+    }
+    if (c == '"') {
+      v.kind = Json::kString;
+      ++i_;
+      while (i_ < s_.size() && s_[i_] != '"') {
+        if (s_[i_] == '\\' && i_ + 1 < s_.size()) ++i_;
+        v.str += s_[i_++];
+      }
+      if (i_ >= s_.size()) Err("unterminated string");
+      ++i_;
+      return v;
+    }
+    for (const char *lit : {"true", "false", "null"}) {
+      if (s_.compare(i_, std::strlen(lit), lit) == 0) {
+        i_ += std::strlen(lit);
+        v.kind = lit[0] == 'n' ? Json::kNull : Json::kBool;
+        v.num = lit[0] == 't';
+        return v;
+      }
+    }
+    char *end = nullptr;
+    v.num = std::strtod(s_.c_str() + i_, &end);
+    if (end == s_.c_str() + i_) Err("unexpected character");
+    i_ = (size_t)(end - s_.c_str());
+    v.kind = Json::kNumber;
+    return v;
+  }
+};
 
 // ---- the parser ----------------------------------------------------------------------------
 class Parser {
@@ -878,7 +968,7 @@ void Parser::Parse(Scope &scope) {
       m.atmosphere_height = 0;
       m.rayleigh_scale_height = 8.0;
       m.mie_scale_height = 1.2;
-      m.mie_g = 0.76f;
+      m.mie_g = 0.7f;  // AtmosphereTemplate's Henyey-Greenstein (src/atmosphere.cxx:26-33)
       const double Na = 6.022e23, rho = 1.225, mol = 29e-3, ior = 1.00028;
       const double N = rho * Na / mol, ior_term = (ior * ior - 1) * (ior * ior - 1);
       const double pref = 8. * kPi * kPi * kPi / 3. * ior_term / N * 1.e27 * 1.e12;
@@ -894,7 +984,48 @@ void Parser::Parse(Scope &scope) {
         Activate(scope.media, scope.medium, w[1], "Medium");
         continue;
       }
-      Fail("tabulatedatmosphere: tabulated atmospheres are not supported by this renderer");
+      const auto n = Numbers(w, 2);
+      if (n.size() != 4 || w.size() != 7) Fail("Error");
+      // Atmosphere::MakeTabulated (src/atmosphere.cxx:219-263, 361-364): the JSON holds H0, deltaH
+      // and per-altitude 36-bin arrays sigma_t, sigma_s_molecules, sigma_s_aerosoles.
+      const Json d = JsonReader(ReadFile(FullPath(w[6]))).Parse();
+      auto num_of = [&](const char *key) {
+        auto it = d.obj.find(key);
+        if (it == d.obj.end() || it->second.kind != Json::kNumber) Fail(std::string("tabulatedatmosphere: missing ") + key);
+        return it->second.num;
+      };
+      auto spectra = [&](const char *key) {
+        auto it = d.obj.find(key);
+        if (it == d.obj.end() || it->second.kind != Json::kArray) Fail(std::string("tabulatedatmosphere: missing ") + key);
+        auto t = std::make_unique<std::vector<float>>();
+        for (const Json &row : it->second.arr) {
+          if (row.kind != Json::kArray || row.arr.size() != (size_t)TTGPU_NBINS)
+            Fail("The number of wavelengths in the array does not match the requirement.");
+          for (const Json &x : row.arr) t->push_back((float)x.num);
+        }
+        return t;
+      };
+      auto st = spectra("sigma_t"), mol = spectra("sigma_s_molecules"), aer = spectra("sigma_s_aerosoles");
+      if (st->size() != mol->size() || st->size() != aer->size())
+        Fail("The number of altitude support points is inconsistent among arrays.");
+      ttgpu_medium m;
+      std::memset(&m, 0, sizeof(m));
+      m.kind = TTGPU_MEDIUM_ATMOSPHERE_TABULATED;
+      m.priority = (int)scope.media.size();
+      m.phase_kind = TTGPU_PHASE_RAYLEIGH;
+      m.planet_center[0] = n[0]; m.planet_center[1] = n[1]; m.planet_center[2] = n[2];
+      m.planet_radius = n[3];
+      m.mie_g = 0.7f;  // AtmosphereTemplate's Henyey-Greenstein (src/atmosphere.cxx:26-33)
+      m.table_size = (int32_t)(st->size() / TTGPU_NBINS);
+      m.table_h0 = num_of("H0");
+      m.table_dh = num_of("deltaH");
+      m.table_sigma_t = st->data();
+      m.table_sigma_s_molecules = mol->data();
+      m.table_sigma_s_aerosols = aer->data();
+      sd_.tables.push_back(std::move(st));
+      sd_.tables.push_back(std::move(mol));
+      sd_.tables.push_back(std::move(aer));
+      AddMedium(w[1], scope, m);
     } else if (tok == "lsun") {
       if (num.size() != 4 && num.size() != 5) Fail("Error");
       const V3 d = Normalized({num[0], num[1], num[2]});

@@ -28,6 +28,7 @@ struct SceneData {
   std::vector<ttgpu_area_emitter> emitters;
   std::vector<ttgpu_env_light> envs;
   std::vector<std::unique_ptr<std::vector<float>>> textures;  // env-map texels (RGB)
+  std::vector<std::unique_ptr<std::vector<float>>> tables;    // tabulated atmospheres' coefficient tables
   std::vector<ttgpu_point_light> points;
   ttgpu_camera camera{};
   bool has_camera = false;

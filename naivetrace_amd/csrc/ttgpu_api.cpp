@@ -227,7 +227,12 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   }
   for (int i = 0; i < s.num_media; ++i) {
     const auto &m = s.media[i];
-    if (m.kind < TTGPU_MEDIUM_VACUUM || m.kind > TTGPU_MEDIUM_ATMOSPHERE) bad("unknown medium kind");
+    if (m.kind < TTGPU_MEDIUM_VACUUM || m.kind > TTGPU_MEDIUM_ATMOSPHERE_TABULATED) bad("unknown medium kind");
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) {
+      if (m.table_size < 2 || !(m.table_dh > 0.) || !m.table_sigma_t || !m.table_sigma_s_molecules || !m.table_sigma_s_aerosols ||
+          !(m.planet_radius > 0.))
+        bad("tabulated atmosphere needs at least two altitudes, a positive step, its three tables and a planet radius");
+    }
     if (m.kind == TTGPU_MEDIUM_HOMOGENEOUS && (m.phase_kind < TTGPU_PHASE_UNIFORM || m.phase_kind > TTGPU_PHASE_RAYLEIGH))
       bad("unknown phase function kind");
     if (m.kind == TTGPU_MEDIUM_ATMOSPHERE && !(m.rayleigh_scale_height > 0. && m.mie_scale_height > 0. && m.planet_radius > 0.))
@@ -619,6 +624,35 @@ void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
       std::memcpy(d.rayleigh_sigma_s, a.rayleigh_sigma_s, sizeof(d.rayleigh_sigma_s));
       std::memcpy(d.mie_sigma_s, a.mie_sigma_s, sizeof(d.mie_sigma_s));
       std::memcpy(d.mie_sigma_a, a.mie_sigma_a, sizeof(d.mie_sigma_a));
+    } else if (a.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) {
+      // TabulatedConstituents (src/atmosphere.cxx:219-263): the majorant per altitude is the
+      // largest extinction over the wavelengths, then the running maximum from the top down so
+      // that the lowest point of a segment bounds the whole segment.
+      for (int k = 0; k < 3; ++k) d.planet_center[k] = a.planet_center[k];
+      d.planet_radius = a.planet_radius;
+      d.mie_g = a.mie_g;
+      const int n = a.table_size;
+      std::vector<float> rows((size_t)n * 3 * kNBins), maj(n);
+      for (int r = 0; r < n; ++r) {
+        float mx = 0.f;
+        for (int b = 0; b < kNBins; ++b) {
+          rows[(size_t)r * 3 * kNBins + b] = a.table_sigma_t[(size_t)r * kNBins + b];
+          rows[(size_t)r * 3 * kNBins + kNBins + b] = a.table_sigma_s_molecules[(size_t)r * kNBins + b];
+          rows[(size_t)r * 3 * kNBins + 2 * kNBins + b] = a.table_sigma_s_aerosols[(size_t)r * kNBins + b];
+          mx = std::max(mx, a.table_sigma_t[(size_t)r * kNBins + b]);
+        }
+        maj[r] = mx;
+      }
+      for (int r = n - 2; r >= 0; --r) maj[r] = std::max(maj[r], maj[r + 1]);
+      if (maj[0] < maj[n - 1])
+        throw std::invalid_argument("tabulated atmosphere: the extinction majorant at the lowest altitude is below the highest one");
+      d.table_n = n;
+      d.table_h0 = a.table_h0;
+      d.table_inv_dh = 1.0 / a.table_dh;
+      d.lower_cutoff = a.table_h0;
+      d.table = h->Upload(rows.data(), rows.size());
+      d.table_majorant = h->Upload(maj.data(), maj.size());
+      h->stats.scene_bytes += (rows.size() + maj.size()) * 4;
     }
   }
   h->dsc.media = h->Upload(media.data(), media.size());

@@ -71,6 +71,7 @@ class Scene:
         self.camera = abi.Camera()
         self.empty_space_medium = -1
         self._has_normals = False
+        self._keep = []
 
     # ---- tables -----------------------------------------------------------
     def add_shader(self, kind, reflectance=0.8, alpha=0.0, ior=1.5, ior_lambda_coeff=0.0, specular=0.0, alpha_min=None):
@@ -111,6 +112,28 @@ class Scene:
         m.rayleigh_sigma_s = _spec(rayleigh_sigma_s)
         m.mie_sigma_s = _spec(mie_sigma_s)
         m.mie_sigma_a = _spec(mie_sigma_a)
+        self.media.append(m)
+        return len(self.media) - 1
+
+    def add_tabulated_atmosphere(self, center, radius, h0, dh, sigma_t, sigma_s_molecules, sigma_s_aerosols, mie_g=0.7,
+                                 priority=None):
+        """Atmosphere with coefficient tables over altitude (rows h0 + i * dh, 36 bins each)."""
+        tabs = [np.ascontiguousarray(np.asarray(t, dtype=np.float32).reshape(-1, NBINS))
+                for t in (sigma_t, sigma_s_molecules, sigma_s_aerosols)]
+        if not (tabs[0].shape == tabs[1].shape == tabs[2].shape):
+            raise ValueError("the three tables need the same number of altitudes")
+        self._keep.extend(tabs)
+        m = abi.Medium()
+        m.kind = abi.MEDIUM_ATMOSPHERE_TABULATED
+        m.priority = len(self.media) if priority is None else priority
+        m.phase_kind = abi.PHASE_RAYLEIGH
+        m.planet_center[:] = [float(x) for x in center]
+        m.planet_radius = radius
+        m.mie_g = mie_g
+        m.table_size = len(tabs[0])
+        m.table_h0, m.table_dh = h0, dh
+        fp = C.POINTER(C.c_float)
+        m.table_sigma_t, m.table_sigma_s_molecules, m.table_sigma_s_aerosols = (t.ctypes.data_as(fp) for t in tabs)
         self.media.append(m)
         return len(self.media) - 1
 
@@ -258,7 +281,7 @@ class Scene:
             if not items:
                 return 0, None
             a = (T * len(items))(*items)
-            self.__dict__.setdefault("_keep", []).append(a)
+            self._keep.append(a)
             return len(items), C.cast(a, P(T))
         d.num_shaders, d.shaders = arr(abi.Shader, self.shaders)
         d.num_media, d.media = arr(abi.Medium, self.media)

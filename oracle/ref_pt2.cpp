@@ -84,7 +84,54 @@ template <int NW> inline Spec<NW> RowMean(const Mat<NW> &m, double scale) {
 inline double AtmoAltitude(const ttgpu_medium &m, V3 p) {
   return Length(p - V3{m.planet_center[0], m.planet_center[1], m.planet_center[2]}) - m.planet_radius;
 }
-inline double AtmoLowerCutoffOf(const ttgpu_medium &m) { return std::max(-m.rayleigh_scale_height, -m.mie_scale_height); }
+inline bool IsAtmosphere(const ttgpu_medium &m) {
+  return m.kind == TTGPU_MEDIUM_ATMOSPHERE || m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED;
+}
+inline double AtmoLowerCutoffOf(const ttgpu_medium &m) {
+  if (m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) return m.table_h0;
+  return std::max(-m.rayleigh_scale_height, -m.mie_scale_height);
+}
+// TabulatedConstituents (atmosphere.cxx:265-352): real-valued row index (alt - H0) / dH; zero at or
+// above the last row; the index truncates towards zero (so altitudes less than one step below the
+// table extrapolate the first interval); further below, the first row.
+inline bool AtmoTableLerp(const ttgpu_medium &m, double alt, int &i0, int &i1, double &f) {
+  const double real = (alt - m.table_h0) / m.table_dh;
+  if (real >= (double)(m.table_size - 1)) return false;
+  const int idx = (int)real;
+  if (idx >= 0) { i0 = idx; i1 = idx + 1; f = real - idx; }
+  else { i0 = i1 = 0; f = 0.; }
+  return true;
+}
+// Per-bin coefficients of a tabulated atmosphere: Rayleigh scattering, aerosol scattering,
+// absorption (extinction minus scattering).
+inline void AtmoTableCoeffs(const ttgpu_medium &m, double alt, int bin, double &ss_ray, double &ss_mie, double &sa) {
+  int i0, i1;
+  double f;
+  if (!AtmoTableLerp(m, alt, i0, i1, f)) { ss_ray = ss_mie = sa = 0.; return; }
+  auto lerp = [&](const float *t) { return (1. - f) * t[(size_t)i0 * 36 + bin] + f * t[(size_t)i1 * 36 + bin]; };
+  ss_ray = lerp(m.table_sigma_s_molecules);
+  ss_mie = lerp(m.table_sigma_s_aerosols);
+  sa = lerp(m.table_sigma_t) - (ss_ray + ss_mie);
+}
+// Extinction majorant table of a tabulated atmosphere: the per-altitude maximum over the
+// wavelengths, made non-increasing with altitude (atmosphere.cxx:238-254).
+inline std::vector<double> AtmoTableMajorants(const ttgpu_medium &m) {
+  std::vector<double> maj(m.table_size);
+  for (int r = 0; r < m.table_size; ++r) {
+    double mx = 0.;
+    for (int b = 0; b < 36; ++b) mx = std::max(mx, (double)m.table_sigma_t[(size_t)r * 36 + b]);
+    maj[r] = mx;
+  }
+  for (int r = m.table_size - 2; r >= 0; --r) maj[r] = std::max(maj[r], maj[r + 1]);
+  return maj;
+}
+// The majorant at an altitude, interpolated like the coefficients.
+inline double AtmoTableMajorant(const ttgpu_medium &m, const std::vector<double> &maj, double alt) {
+  int i0, i1;
+  double f;
+  if (!AtmoTableLerp(m, alt, i0, i1, f)) return 0.;
+  return (1. - f) * maj[i0] + f * maj[i1];
+}
 // Point of the segment org + t dir, t in [0, len], closest to the planet centre.
 inline V3 AtmoLowestPoint(const ttgpu_medium &m, V3 org, V3 dir, double len) {
   V3 c{m.planet_center[0], m.planet_center[1], m.planet_center[2]};
@@ -99,6 +146,7 @@ template <int NW> struct Engine {
   const Scene &sc;
   const ttgpu_params &prm;
   Shading<NW> sh;
+  std::vector<std::vector<double>> table_majorants;  // per medium (tabulated atmospheres)
   double type_probs[4] = {0, 0, 0, 0};
   int counts[4] = {0, 0, 0, 0};
   bool use_nee = true, use_emission = true;
@@ -121,6 +169,8 @@ template <int NW> struct Engine {
   std::vector<double> area_cmf, point_cmf;
 
   Engine(const Scene &s, const ttgpu_params &p) : sc(s), prm(p), sh(s) {
+    for (const auto &m : sc.media)
+      table_majorants.push_back(m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED ? AtmoTableMajorants(m) : std::vector<double>{});
     if (p.sample_mode == TTGPU_SAMPLE_BSDF) use_nee = false;
     if (p.sample_mode == TTGPU_SAMPLE_LIGHTS) use_emission = false;
     counts[IDX_POINT] = (int)sc.points.size();
@@ -381,6 +431,10 @@ template <int NW> struct Engine {
   double Altitude(const ttgpu_medium &m, V3 p) const { return AtmoAltitude(m, p); }
   double AtmoLowerCutoff(const ttgpu_medium &m) const { return AtmoLowerCutoffOf(m); }
   void AtmoCoeffs(const ttgpu_medium &m, double alt, const PathContext<NW> &ctx, Spec<NW> &ss_ray, Spec<NW> &ss_mie, Spec<NW> &sa) const {
+    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) {
+      for (int k = 0; k < NW; ++k) AtmoTableCoeffs(m, alt, ctx.bins[k], ss_ray[k], ss_mie[k], sa[k]);
+      return;
+    }
     alt = alt > AtmoLowerCutoff(m) ? alt : AtmoLowerCutoff(m);
     double rr = std::exp(-alt / m.rayleigh_scale_height), rm = std::exp(-alt / m.mie_scale_height);
     for (int k = 0; k < NW; ++k) {
@@ -399,7 +453,7 @@ template <int NW> struct Engine {
       }
       return;
     }
-    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    if (IsAtmosphere(m)) {
       Spec<NW> a, b, c;
       AtmoCoeffs(m, Altitude(m, pos), ctx, a, b, c);
       ss = a + b;
@@ -412,8 +466,9 @@ template <int NW> struct Engine {
   Spec<NW> Majorant(int mid, V3 org, V3 dir, double len, const PathContext<NW> &ctx) const {
     if (mid == VACUUM) return SpecConst<NW>(0.);
     const auto &m = sc.media[mid];
-    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    if (IsAtmosphere(m)) {
       double alt = std::max(AtmoLowerCutoff(m), Altitude(m, AtmoLowestPoint(m, org, dir, len)));
+      if (m.kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) return SpecConst<NW>(AtmoTableMajorant(m, table_majorants[mid], alt));
       Spec<NW> a, b, cc;
       AtmoCoeffs(m, alt, ctx, a, b, cc);
       return a + b + cc;
@@ -425,7 +480,7 @@ template <int NW> struct Engine {
   void MediumPhaseSample(int mid, V3 wi, V3 pos, Sampler &smp, const PathContext<NW> &ctx, ScatterSample<NW> &s) const {
     if (mid == VACUUM) { s = {-wi, SpecConst<NW>(1.), Pdf(1.)}; return; }
     const auto &m = sc.media[mid];
-    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    if (IsAtmosphere(m)) {
       // SimpleCombined of Rayleigh and HG weighted by the constituents' sigma_s (phasefunctions.cxx:199-270)
       Spec<NW> w[2], sa;
       AtmoCoeffs(m, Altitude(m, pos), ctx, w[0], w[1], sa);
@@ -461,7 +516,7 @@ template <int NW> struct Engine {
   Spec<NW> MediumPhaseEval(int mid, V3 wi, V3 pos, V3 wo, const PathContext<NW> &ctx, double *pdf) const {
     if (mid == VACUUM) { *pdf = 1.; return SpecConst<NW>(0.); }
     const auto &m = sc.media[mid];
-    if (m.kind == TTGPU_MEDIUM_ATMOSPHERE) {
+    if (IsAtmosphere(m)) {
       Spec<NW> w[2], sa;
       AtmoCoeffs(m, Altitude(m, pos), ctx, w[0], w[1], sa);
       double sel[2];
@@ -1130,6 +1185,15 @@ extern "C" void ttref_atmo_lowest_point(const ttgpu_medium *m, const double org[
 }
 // Scattering and absorption coefficients of all 36 bins at an altitude.
 extern "C" void ttref_atmo_coeffs(const ttgpu_medium *m, double altitude, double sigma_s[36], double sigma_a[36]) {
+  if (m->kind == TTGPU_MEDIUM_ATMOSPHERE_TABULATED) {
+    for (int b = 0; b < 36; ++b) {
+      double r, a, sa;
+      AtmoTableCoeffs(*m, altitude, b, r, a, sa);
+      sigma_s[b] = r + a;
+      sigma_a[b] = sa;
+    }
+    return;
+  }
   const double cut = AtmoLowerCutoffOf(*m);
   const double alt = altitude > cut ? altitude : cut;
   const double rr = std::exp(-alt / m->rayleigh_scale_height), rm = std::exp(-alt / m->mie_scale_height);
@@ -1137,5 +1201,9 @@ extern "C" void ttref_atmo_coeffs(const ttgpu_medium *m, double altitude, double
     sigma_s[b] = m->rayleigh_sigma_s[b] * rr + m->mie_sigma_s[b] * rm;
     sigma_a[b] = m->mie_sigma_a[b] * rm;
   }
+}
+// Extinction majorant of a tabulated atmosphere at an altitude (clamped to the table's lowest).
+extern "C" double ttref_atmo_table_majorant(const ttgpu_medium *m, double altitude) {
+  return AtmoTableMajorant(*m, AtmoTableMajorants(*m), std::max(altitude, m->table_h0));
 }
 extern "C" long ttref_bisection(const double *vals, long n, double r) { return BisectionSearch(vals, n, r); }

@@ -34,6 +34,7 @@ long ttref_bisection(const double *vals, long n, double r);
 double ttref_atmo_altitude(const ttgpu_medium *m, const double pos[3]);
 void ttref_atmo_lowest_point(const ttgpu_medium *m, const double org[3], const double dir[3], double len, double out[3]);
 void ttref_atmo_coeffs(const ttgpu_medium *m, double altitude, double sigma_s[36], double sigma_a[36]);
+double ttref_atmo_table_majorant(const ttgpu_medium *m, double altitude);
 /* guiding building blocks (ref_guiding.cpp) */
 void ttref_vmf_init(ttgpu_vmf8 *out);
 void ttref_vmf_pdf(const ttgpu_vmf8 *m, int64_t n, const double *dirs, double *out);

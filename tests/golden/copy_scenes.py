@@ -11,6 +11,7 @@ SCENES = [
     "cornelbox_translucentsphere.nff", "cornelbox.dae", "unitrectangle.obj", "unitrectangle.nff", "medium_in_medium.nff",
     "unitcube.dae", "veach_mis_scene_distant.nff", "veach_mis_scene_distant.obj", "nested_media1.nff", "litcorner.nff",
     "env_sphere_mirror.nff", "unitdisc_24_xz.obj", "cubelight.nff", "off_center_rgb_cmy.nff", "medium_backlighting_area.nff",
+    "atmosphere.nff", "earth_atmosphere_collision_coefficients.json",
 ]
 
 if __name__ == "__main__":

@@ -40,6 +40,9 @@ FIXTURES = {
                               32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
     "nff_medium_in_medium32": (lambda w, h: scene_file.SceneFile(os.path.join(HERE, "scenes", "medium_in_medium.nff")),
                                32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+    # the reference's tabulated earth atmosphere (atmosphere.nff), backlit horizon from 40 km
+    "nff_atmosphere32": (lambda w, h: scene_file.SceneFile(os.path.join(HERE, "scenes", "atmosphere_backlit_plain.nff")),
+                         32, 32, 8192, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
 }
 
 

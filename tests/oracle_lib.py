@@ -50,6 +50,8 @@ def lib():
         L.ttref_atmo_altitude.restype = C.c_double
         L.ttref_atmo_lowest_point.argtypes = [P(abi.Medium), f64p, f64p, C.c_double, f64p]
         L.ttref_atmo_coeffs.argtypes = [P(abi.Medium), C.c_double, f64p, f64p]
+        L.ttref_atmo_table_majorant.argtypes = [P(abi.Medium), C.c_double]
+        L.ttref_atmo_table_majorant.restype = C.c_double
         L.ttref_vmf_init.argtypes = [P(abi.Vmf8)]
         L.ttref_vmf_pdf.argtypes = [P(abi.Vmf8), C.c_int64, f64p, f64p]
         L.ttref_vmf_sample.argtypes = [P(abi.Vmf8), C.c_int64, f64p, f64p]
@@ -186,6 +188,10 @@ def atmo_coeffs(medium, altitude):
     sa = np.zeros(36)
     lib().ttref_atmo_coeffs(C.byref(medium), altitude, _p(ss), _p(sa))
     return ss, sa
+
+
+def atmo_table_majorant(medium, altitude):
+    return lib().ttref_atmo_table_majorant(C.byref(medium), altitude)
 
 
 def vmf_init():

@@ -40,6 +40,15 @@ def test_gpu_absorber_transmittance(gpu_lib, mode):
     assert np.all(np.abs(mean - expected) < 4 * se + 2e-3 * expected)
 
 
+def test_gpu_tabulated_absorber_shell(gpu_lib):
+    # Altitude-constant tabulated absorber inside a sphere of radius 10: L exp(-sigma_a * 10) per bin.
+    s = configs.tabulated_absorber_shell(16, 16)
+    mean, se = _mean_se(s, 16, 16, 4096, 4)
+    expected = absorber_expected(s, s.shell_radius)
+    print("tabulated absorber", mean, "+-", se, "expected", expected, flush=True)
+    assert np.all(np.abs(mean - expected) < 4 * se + 2e-3 * expected)
+
+
 def test_gpu_nee_transmission(gpu_lib):
     s = configs.point_lit_plane_behind_absorber(16, 16)
     mean, se = _mean_se(s, 16, 16, 1024, 2)

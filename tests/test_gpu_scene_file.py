@@ -47,3 +47,7 @@ def test_gpu_write_hdr_images(gpu_lib, tmp_path):
     ch = read_exr(tmp_path / "a.exr")
     np.testing.assert_array_equal(np.stack([ch["R"], ch["G"], ch["B"]], axis=-1), mean)
     np.testing.assert_array_equal(read_pfm(tmp_path / "a.pfm"), mean)
+
+
+def test_gpu_nff_tabulated_atmosphere_parity(gpu_lib):
+    render_parity("nff_atmosphere32", builder=lambda w, h: SceneFile(os.path.join(SCENES, "atmosphere_backlit_plain.nff")))

@@ -437,3 +437,59 @@ def test_oracle_fog_furnace(oracle, variant, mode):
     mean, se = _render_mean(oracle, s, 4, 4, 512, 80, abi.MIS_CONSISTENT, mode)
     expected = spectrum_to_rgb(flat(1.0 / (1 - rho)))
     assert np.all(np.abs(mean - expected) < 4 * se + 2e-3 * np.abs(expected)), (mean, se, expected)
+
+
+# ---- tabulated atmosphere (TabulatedConstituents, src/atmosphere.cxx:219-353) ------------------
+def _earth_table_medium():
+    import json
+    d = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes",
+                                    "earth_atmosphere_collision_coefficients.json")))
+    s = Scene()
+    s.add_tabulated_atmosphere([0.0, 0.0, 0.0], 6300.0, d["H0"], d["deltaH"], d["sigma_t"], d["sigma_s_molecules"],
+                               d["sigma_s_aerosoles"])
+    t = {k: np.asarray(d[k], dtype=np.float32).astype(np.float64) for k in ("sigma_t", "sigma_s_molecules", "sigma_s_aerosoles")}
+    return s, s.media[0], d["H0"], d["deltaH"], t
+
+
+def test_oracle_tabulated_atmosphere_lookup(oracle):
+    s, m, h0, dh, t = _earth_table_medium()
+    n = len(t["sigma_t"])
+    sig_s = t["sigma_s_molecules"] + t["sigma_s_aerosoles"]
+    for i in (0, 1, 57, n - 2):
+        # At the support altitudes the coefficients are the table's rows ...
+        ss, sa = oracle.atmo_coeffs(m, h0 + i * dh)
+        np.testing.assert_allclose(ss, sig_s[i], rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(sa, t["sigma_t"][i] - sig_s[i], rtol=1e-12, atol=1e-15)
+        # ... and halfway between two rows their average.
+        ss, sa = oracle.atmo_coeffs(m, h0 + (i + 0.5) * dh)
+        np.testing.assert_allclose(ss, 0.5 * (sig_s[i] + sig_s[i + 1]), rtol=1e-12, atol=1e-15)
+    # At and above the last altitude: empty space.
+    for alt in (h0 + (n - 1) * dh, h0 + n * dh + 7.0):
+        ss, sa = oracle.atmo_coeffs(m, alt)
+        assert np.all(ss == 0) and np.all(sa == 0)
+    # Far below the table: the first row; less than one step below: the first interval extrapolated.
+    ss, _ = oracle.atmo_coeffs(m, h0 - 3.7 * dh)
+    np.testing.assert_allclose(ss, sig_s[0], rtol=1e-12)
+    ss, _ = oracle.atmo_coeffs(m, h0 - 0.5 * dh)
+    np.testing.assert_allclose(ss, 1.5 * sig_s[0] - 0.5 * sig_s[1], rtol=1e-9)
+
+
+def test_oracle_tabulated_majorant_bounds_and_decreases(oracle):
+    s, m, h0, dh, t = _earth_table_medium()
+    n = len(t["sigma_t"])
+    alts = np.sort(np.random.default_rng(3).uniform(h0, h0 + (n - 1) * dh, 2000))
+    maj = np.array([oracle.atmo_table_majorant(m, a) for a in alts])
+    assert np.all(np.diff(maj) <= 1e-15)
+    for a, mj in zip(alts[::50], maj[::50]):
+        ss, sa = oracle.atmo_coeffs(m, a)
+        assert np.all(ss + sa <= mj * (1 + 1e-12))
+    # Below the lowest altitude the majorant is the bottom row's.
+    assert oracle.atmo_table_majorant(m, h0 - 5.0) == pytest.approx(t["sigma_t"][0].max(), rel=1e-12)
+
+
+def test_oracle_tabulated_absorber_shell(oracle):
+    """Beer-Lambert through an altitude-constant tabulated absorber: L exp(-sigma_a R) per bin."""
+    s = configs.tabulated_absorber_shell(4, 4)
+    mean, se = _render_mean(oracle, s, 4, 4, 20000, 4, abi.MIS_REFERENCE, abi.SAMPLE_BOTH)
+    expected = absorber_expected(s, s.shell_radius)
+    assert np.all(np.abs(mean - expected) < 4 * se + 1e-3 * expected), (mean, se, expected)

@@ -272,7 +272,7 @@ def test_errors_name_the_line():
         nff("s 0 0 0 1\nbogus 1 2 3\n")
     with pytest.raises(ValueError, match="textures are not supported"):
         nff("diffuse d 1 1 1 1\ndiffusetexture checker.png\n")
-    with pytest.raises(ValueError, match="tabulated atmospheres"):
+    with pytest.raises(RuntimeError, match="could not open data.json"):
         nff("tabulatedatmosphere a 0 0 0 6300 data.json\n")
     with pytest.raises(ValueError, match="Bad number of arguments to transform"):
         nff("transform 1 2\n")

@@ -283,8 +283,11 @@ def main():
     else:
         alg_bytes = SHADE_BYTES_PER_PATH * rays_closest / max(launches, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    # Closest-hit rays run in the persistent traversal kernel unless TTGPU_PERSISTENT_TRAVERSAL=0.
+    persistent = os.environ.get("TTGPU_PERSISTENT_TRAVERSAL", "1")[:1] != "0"
+    kname = "k_trace_dyn" if dom == "trace" and persistent else f"k_{dom}"
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(f"k_{dom}"), "kernel": f"k_{dom}",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": load_traffic(kname), "kernel": kname,
                 "avg_launch_ms": round(avg_ms, 4),
                 "bytes_per_unit": {"trace": TRACE_BYTES_PER_RAY, "shade": SHADE_BYTES_PER_PATH, "shadow": SHADOW_BYTES_PER_RAY}[dom]}
     ktimes = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in times.items()}

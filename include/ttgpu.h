@@ -213,7 +213,12 @@ enum ttgpu_mis_mode {
    * multiplies the selection probability into the NEE side as well. */
   TTGPU_MIS_REFERENCE = 0, TTGPU_MIS_CONSISTENT = 1
 };
-enum ttgpu_guide_kind { TTGPU_GUIDE_NONE = 0, TTGPU_GUIDE_VMF_TREE = 1, TTGPU_GUIDE_LEARNED = 2 };
+/* VMF_TREE: kd-tree cells with 8-lobe vMF mixtures; QUADTREE: kd-tree cells with
+ * quadtree directional distributions and UCB node weights, the reference's
+ * active model (src/path_guiding.hxx:343-344); LEARNED: hash grid + MLP. */
+enum ttgpu_guide_kind {
+  TTGPU_GUIDE_NONE = 0, TTGPU_GUIDE_VMF_TREE = 1, TTGPU_GUIDE_LEARNED = 2, TTGPU_GUIDE_QUADTREE = 3
+};
 
 typedef struct ttgpu_params {
   int32_t algo;                 /* ttgpu_algo */
@@ -363,6 +368,55 @@ int ttgpu_guide_build_kdtree(const double *pts, int64_t n, int32_t min_points, i
  * (vmf_fitting::Sample / Pdf, distribution_mixture_models.cxx:88-150). */
 int ttgpu_guide_sample_vmf(const ttgpu_vmf8 *m, uint64_t seed, int64_t count, float *out_dir, float *out_pdf);
 int ttgpu_guide_pdf_vmf(const ttgpu_vmf8 *m, int64_t count, const float *dirs, float *out_pdf);
+
+/* ---- quadtree directional distribution (guide_kind TTGPU_GUIDE_QUADTREE) -------
+ * A tree over the unit square, to which the sphere maps by v = (z + 1) / 2,
+ * u = (atan2(y, x) / pi + 1) / 2 (equal area: densities differ by 1 / (4 pi);
+ * src/path_guiding.cxx:112-129).  Trees are given in pre-order with the root at
+ * index 0: children[4 n + c] is child c of node n (0 bottom-left, 1 bottom-right,
+ * 2 top-left, 3 top-right), all four -1 for a leaf; at most 8 levels.
+ * Builder (src/path_guiding_quadtree.hxx:317-352, .cxx:32-88): nodes whose
+ * weight exceeds fraction x the total weight are split.  TreeAdaptor
+ * (src/path_guiding_quadtree.hxx:209-315): threshold fraction x root weight;
+ * out_old_index / out_fraction = NewToOld mapping.  `capacity` bounds the
+ * output node count (*num_nodes reports the size even when it does not fit). */
+int ttgpu_quadtree_build(const float *uv, const float *weights, int64_t n, float fraction, int32_t capacity,
+                         int32_t *children, float *node_weights, int32_t *num_nodes);
+int ttgpu_quadtree_adapt(const int32_t *children, const float *weights, int32_t num_nodes, float fraction,
+                         int32_t capacity, int32_t *out_children, float *out_weights, int32_t *out_old_index,
+                         float *out_fraction, int32_t *out_num_nodes);
+/* GPU warping of given uniforms u (2 per sample) through a tree with node
+ * sampling weights `probs` (quadtree::Sample, src/path_guiding_quadtree.cxx:168-244),
+ * and the density over the sphere (quadtree::Pdf x 1/(4 pi), .cxx:247-259;
+ * RadianceDistributionSampled::Pdf, src/path_guiding.hxx:248-254), as the
+ * guided kernels use them. */
+int ttgpu_quadtree_sample(const int32_t *children, const float *probs, int32_t num_nodes, int64_t count,
+                          const float *u, float *out_uv, float *out_dir, float *out_pdf);
+int ttgpu_quadtree_pdf(const int32_t *children, const float *probs, int32_t num_nodes, int64_t count,
+                       const float *dirs, float *out_pdf);
+/* One cell's learned and sampled quadtree distributions (host code of the
+ * library; RadianceDistributionLearned / RadianceDistributionSampled,
+ * src/path_guiding.hxx:200-340, src/path_guiding.cxx:156-294):
+ * initial_fit = InitialFit over all samples (builder fraction 0.1);
+ * bake = Bake (sampled <- learned); incremental_fit = IncrementalFit of the
+ * samples with original == 0 at the given round (UCB exploration term
+ * sqrt(log(round) / (count + 1)) x root flux); update_and_bake =
+ * IterationUpdateAndBake (adaptor fraction 0.05, statistics reset).
+ * which = 0 learned tree, 1 sampled tree.  filter_direction =
+ * ComputeStochasticFilteredDirection of the sampled tree (src/path_guiding.cxx:132-153). */
+typedef struct ttgpu_qmodel ttgpu_qmodel;
+ttgpu_qmodel *ttgpu_qmodel_create(void);
+void ttgpu_qmodel_free(ttgpu_qmodel *m);
+int ttgpu_qmodel_initial_fit(ttgpu_qmodel *m, const float *dirs, const float *weights, int64_t n);
+int ttgpu_qmodel_bake(ttgpu_qmodel *m);
+int ttgpu_qmodel_incremental_fit(ttgpu_qmodel *m, const float *dirs, const float *weights, const uint8_t *original,
+                                 int64_t n, int32_t round);
+int ttgpu_qmodel_update_and_bake(ttgpu_qmodel *m);
+int ttgpu_qmodel_num_nodes(const ttgpu_qmodel *m, int32_t which, int32_t *out);
+int ttgpu_qmodel_get_learned(const ttgpu_qmodel *m, int32_t *children, int64_t *counts, double *means, double *m2);
+int ttgpu_qmodel_get_sampled(const ttgpu_qmodel *m, int32_t *children, double *means, double *stddev, float *probs,
+                             float *prior);
+int ttgpu_qmodel_filter_direction(const ttgpu_qmodel *m, const float dir[3], double u0, double u1, float out[3]);
 
 /* ---- learned guide (guide_kind TTGPU_GUIDE_LEARNED) ---------------------------
  * A multiresolution hash-grid encoding of the position (levels x 2 features,

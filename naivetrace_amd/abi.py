@@ -26,7 +26,7 @@ CAMERA_PINHOLE, CAMERA_FISHEYE = range(2)
 ALGO_PT2, ALGO_PTG = range(2)
 SAMPLE_BOTH, SAMPLE_BSDF, SAMPLE_LIGHTS = range(3)
 MIS_REFERENCE, MIS_CONSISTENT = range(2)
-GUIDE_NONE, GUIDE_VMF_TREE, GUIDE_LEARNED = range(3)
+GUIDE_NONE, GUIDE_VMF_TREE, GUIDE_LEARNED, GUIDE_QUADTREE = range(4)
 OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_DEVICE, ERR_INTERRUPTED, ERR_OUT_OF_MEMORY = range(6)
 IMAGE_PFM, IMAGE_EXR = range(2)
 
@@ -142,6 +142,10 @@ EXPORTS = [
     "ttgpu_nn_init_params", "ttgpu_nn_forward", "ttgpu_nn_gradient", "ttgpu_nn_train",
     "ttgpu_scene_file_load", "ttgpu_scene_file_parse", "ttgpu_scene_file_scene", "ttgpu_scene_file_resolution",
     "ttgpu_scene_file_free", "ttgpu_write_rgb_image", "ttgpu_write_image",
+    "ttgpu_quadtree_build", "ttgpu_quadtree_adapt", "ttgpu_quadtree_sample", "ttgpu_quadtree_pdf",
+    "ttgpu_qmodel_create", "ttgpu_qmodel_free", "ttgpu_qmodel_initial_fit", "ttgpu_qmodel_bake",
+    "ttgpu_qmodel_incremental_fit", "ttgpu_qmodel_update_and_bake", "ttgpu_qmodel_num_nodes",
+    "ttgpu_qmodel_get_learned", "ttgpu_qmodel_get_sampled", "ttgpu_qmodel_filter_direction",
 ]
 
 _lib = None
@@ -195,6 +199,20 @@ def _declare(lib):
         "ttgpu_scene_file_free": (None, [C.c_void_p]),
         "ttgpu_write_rgb_image": (C.c_int, [f32p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]),
         "ttgpu_write_image": (C.c_int, [h, C.c_char_p, C.c_int32]),
+        "ttgpu_quadtree_build": (C.c_int, [f32p, f32p, C.c_int64, C.c_float, C.c_int32, i32p, f32p, i32p]),
+        "ttgpu_quadtree_adapt": (C.c_int, [i32p, f32p, C.c_int32, C.c_float, C.c_int32, i32p, f32p, i32p, f32p, i32p]),
+        "ttgpu_quadtree_sample": (C.c_int, [i32p, f32p, C.c_int32, C.c_int64, f32p, f32p, f32p, f32p]),
+        "ttgpu_quadtree_pdf": (C.c_int, [i32p, f32p, C.c_int32, C.c_int64, f32p, f32p]),
+        "ttgpu_qmodel_create": (C.c_void_p, []),
+        "ttgpu_qmodel_free": (None, [C.c_void_p]),
+        "ttgpu_qmodel_initial_fit": (C.c_int, [C.c_void_p, f32p, f32p, C.c_int64]),
+        "ttgpu_qmodel_bake": (C.c_int, [C.c_void_p]),
+        "ttgpu_qmodel_incremental_fit": (C.c_int, [C.c_void_p, f32p, f32p, u8p, C.c_int64, C.c_int32]),
+        "ttgpu_qmodel_update_and_bake": (C.c_int, [C.c_void_p]),
+        "ttgpu_qmodel_num_nodes": (C.c_int, [C.c_void_p, C.c_int32, i32p]),
+        "ttgpu_qmodel_get_learned": (C.c_int, [C.c_void_p, i32p, i64p, f64p, f64p]),
+        "ttgpu_qmodel_get_sampled": (C.c_int, [C.c_void_p, i32p, f64p, f64p, f32p, f32p]),
+        "ttgpu_qmodel_filter_direction": (C.c_int, [C.c_void_p, f32p, C.c_double, C.c_double, f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

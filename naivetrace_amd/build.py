@@ -14,7 +14,7 @@ OUT = os.path.join(HERE, "libttgpu.so")
 ARCH = os.environ.get("TTGPU_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "nn.hip", "ttgpu_api.cpp"]
-CPP_SOURCES = ["bvh_build.cpp", "guiding.cpp", "scene_file.cpp", "image_io.cpp"]
+CPP_SOURCES = ["bvh_build.cpp", "guiding.cpp", "quadtree.cpp", "scene_file.cpp", "image_io.cpp"]
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-result", "-ffp-contract=fast"]

@@ -123,6 +123,19 @@ struct GuideCell {
   float cdf[kGuideLobes];
 };
 
+// Quadtree directional distribution of one cell over the equal-area square of the sphere.  Only
+// branches are stored, 32 bytes each: w = sampling weights of the children (0 bottom-left, 1
+// bottom-right, 2 top-left, 3 top-right), child = branch index of a child, -1 for a leaf child.
+constexpr int kQuadMaxDepth = 8;
+struct QuadNode {
+  float4 w;
+  int4 child;
+};
+struct QuadCell {
+  int root;      // branch index of the root; -1: the root is a leaf (uniform over the sphere)
+  float w_root;  // weight of the root (the sum of its children's weights)
+};
+
 struct DeviceScene {
   const Bvh4Node *nodes;             // surface BVH
   const PrimRecord *prims;         // leaf order
@@ -156,7 +169,9 @@ struct DeviceScene {
   int persistent;                  // 1: persistent traversal kernels with dynamic ray fetch
   // guiding (guided integrator only)
   const GuideNode *guide_nodes;
-  const GuideCell *guide_cells;
+  const GuideCell *guide_cells;     // vMF mixtures, or null
+  const QuadCell *guide_qcells;     // quadtree distributions, or null
+  const QuadNode *guide_qnodes;
   int guide_root;                  // root handle (< 0: the tree is a single cell ~root)
   int guide_record;                // 1: training pass, write radiance records
 };

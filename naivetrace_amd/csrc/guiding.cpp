@@ -298,8 +298,9 @@ void PcaFrame(const OnlineCov3 &s, double frame[3][3]) {
 }
 }  // namespace
 
-Recorder::Recorder(int subdiv_factor, uint64_t seed, int threads)
-    : subdiv_factor_(std::max(1, subdiv_factor)), seed_(seed), threads_(threads > 0 ? threads : (int)std::thread::hardware_concurrency()) {
+Recorder::Recorder(int subdiv_factor, uint64_t seed, int threads, bool quadtree)
+    : subdiv_factor_(std::max(1, subdiv_factor)), quadtree_(quadtree), seed_(seed),
+      threads_(threads > 0 ? threads : (int)std::thread::hardware_concurrency()) {
   InitUnitSphere(prior_);
   tree_.root = ~0;
   tree_.num_leafs = 1;
@@ -328,6 +329,7 @@ FitParams Recorder::ParamsOf(const CellData &c) const {
 
 void Recorder::FinalizeRound(std::vector<Record> &records) {
   ++round_;
+  sub_round_ = 1;
   if (round_ <= 1) AdaptInitial(records);
   else FitTheSamples(records);
 }
@@ -346,6 +348,15 @@ void Recorder::AdaptInitial(std::vector<Record> &records) {
     cd.last_num_samples = cd.max_num_samples = e - b;
     for (int k = b; k < e; ++k) cd.leaf_stats.Add(records[order[k]].pos);
     PcaFrame(cd.leaf_stats, cd.cov_frame);
+    if (quadtree_) {
+      // InitialFit on every record of the cell, then Bake (path_guiding.cxx:660-662).
+      std::vector<quad::Sample1> s;
+      s.reserve(e - b);
+      for (int k = b; k < e; ++k) s.push_back({records[order[k]].dir, records[order[k]].weight, true});
+      cd.qlearned.InitialFit(s);
+      cd.qcurrent = cd.qlearned.Bake();
+      return;
+    }
     const FitParams p = ParamsOf(cd);
     for (int k = b; k < e; ++k) {
       const Record &r = records[order[k]];
@@ -375,6 +386,15 @@ void Recorder::FitTheSamples(std::vector<Record> &records) {
       double rv[3] = {U(rng) * 2. - 1., U(rng) * 2. - 1., U(rng) * 2. - 1.};
       for (int a = 0; a < 3; ++a)
         f.pos[a] = r.pos[a] + 3. * (cd.cov_frame[a][0] * rv[0] + cd.cov_frame[a][1] * rv[1] + cd.cov_frame[a][2] * rv[2]);
+      f.original = false;
+      if (quadtree_) {
+        // The quadtree's filter spreads over the leaf's angular footprint (path_guiding.cxx:132-153).
+        const double u0 = U(rng), u1 = U(rng);
+        cd.qcurrent.FilterDirection(r.dir, u0, u1, f.dir);
+        records[n0 + i] = f;
+        cell[n0 + i] = tree_.Lookup(f.pos);
+        continue;
+      }
       // ComputeStochasticFilteredDirection (path_guiding.cxx:80-88)
       const float pdf = Pdf(cd.current, r.dir);
       const float a = -0.1f / (pdf * float(2. * kPi) + 1.e-9f) + 1.f;
@@ -386,7 +406,6 @@ void Recorder::FitTheSamples(std::vector<Record> &records) {
       float fx[3], fy[3];
       FrameZ(r.dir, fx, fy);
       for (int c = 0; c < 3; ++c) f.dir[c] = (float)(fx[c] * std::cos(om) * s + fy[c] * std::sin(om) * s + r.dir[c] * z);
-      f.original = false;
       records[n0 + i] = f;
       cell[n0 + i] = tree_.Lookup(f.pos);
     }
@@ -400,6 +419,16 @@ void Recorder::FitTheSamples(std::vector<Record> &records) {
     std::mt19937_64 rng(seed_ * 0x2545F4914F6CDD1Dull + (uint64_t)round_ * 7919ull + (uint64_t)c);
     std::shuffle(idx.begin(), idx.end(), rng);
     CellData &cd = cells_[c];
+    if (quadtree_) {
+      // IncrementalFit updates the sampled distribution of the cell in place (path_guiding.cxx:476-481).
+      std::vector<quad::Sample1> s;
+      s.reserve(idx.size());
+      for (int i : idx) s.push_back({records[i].dir, records[i].weight, records[i].original});
+      cd.qlearned.IncrementalFit(s, sub_round_, cd.qcurrent);
+      for (int i : idx)
+        if (records[i].original) cd.leaf_stats.Add(records[i].pos);
+      return;
+    }
     const FitParams p = ParamsOf(cd);
     for (int i : idx) {
       const Record &r = records[i];
@@ -407,6 +436,7 @@ void Recorder::FitTheSamples(std::vector<Record> &records) {
       if (r.original) cd.leaf_stats.Add(r.pos);
     }
   });
+  ++sub_round_;
 }
 
 void Recorder::PrepareAdaptedStructures() {
@@ -436,10 +466,16 @@ void Recorder::AdaptIncremental() {
   KdTree t = AdaptTree(tree_, decide, mapping);
   std::vector<CellData> nd(t.num_leafs);
   for (int i = 0; i < (int)cells_.size(); ++i) {
-    const CellData &src = cells_[i];
+    CellData &src = cells_[i];
+    quad::Sampled baked;
+    if (quadtree_) baked = src.qlearned.IterationUpdateAndBake(src.qcurrent);
     for (int dst : {mapping[i].first, mapping[i].second}) {
       if (dst < 0) continue;
       CellData &d = nd[dst];
+      if (quadtree_) {
+        d.qcurrent = baked;
+        d.qlearned = src.qlearned;
+      }
       d.current = src.learned;  // IterationUpdateAndBake
       d.learned = src.learned;
       d.fit = src.fit;

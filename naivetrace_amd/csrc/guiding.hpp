@@ -8,8 +8,10 @@
 //   src/path_guiding.cxx:45-77, 320-812 (records, stochastic filtering, rounds,
 //   initial tree build, adaptive splitting), src/path_guiding_tree.hxx:50-438
 //   (kd-tree, adaptor, builder).
+// Cells hold either the vMF mixture or the quadtree directional distribution
+// (quadtree.hpp, the reference's active model: path_guiding.hxx:343-344).
 // Runs on the host between passes (the reference's recorder is host code too);
-// the GPU samples and evaluates the baked mixtures (guide_vmf.hpp).
+// the GPU samples and evaluates the baked distributions (guide_vmf.hpp, guide_quad.hpp).
 #pragma once
 #include <array>
 #include <cstdint>
@@ -17,6 +19,8 @@
 #include <random>
 #include <utility>
 #include <vector>
+
+#include "quadtree.hpp"
 
 namespace ttgpu {
 namespace guide {
@@ -111,9 +115,11 @@ struct CellData {
   // current estimate (sampled by the renderer)
   Vmf current;
   double cov_frame[3][3] = {};  // eigenvectors scaled by the standard deviations
+  quad::Sampled qcurrent;
   // learned
   Vmf learned;
   FitData fit;
+  quad::Learned qlearned;
   OnlineCov3 leaf_stats;
   int64_t last_num_samples = 0, max_num_samples = 0;
   CellData() {
@@ -125,7 +131,9 @@ struct CellData {
 // Recorder (PathGuiding, path_guiding.cxx:320-812).
 class Recorder {
  public:
-  Recorder(int subdiv_factor, uint64_t seed, int threads);
+  // quadtree: cells hold the quadtree distribution instead of the vMF mixture.
+  Recorder(int subdiv_factor, uint64_t seed, int threads, bool quadtree = false);
+  bool quadtree() const { return quadtree_; }
   // One 1-spp pass worth of records (BeginRound .. FinalizeRound).
   void FinalizeRound(std::vector<Record> &records);
   // After each doubling sweep: split cells and bake the learned mixtures (PrepareAdaptedStructures).
@@ -145,7 +153,11 @@ class Recorder {
   std::vector<CellData> cells_;
   Vmf prior_;
   int subdiv_factor_;
+  bool quadtree_;
   int round_ = 0;
+  // The reference starts every pass with sub_round = 1 (PathGuiding::BeginRound) and counts it up
+  // after the pass's fit, so every fit sees round 1 and the exploration term log(1) = 0.
+  int sub_round_ = 1;
   int64_t previous_max_samples_per_cell_ = 0, previous_total_samples_ = 0;
   uint64_t seed_;
   int threads_;

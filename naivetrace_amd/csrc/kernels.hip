@@ -11,6 +11,7 @@
 #include "trace.hpp"
 #include "media.hpp"
 #include "guide_vmf.hpp"
+#include "guide_quad.hpp"
 #include "nn_device.hpp"
 #include "kernels.hpp"
 
@@ -873,10 +874,12 @@ __device__ __forceinline__ void emit_records(const PathQueues &q, uint32_t i, in
   }
 }
 
-template <bool kLearned>
+// kGuide: 0 the cell's vMF mixture, 1 the learned per-path mixture, 2 the cell's quadtree.
+template <int kGuide>
 __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
                                                       const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                       uint32_t *count_out, double *fb, double *fb_sq) {
+  constexpr bool kLearned = kGuide == 1, kQuad = kGuide == 2;
   const uint32_t count = *count_in;
   const bool ident = count_in[kIdentOffset] != 0;
   const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
@@ -919,10 +922,13 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
       } else {
         const SurfHit h = make_hit(sc, prim, hitv.y, hitv.z, hitv.x, xyz(org4), dir);
         const DevMaterial mat = sc.materials[h.material];
-        // The guide of this vertex: the kd-tree cell's mixture, or the learned per-path mixture.
+        // The guide of this vertex: the kd-tree cell's mixture or quadtree, or the learned
+        // per-path mixture.
         const GuideCell *cellp = nullptr;
         const nn::Mix16 *mixp = nullptr;
+        QuadGuide qg{};
         if constexpr (kLearned) mixp = reinterpret_cast<const nn::Mix16 *>(q.gmix) + i;
+        else if constexpr (kQuad) qg = quad_guide(sc, guide_lookup(sc, h.pos));
         else cellp = sc.guide_cells + guide_lookup(sc, h.pos);
         if (use_emission && mat.emitter >= 0) {
           const DevEmitter &em = sc.emitters[mat.emitter];
@@ -951,6 +957,7 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
             if (!ls.delta && mis) {
               float gp;
               if constexpr (kLearned) gp = guided_pdf(shd, *mixp, ls.dir, bsdf_pdf);
+              else if constexpr (kQuad) gp = guided_pdf(shd, qg, ls.dir, bsdf_pdf);
               else gp = guided_pdf(shd, *cellp, ls.dir, bsdf_pdf);
               w = power_heuristic(ls.pdf, gp);
             }
@@ -969,6 +976,7 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
         rough = fmaxf(my_roughness(shd), rough);
         Scatter s;
         if constexpr (kLearned) s = sample_guided(shd, *mixp, h, wi, bins, rough, lambda0, rng);
+        else if constexpr (kQuad) s = sample_guided(shd, qg, h, wi, bins, rough, lambda0, rng);
         else s = sample_guided(shd, *cellp, h, wi, bins, rough, lambda0, rng);
         const uint32_t node = (flags & kFlagNodeMask) + 1u;
         bool alive = node <= (uint32_t)sc.max_depth;
@@ -1189,6 +1197,20 @@ __global__ void k_guide_pdf(GuideCell c, int64_t n, const float *dirs, float *ou
   for (int64_t i = gtid(); i < n; i += grid_stride()) out_pdf[i] = guide_pdf(c, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
 }
 
+// Quadtree warp of given uniforms (2 per sample): the square point, the direction and its pdf.
+__global__ void k_quad_sample(QuadGuide g, int64_t n, const float *u, float *out_uv, float *out_dir, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) {
+    const float2 p = quad_warp(g, u[2 * i], u[2 * i + 1]);
+    const float3 d = quad_square_to_sphere(p.x, p.y);
+    out_uv[2 * i] = p.x; out_uv[2 * i + 1] = p.y;
+    out_dir[3 * i] = d.x; out_dir[3 * i + 1] = d.y; out_dir[3 * i + 2] = d.z;
+    out_pdf[i] = guide_pdf(g, d);
+  }
+}
+__global__ void k_quad_pdf(QuadGuide g, int64_t n, const float *dirs, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) out_pdf[i] = guide_pdf(g, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
 // ---- host-side launchers ------------------------------------------------------------------
 static int g_trace_blocks = 0;
 
@@ -1247,10 +1269,13 @@ void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
 void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
                  uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
   if (sc.guide_cells)
-    hipLaunchKernelGGL(k_shade_guided<false>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
+    hipLaunchKernelGGL(k_shade_guided<0>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
+                       queue_out, count_out, fb, fb_sq);
+  else if (sc.guide_qcells)
+    hipLaunchKernelGGL(k_shade_guided<2>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
                        queue_out, count_out, fb, fb_sq);
   else if (q.gmix)
-    hipLaunchKernelGGL(k_shade_guided<true>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
+    hipLaunchKernelGGL(k_shade_guided<1>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
                        queue_out, count_out, fb, fb_sq);
   else if (sc.has_media)
     hipLaunchKernelGGL(k_shade_media, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, queue_in, count_in,
@@ -1288,6 +1313,13 @@ void LaunchGuideInfer(const DeviceScene &sc, const PathQueues &q, const uint32_t
 }
 void LaunchGuideSample(const GuideCell &c, uint64_t seed, int64_t n, float *d, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_guide_sample, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, c, seed, n, d, p);
+}
+void LaunchQuadSample(const QuadNode *nodes, int root, float w_root, int64_t n, const float *u, float *uv, float *d, float *p,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_quad_sample, dim3(Blocks(n, 256, 4096)), dim3(256), 0, st, QuadGuide{nodes, root, w_root}, n, u, uv, d, p);
+}
+void LaunchQuadPdf(const QuadNode *nodes, int root, float w_root, int64_t n, const float *dirs, float *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_quad_pdf, dim3(Blocks(n, 256, 4096)), dim3(256), 0, st, QuadGuide{nodes, root, w_root}, n, dirs, p);
 }
 void LaunchGuidePdf(const GuideCell &c, int64_t n, const float *dirs, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_guide_pdf, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, c, n, dirs, p);

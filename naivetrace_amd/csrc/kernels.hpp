@@ -30,6 +30,9 @@ void LaunchEvalShader(const DevShader &s, float3 wi, float3 ng, float3 ns, uint3
 void LaunchGuideInfer(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue, const uint32_t *count, uint32_t max_count,
                       const nn::NetView &net, hipStream_t st);
 void LaunchGuideSample(const GuideCell &c, uint64_t seed, int64_t n, float *d, float *p, hipStream_t st);
+void LaunchQuadSample(const QuadNode *nodes, int root, float w_root, int64_t n, const float *u, float *uv, float *d, float *p,
+                      hipStream_t st);
+void LaunchQuadPdf(const QuadNode *nodes, int root, float w_root, int64_t n, const float *dirs, float *p, hipStream_t st);
 void LaunchGuidePdf(const GuideCell &c, int64_t n, const float *dirs, float *p, hipStream_t st);
 void LaunchSamplePhase(int kind, float g, float3 wi, uint64_t seed, int64_t n, float *d, float *v, float *p, hipStream_t st);
 

@@ -99,12 +99,14 @@ struct ttgpu_handle {
   int num_triangles = 0;
   // guided integrator
   bool guided = false, trained = false;
-  std::unique_ptr<guide::Recorder> recorder;       // vMF-tree guide
+  std::unique_ptr<guide::Recorder> recorder;       // vMF-tree or quadtree guide
   std::unique_ptr<nn::NeuralGuide> net;            // learned guide
   bool learned = false;
   int train_rounds = 0;
   float last_loss = 0.f;
   void *d_guide_nodes = nullptr, *d_guide_cells = nullptr;
+  void *d_guide_qnodes = nullptr, *d_guide_qcells = nullptr;
+  size_t guide_qnodes_cap = 0, guide_qcells_cap = 0;  // allocated element counts
   int64_t records_last_pass = 0, records_dropped = 0;
 
   template <class T> T *Alloc(size_t n) {
@@ -160,6 +162,8 @@ struct ttgpu_handle {
     for (void *p : allocs) (void)hipFree(p);
     if (d_guide_nodes) (void)hipFree(d_guide_nodes);
     if (d_guide_cells) (void)hipFree(d_guide_cells);
+    if (d_guide_qnodes) (void)hipFree(d_guide_qnodes);
+    if (d_guide_qcells) (void)hipFree(d_guide_qcells);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -190,7 +194,7 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (p.num_wavelengths != 0 && p.num_wavelengths != 4) bad("the GPU path traces 4 wavelengths per path");
   if (p.sample_mode < 0 || p.sample_mode > 2) bad("bad sample_mode");
   if (p.algo != TTGPU_ALGO_PT2 && p.algo != TTGPU_ALGO_PTG) bad("unknown algorithm");
-  if (p.guide_kind < TTGPU_GUIDE_NONE || p.guide_kind > TTGPU_GUIDE_LEARNED) bad("unknown guide kind");
+  if (p.guide_kind < TTGPU_GUIDE_NONE || p.guide_kind > TTGPU_GUIDE_QUADTREE) bad("unknown guide kind");
   if (s.num_triangles < 0 || s.num_spheres < 0 || s.num_vertices < 0) bad("negative counts");
   if (s.num_triangles > 0 && (!s.vertices || !s.indices || !s.tri_material)) bad("triangles need vertices, indices, materials");
   if (s.num_spheres > 0 && (!s.spheres || !s.sphere_material)) bad("spheres need data and materials");
@@ -749,7 +753,47 @@ void AllocateQueues(ttgpu_handle *h) {
   Check(hipStreamSynchronize(h->stream), "sync");
 }
 
-// Bakes the recorder's kd-tree and current mixtures into the device layout (GuideNode, GuideCell).
+// Appends the branches of a quadtree (pre-order) to `out` as QuadNodes.  A tree whose root weight
+// is not positive and finite, or with a negative or non-finite weight, becomes the uniform
+// distribution (root -1): such weights do not define a density.
+QuadCell FlattenQuad(const guide::quad::Tree &t, const std::vector<float> &w, std::vector<QuadNode> &out) {
+  const float wr = w[t.root];
+  bool ok = wr > 0.f && std::isfinite(wr);
+  for (size_t i = 0; ok && i < w.size(); ++i) ok = w[i] >= 0.f && std::isfinite(w[i]);
+  if (!ok || t.IsLeaf(t.root)) return {-1, 1.f};
+  std::vector<int> dev(t.NumNodes(), -1);
+  int k = (int)out.size();
+  for (int n = 0; n < t.NumNodes(); ++n)
+    if (!t.IsLeaf(n)) dev[n] = k++;
+  out.resize(k);
+  for (int n = 0; n < t.NumNodes(); ++n) {
+    if (t.IsLeaf(n)) continue;
+    float cw[4];
+    int cc[4];
+    for (int c = 0; c < 4; ++c) {
+      const int ch = t.child[n][c];
+      cw[c] = w[ch];
+      cc[c] = t.IsLeaf(ch) ? -1 : dev[ch];
+    }
+    out[dev[n]] = QuadNode{make_float4(cw[0], cw[1], cw[2], cw[3]), make_int4(cc[0], cc[1], cc[2], cc[3])};
+  }
+  return {dev[t.root], wr};
+}
+
+// Uploads `n` elements into a device buffer that grows as needed.
+template <class T>
+void UploadGrow(void *&buf, size_t &cap, const std::vector<T> &v) {
+  if (v.size() > cap || !buf) {
+    if (buf) Check(hipFree(buf), "hipFree");
+    buf = nullptr;
+    cap = std::max<size_t>(1, v.size() + v.size() / 4);
+    Check(hipMalloc(&buf, cap * sizeof(T)), "hipMalloc guide");
+  }
+  if (!v.empty()) Check(hipMemcpy(buf, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload guide");
+}
+
+// Bakes the recorder's kd-tree and current distributions into the device layout (GuideNode and
+// GuideCell for vMF mixtures; QuadCell and QuadNode for quadtrees).
 void UploadGuide(ttgpu_handle *h) {
   const guide::KdTree &t = h->recorder->tree();
   const auto &cells = h->recorder->cells();
@@ -757,6 +801,23 @@ void UploadGuide(ttgpu_handle *h) {
   for (size_t k = 0; k < t.nodes.size(); ++k) {
     const auto &n = t.nodes[k];
     nodes[k] = {(float)n.pos, n.axis, n.left, n.right};
+  }
+  if (h->recorder->quadtree()) {
+    std::vector<QuadCell> qc(std::max<size_t>(1, cells.size()), QuadCell{-1, 1.f});
+    std::vector<QuadNode> qn;
+    for (size_t c = 0; c < cells.size(); ++c) qc[c] = FlattenQuad(cells[c].qcurrent.tree, cells[c].qcurrent.probs, qn);
+    if (h->d_guide_nodes) Check(hipFree(h->d_guide_nodes), "hipFree");
+    h->d_guide_nodes = nullptr;
+    Check(hipMalloc(&h->d_guide_nodes, nodes.size() * sizeof(GuideNode)), "hipMalloc guide");
+    Check(hipMemcpy(h->d_guide_nodes, nodes.data(), nodes.size() * sizeof(GuideNode), hipMemcpyHostToDevice), "upload guide");
+    UploadGrow(h->d_guide_qcells, h->guide_qcells_cap, qc);
+    UploadGrow(h->d_guide_qnodes, h->guide_qnodes_cap, qn);
+    h->dsc.guide_nodes = (const GuideNode *)h->d_guide_nodes;
+    h->dsc.guide_cells = nullptr;
+    h->dsc.guide_qcells = (const QuadCell *)h->d_guide_qcells;
+    h->dsc.guide_qnodes = (const QuadNode *)h->d_guide_qnodes;
+    h->dsc.guide_root = t.root;
+    return;
   }
   std::vector<GuideCell> dc(std::max<size_t>(1, cells.size()));
   for (size_t c = 0; c < cells.size(); ++c) {
@@ -1034,7 +1095,8 @@ int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_han
       h->net = std::make_unique<nn::NeuralGuide>(nn::NeuralGuideConfig{}, flo, fhi, params->seed);
     } else if (h->guided) {
       const int subdiv = params->guide_subdiv_factor > 0 ? params->guide_subdiv_factor : 100;
-      h->recorder = std::make_unique<guide::Recorder>(subdiv, params->seed, params->num_threads);
+      h->recorder = std::make_unique<guide::Recorder>(subdiv, params->seed, params->num_threads,
+                                                      params->guide_kind == TTGPU_GUIDE_QUADTREE);
       UploadGuide(h);
     }
     auto t1 = std::chrono::steady_clock::now();
@@ -1107,6 +1169,8 @@ int ttgpu_run(ttgpu_handle *h, int32_t max_passes) {
         for (int k = 0; k < num && !h->stop.load(); ++k) {
           RenderPass(h, 1);
           LearnFromPass(h);
+          // The quadtree's fit updates the sampled distributions after every pass.
+          if (h->recorder && h->recorder->quadtree()) UploadGuide(h);
         }
         if (h->cb) h->cb(h->cb_user, 1);
         if (!h->learned) {
@@ -1588,6 +1652,247 @@ int ttgpu_guide_pdf_vmf(const ttgpu_vmf8 *m, int64_t count, const float *dirs, f
   (void)hipFree(dd);
   (void)hipFree(dp);
   return rc;
+}
+
+
+// ---- quadtree guide building blocks --------------------------------------------------------
+
+static void TreeFromArrays(const int32_t *children, int32_t num_nodes, guide::quad::Tree &t) {
+  if (!children || num_nodes <= 0 || num_nodes > guide::quad::kMaxNodes) throw std::invalid_argument("bad quadtree");
+  t.child.assign(num_nodes, {-1, -1, -1, -1});
+  t.root = 0;
+  for (int32_t n = 0; n < num_nodes; ++n) {
+    const int32_t *c = children + 4 * (size_t)n;
+    const bool leaf = c[0] < 0;
+    for (int k = 0; k < 4; ++k) {
+      if ((c[k] < 0) != leaf) throw std::invalid_argument("quadtree node with some but not all children");
+      if (!leaf && (c[k] <= n || c[k] >= num_nodes)) throw std::invalid_argument("quadtree children must follow their parent (pre-order)");
+      t.child[n][k] = c[k];
+    }
+  }
+  // Every node but the root has exactly one parent, and descents are at most 8 levels deep.
+  std::vector<int> parents(num_nodes, 0), depth(num_nodes, 0);
+  for (int32_t n = 0; n < num_nodes; ++n)
+    if (!t.IsLeaf(n))
+      for (int k = 0; k < 4; ++k) {
+        ++parents[t.child[n][k]];
+        depth[t.child[n][k]] = depth[n] + 1;
+      }
+  for (int32_t n = 1; n < num_nodes; ++n)
+    if (parents[n] != 1) throw std::invalid_argument("quadtree node without exactly one parent");
+  for (int32_t n = 0; n < num_nodes; ++n)
+    if (depth[n] >= guide::quad::kMaxDepth) throw std::invalid_argument("quadtree deeper than 8 levels");
+}
+
+static void TreeToArrays(const guide::quad::Tree &t, int32_t *children) {
+  for (int n = 0; n < t.NumNodes(); ++n)
+    for (int k = 0; k < 4; ++k) children[4 * (size_t)n + k] = t.child[n][k];
+}
+
+int ttgpu_quadtree_build(const float *uv, const float *weights, int64_t n, float fraction, int32_t capacity,
+                         int32_t *children, float *node_weights, int32_t *num_nodes) {
+  if (n < 0 || (n > 0 && (!uv || !weights)) || !children || !node_weights || !num_nodes) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    std::vector<guide::quad::Point> pts(n);
+    std::vector<float> w(weights, weights + n), nw;
+    for (int64_t i = 0; i < n; ++i) pts[i] = {uv[2 * i], uv[2 * i + 1]};
+    const guide::quad::Tree t = guide::quad::Build(pts, w, fraction, nw);
+    *num_nodes = t.NumNodes();
+    if (t.NumNodes() > capacity) return Fail(nullptr, TTGPU_ERR_INVALID_ARGUMENT, "quadtree capacity too small");
+    TreeToArrays(t, children);
+    std::copy(nw.begin(), nw.end(), node_weights);
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_quadtree_adapt(const int32_t *children, const float *weights, int32_t num_nodes, float fraction, int32_t capacity,
+                         int32_t *out_children, float *out_weights, int32_t *out_old_index, float *out_fraction,
+                         int32_t *out_num_nodes) {
+  if (!weights || !out_children || !out_weights || !out_old_index || !out_fraction || !out_num_nodes)
+    return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    guide::quad::Tree t;
+    TreeFromArrays(children, num_nodes, t);
+    std::vector<float> w(weights, weights + num_nodes), nw;
+    std::vector<guide::quad::NewToOld> map;
+    const guide::quad::Tree nt = guide::quad::Adapt(t, w, fraction, nw, map);
+    *out_num_nodes = nt.NumNodes();
+    if (nt.NumNodes() > capacity) return Fail(nullptr, TTGPU_ERR_INVALID_ARGUMENT, "quadtree capacity too small");
+    TreeToArrays(nt, out_children);
+    for (int i = 0; i < nt.NumNodes(); ++i) {
+      out_weights[i] = nw[i];
+      out_old_index[i] = map[i].idx;
+      out_fraction[i] = map[i].fractional_size;
+    }
+    return TTGPU_OK;
+  })
+}
+
+namespace {
+// Device copy of one quadtree for the sampling and pdf entry points.
+struct DeviceQuad {
+  QuadNode *nodes = nullptr;
+  QuadCell cell{-1, 1.f};
+  ~DeviceQuad() {
+    if (nodes) (void)hipFree(nodes);
+  }
+};
+void UploadQuad(const int32_t *children, const float *probs, int32_t num_nodes, DeviceQuad &d) {
+  guide::quad::Tree t;
+  TreeFromArrays(children, num_nodes, t);
+  if (!probs) throw std::invalid_argument("null weights");
+  std::vector<float> w(probs, probs + num_nodes);
+  std::vector<QuadNode> nodes;
+  d.cell = FlattenQuad(t, w, nodes);
+  nodes.resize(std::max<size_t>(1, nodes.size()));
+  Check(hipMalloc(&d.nodes, nodes.size() * sizeof(QuadNode)), "hipMalloc");
+  Check(hipMemcpy(d.nodes, nodes.data(), nodes.size() * sizeof(QuadNode), hipMemcpyHostToDevice), "upload");
+}
+// Device scratch of `count` floats per column, freed on scope exit.
+struct DeviceFloats {
+  std::vector<float *> p;
+  ~DeviceFloats() {
+    for (float *x : p)
+      if (x) (void)hipFree(x);
+  }
+  float *Get(size_t n) {
+    float *x = nullptr;
+    Check(hipMalloc(&x, std::max<size_t>(1, n) * sizeof(float)), "hipMalloc");
+    p.push_back(x);
+    return x;
+  }
+};
+}  // namespace
+
+int ttgpu_quadtree_sample(const int32_t *children, const float *probs, int32_t num_nodes, int64_t count, const float *u,
+                          float *out_uv, float *out_dir, float *out_pdf) {
+  if (count < 0 || (count > 0 && (!u || !out_uv || !out_dir || !out_pdf))) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    DeviceQuad q;
+    UploadQuad(children, probs, num_nodes, q);
+    if (count == 0) return TTGPU_OK;
+    DeviceFloats f;
+    float *du = f.Get(2 * count), *duv = f.Get(2 * count), *dd = f.Get(3 * count), *dp = f.Get(count);
+    Check(hipMemcpy(du, u, 2 * count * sizeof(float), hipMemcpyHostToDevice), "upload");
+    LaunchQuadSample(q.nodes, q.cell.root, q.cell.w_root, count, du, duv, dd, dp, 0);
+    Check(hipGetLastError(), "launch");
+    Check(hipDeviceSynchronize(), "sync");
+    Check(hipMemcpy(out_uv, duv, 2 * count * sizeof(float), hipMemcpyDeviceToHost), "download");
+    Check(hipMemcpy(out_dir, dd, 3 * count * sizeof(float), hipMemcpyDeviceToHost), "download");
+    Check(hipMemcpy(out_pdf, dp, count * sizeof(float), hipMemcpyDeviceToHost), "download");
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_quadtree_pdf(const int32_t *children, const float *probs, int32_t num_nodes, int64_t count, const float *dirs,
+                       float *out_pdf) {
+  if (count < 0 || (count > 0 && (!dirs || !out_pdf))) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    DeviceQuad q;
+    UploadQuad(children, probs, num_nodes, q);
+    if (count == 0) return TTGPU_OK;
+    DeviceFloats f;
+    float *dd = f.Get(3 * count), *dp = f.Get(count);
+    Check(hipMemcpy(dd, dirs, 3 * count * sizeof(float), hipMemcpyHostToDevice), "upload");
+    LaunchQuadPdf(q.nodes, q.cell.root, q.cell.w_root, count, dd, dp, 0);
+    Check(hipGetLastError(), "launch");
+    Check(hipDeviceSynchronize(), "sync");
+    Check(hipMemcpy(out_pdf, dp, count * sizeof(float), hipMemcpyDeviceToHost), "download");
+    return TTGPU_OK;
+  })
+}
+
+struct ttgpu_qmodel {
+  guide::quad::Learned learned;
+  guide::quad::Sampled sampled;
+};
+
+ttgpu_qmodel *ttgpu_qmodel_create(void) {
+  try {
+    return new ttgpu_qmodel();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void ttgpu_qmodel_free(ttgpu_qmodel *m) { delete m; }
+
+static std::vector<guide::quad::Sample1> QSamples(const float *dirs, const float *weights, const uint8_t *original, int64_t n) {
+  if (n < 0 || (n > 0 && (!dirs || !weights))) throw std::invalid_argument("bad sample arrays");
+  std::vector<guide::quad::Sample1> s(n);
+  for (int64_t i = 0; i < n; ++i) s[i] = {dirs + 3 * i, weights[i], original ? original[i] != 0 : true};
+  return s;
+}
+
+int ttgpu_qmodel_initial_fit(ttgpu_qmodel *m, const float *dirs, const float *weights, int64_t n) {
+  if (!m) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    m->learned.InitialFit(QSamples(dirs, weights, nullptr, n));
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_qmodel_bake(ttgpu_qmodel *m) {
+  if (!m) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    m->sampled = m->learned.Bake();
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_qmodel_incremental_fit(ttgpu_qmodel *m, const float *dirs, const float *weights, const uint8_t *original, int64_t n,
+                                 int32_t round) {
+  if (!m) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    m->learned.IncrementalFit(QSamples(dirs, weights, original, n), round, m->sampled);
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_qmodel_update_and_bake(ttgpu_qmodel *m) {
+  if (!m) return TTGPU_ERR_INVALID_ARGUMENT;
+  TTGPU_GUARD(nullptr, {
+    m->sampled = m->learned.IterationUpdateAndBake(m->sampled);
+    return TTGPU_OK;
+  })
+}
+
+int ttgpu_qmodel_num_nodes(const ttgpu_qmodel *m, int32_t which, int32_t *out) {
+  if (!m || !out || (which != 0 && which != 1)) return TTGPU_ERR_INVALID_ARGUMENT;
+  *out = which == 0 ? m->learned.tree.NumNodes() : m->sampled.tree.NumNodes();
+  return TTGPU_OK;
+}
+
+int ttgpu_qmodel_get_learned(const ttgpu_qmodel *m, int32_t *children, int64_t *counts, double *means, double *m2) {
+  if (!m || !children || !counts || !means || !m2) return TTGPU_ERR_INVALID_ARGUMENT;
+  const auto &l = m->learned;
+  TreeToArrays(l.tree, children);
+  for (int i = 0; i < l.tree.NumNodes(); ++i) {
+    counts[i] = l.stats.n[i];
+    means[i] = l.stats.mean[i];
+    m2[i] = l.stats.m2[i];
+  }
+  return TTGPU_OK;
+}
+
+int ttgpu_qmodel_get_sampled(const ttgpu_qmodel *m, int32_t *children, double *means, double *stddev, float *probs,
+                             float *prior) {
+  if (!m || !children || !means || !stddev || !probs || !prior) return TTGPU_ERR_INVALID_ARGUMENT;
+  const auto &d = m->sampled;
+  TreeToArrays(d.tree, children);
+  for (int i = 0; i < d.tree.NumNodes(); ++i) {
+    means[i] = d.means[i];
+    stddev[i] = d.stddev[i];
+    probs[i] = d.probs[i];
+    prior[i] = d.prior[i];
+  }
+  return TTGPU_OK;
+}
+
+int ttgpu_qmodel_filter_direction(const ttgpu_qmodel *m, const float *dir, double u0, double u1, float *out) {
+  if (!m || !dir || !out) return TTGPU_ERR_INVALID_ARGUMENT;
+  m->sampled.FilterDirection(dir, u0, u1, out);
+  return TTGPU_OK;
 }
 
 }  // extern "C"

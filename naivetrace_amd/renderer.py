@@ -238,6 +238,131 @@ def vmf_pdf_gpu(mixture, dirs):
     return p
 
 
+def _i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _quad_call(fn, *args):
+    lib = abi.load_library()
+    rc = getattr(lib, fn)(*args)
+    if rc:
+        raise TTGpuError(rc, f"{fn}: {lib.ttgpu_create_error().decode()}")
+
+
+def quadtree_build(uv, weights, fraction=0.1, capacity=1 << 16):
+    """Quadtree builder of the library: (children [N, 4] int32, node weights [N] float32), pre-order."""
+    uv = _f32(uv, (-1, 2))
+    w = _f32(weights)
+    ch = np.zeros((capacity, 4), np.int32)
+    nw = np.zeros(capacity, np.float32)
+    n = C.c_int32()
+    _quad_call("ttgpu_quadtree_build", _f32p(uv), _f32p(w), len(w), fraction, capacity, _i32p(ch), _f32p(nw), C.byref(n))
+    return ch[:n.value].copy(), nw[:n.value].copy()
+
+
+def quadtree_adapt(children, weights, fraction=0.05, capacity=1 << 16):
+    """Quadtree adaptor: (children, weights, old index, fractional size) of the new tree."""
+    ch = np.ascontiguousarray(children, dtype=np.int32).reshape(-1, 4)
+    w = _f32(weights)
+    oc = np.zeros((capacity, 4), np.int32)
+    ow = np.zeros(capacity, np.float32)
+    oi = np.zeros(capacity, np.int32)
+    of = np.zeros(capacity, np.float32)
+    n = C.c_int32()
+    _quad_call("ttgpu_quadtree_adapt", _i32p(ch), _f32p(w), len(ch), fraction, capacity, _i32p(oc), _f32p(ow), _i32p(oi),
+               _f32p(of), C.byref(n))
+    k = n.value
+    return oc[:k].copy(), ow[:k].copy(), oi[:k].copy(), of[:k].copy()
+
+
+def quadtree_sample_gpu(children, probs, u):
+    """GPU warp of uniforms u [n, 2]: (square points [n, 2], directions [n, 3], pdfs over the sphere [n])."""
+    ch = np.ascontiguousarray(children, dtype=np.int32).reshape(-1, 4)
+    pr = _f32(probs)
+    u = _f32(u, (-1, 2))
+    uv = np.zeros((len(u), 2), np.float32)
+    d = np.zeros((len(u), 3), np.float32)
+    p = np.zeros(len(u), np.float32)
+    _quad_call("ttgpu_quadtree_sample", _i32p(ch), _f32p(pr), len(ch), len(u), _f32p(u), _f32p(uv), _f32p(d), _f32p(p))
+    return uv, d, p
+
+
+def quadtree_pdf_gpu(children, probs, dirs):
+    ch = np.ascontiguousarray(children, dtype=np.int32).reshape(-1, 4)
+    pr = _f32(probs)
+    d = _f32(dirs, (-1, 3))
+    p = np.zeros(len(d), np.float32)
+    _quad_call("ttgpu_quadtree_pdf", _i32p(ch), _f32p(pr), len(ch), len(d), _f32p(d), _f32p(p))
+    return p
+
+
+class QuadModel:
+    """One cell's learned and sampled quadtree distributions (the recorder's host code)."""
+
+    def __init__(self):
+        self._lib = abi.load_library()
+        self._m = self._lib.ttgpu_qmodel_create()
+        if not self._m:
+            raise MemoryError("ttgpu_qmodel_create")
+
+    def close(self):
+        if self._m:
+            self._lib.ttgpu_qmodel_free(self._m)
+            self._m = None
+
+    def __del__(self):
+        self.close()
+
+    def initial_fit(self, dirs, weights):
+        d, w = _f32(dirs, (-1, 3)), _f32(weights)
+        _quad_call("ttgpu_qmodel_initial_fit", self._m, _f32p(d), _f32p(w), len(w))
+
+    def bake(self):
+        _quad_call("ttgpu_qmodel_bake", self._m)
+
+    def incremental_fit(self, dirs, weights, original, round_):
+        d, w = _f32(dirs, (-1, 3)), _f32(weights)
+        o = np.ascontiguousarray(original, dtype=np.uint8)
+        _quad_call("ttgpu_qmodel_incremental_fit", self._m, _f32p(d), _f32p(w), o.ctypes.data_as(C.POINTER(C.c_uint8)),
+                   len(w), int(round_))
+
+    def update_and_bake(self):
+        _quad_call("ttgpu_qmodel_update_and_bake", self._m)
+
+    def _n(self, which):
+        n = C.c_int32()
+        _quad_call("ttgpu_qmodel_num_nodes", self._m, which, C.byref(n))
+        return n.value
+
+    def learned(self):
+        n = self._n(0)
+        ch = np.zeros((n, 4), np.int32)
+        cnt = np.zeros(n, np.int64)
+        mean = np.zeros(n)
+        m2 = np.zeros(n)
+        _quad_call("ttgpu_qmodel_get_learned", self._m, _i32p(ch), cnt.ctypes.data_as(C.POINTER(C.c_int64)),
+                   mean.ctypes.data_as(C.POINTER(C.c_double)), m2.ctypes.data_as(C.POINTER(C.c_double)))
+        return {"children": ch, "counts": cnt, "means": mean, "m2": m2}
+
+    def sampled(self):
+        n = self._n(1)
+        ch = np.zeros((n, 4), np.int32)
+        means = np.zeros(n)
+        sd = np.zeros(n)
+        probs = np.zeros(n, np.float32)
+        prior = np.zeros(n, np.float32)
+        dp = C.POINTER(C.c_double)
+        _quad_call("ttgpu_qmodel_get_sampled", self._m, _i32p(ch), means.ctypes.data_as(dp), sd.ctypes.data_as(dp),
+                   _f32p(probs), _f32p(prior))
+        return {"children": ch, "means": means, "stddev": sd, "probs": probs, "prior": prior}
+
+    def filter_direction(self, d, u0, u1):
+        d = _f32(d, (3,))
+        out = np.zeros(3, np.float32)
+        _quad_call("ttgpu_qmodel_filter_direction", self._m, _f32p(d), float(u0), float(u1), _f32p(out))
+        return out
+
+
 def _nn_call(fn, *args):
     lib = abi.load_library()
     rc = getattr(lib, fn)(*args)

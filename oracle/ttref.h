@@ -41,6 +41,30 @@ void ttref_vmf_pdf(const ttgpu_vmf8 *m, int64_t n, const double *dirs, double *o
 void ttref_vmf_sample(const ttgpu_vmf8 *m, int64_t n, const double *u, double *out);
 void ttref_vmf_fit(ttgpu_vmf8 *m, const double *dirs, const double *weights, int64_t n, int64_t max_num_samples);
 int ttref_kdtree_build(const double *pts, int64_t n, int32_t min_points, int32_t max_depth, int32_t *leaf_of_point);
+/* Quadtree directional distribution (ref_quadtree.cpp).  Trees: pre-order children arrays,
+ * root 0, 4 per node, -1 for a leaf's children.  Build / adapt return the node count, or minus it
+ * when `capacity` is too small. */
+void ttref_qt_sphere_to_square(int64_t n, const double *dirs, double *uv);
+void ttref_qt_square_to_sphere(int64_t n, const double *uv, double *dirs);
+int32_t ttref_qt_build(const double *uv, const double *w, int64_t n, double fraction, int32_t capacity, int32_t *children,
+                       double *node_w);
+int32_t ttref_qt_adapt(const int32_t *children, const double *w, int32_t nn, double fraction, int32_t capacity,
+                       int32_t *out_children, double *out_w, int32_t *out_old, double *out_frac);
+void ttref_qt_sample(const int32_t *children, const double *probs, int32_t nn, int64_t count, const double *u, double *out_uv,
+                     double *out_pdf);
+void ttref_qt_pdf(const int32_t *children, const double *probs, int32_t nn, int64_t count, const double *uv, double *out_pdf);
+void ttref_qt_propagate(const int32_t *children, int32_t nn, double *w);
+void *ttref_qm_create(void);
+void ttref_qm_free(void *m);
+void ttref_qm_initial_fit(void *m, const double *dirs, const double *w, int64_t n);
+void ttref_qm_bake(void *m);
+void ttref_qm_incremental_fit(void *m, const double *dirs, const double *w, const uint8_t *original, int64_t n, int32_t round);
+void ttref_qm_update_and_bake(void *m);
+int32_t ttref_qm_num_nodes(const void *m, int32_t which);
+void ttref_qm_get_learned(const void *m, int32_t *children, double *counts, double *means, double *m2);
+void ttref_qm_get_sampled(const void *m, int32_t *children, double *means, double *stddev, double *probs, double *prior);
+double ttref_qm_filter_angle(const void *m, const double *dir);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,28 @@ def lib():
         L.ttref_vmf_fit.argtypes = [P(abi.Vmf8), f64p, f64p, C.c_int64, C.c_int64]
         L.ttref_kdtree_build.argtypes = [f64p, C.c_int64, C.c_int32, C.c_int32, i32p]
         L.ttref_kdtree_build.restype = C.c_int
+        vp, u8p = C.c_void_p, P(C.c_uint8)
+        L.ttref_qt_sphere_to_square.argtypes = [C.c_int64, f64p, f64p]
+        L.ttref_qt_square_to_sphere.argtypes = [C.c_int64, f64p, f64p]
+        L.ttref_qt_build.argtypes = [f64p, f64p, C.c_int64, C.c_double, C.c_int32, i32p, f64p]
+        L.ttref_qt_build.restype = C.c_int32
+        L.ttref_qt_adapt.argtypes = [i32p, f64p, C.c_int32, C.c_double, C.c_int32, i32p, f64p, i32p, f64p]
+        L.ttref_qt_adapt.restype = C.c_int32
+        L.ttref_qt_sample.argtypes = [i32p, f64p, C.c_int32, C.c_int64, f64p, f64p, f64p]
+        L.ttref_qt_pdf.argtypes = [i32p, f64p, C.c_int32, C.c_int64, f64p, f64p]
+        L.ttref_qt_propagate.argtypes = [i32p, C.c_int32, f64p]
+        L.ttref_qm_create.restype = vp
+        L.ttref_qm_free.argtypes = [vp]
+        L.ttref_qm_initial_fit.argtypes = [vp, f64p, f64p, C.c_int64]
+        L.ttref_qm_bake.argtypes = [vp]
+        L.ttref_qm_incremental_fit.argtypes = [vp, f64p, f64p, u8p, C.c_int64, C.c_int32]
+        L.ttref_qm_update_and_bake.argtypes = [vp]
+        L.ttref_qm_num_nodes.argtypes = [vp, C.c_int32]
+        L.ttref_qm_num_nodes.restype = C.c_int32
+        L.ttref_qm_get_learned.argtypes = [vp, i32p, f64p, f64p, f64p]
+        L.ttref_qm_get_sampled.argtypes = [vp, i32p, f64p, f64p, f64p, f64p]
+        L.ttref_qm_filter_angle.argtypes = [vp, f64p]
+        L.ttref_qm_filter_angle.restype = C.c_double
         _lib = L
     return _lib
 
@@ -226,3 +248,117 @@ def kdtree_build(points, min_points, max_depth):
     leaf = np.zeros(len(p), np.int32)
     n = lib().ttref_kdtree_build(_p(p), len(p), min_points, max_depth, _p(leaf, C.c_int32))
     return leaf, n
+
+
+# ---- quadtree directional distribution ----------------------------------------------------
+
+def qt_sphere_to_square(dirs):
+    d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    uv = np.zeros((len(d), 2))
+    lib().ttref_qt_sphere_to_square(len(d), _p(d), _p(uv))
+    return uv
+
+
+def qt_square_to_sphere(uv):
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    d = np.zeros((len(uv), 3))
+    lib().ttref_qt_square_to_sphere(len(uv), _p(uv), _p(d))
+    return d
+
+
+def qt_build(uv, weights, fraction, capacity=1 << 16):
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    ch = np.zeros((capacity, 4), np.int32)
+    nw = np.zeros(capacity)
+    n = lib().ttref_qt_build(_p(uv), _p(w), len(w), fraction, capacity, _p(ch, C.c_int32), _p(nw))
+    assert n > 0
+    return ch[:n].copy(), nw[:n].copy()
+
+
+def qt_adapt(children, weights, fraction, capacity=1 << 16):
+    ch = np.ascontiguousarray(children, dtype=np.int32).reshape(-1, 4)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    oc = np.zeros((capacity, 4), np.int32)
+    ow = np.zeros(capacity)
+    oi = np.zeros(capacity, np.int32)
+    of = np.zeros(capacity)
+    n = lib().ttref_qt_adapt(_p(ch, C.c_int32), _p(w), len(ch), fraction, capacity, _p(oc, C.c_int32), _p(ow),
+                             _p(oi, C.c_int32), _p(of))
+    assert n > 0
+    return oc[:n].copy(), ow[:n].copy(), oi[:n].copy(), of[:n].copy()
+
+
+def qt_sample(children, probs, u):
+    """Warped square points and their densities on the square."""
+    ch = np.ascontiguousarray(children, dtype=np.int32).reshape(-1, 4)
+    pr = np.ascontiguousarray(probs, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 2)
+    uv = np.zeros((len(u), 2))
+    pdf = np.zeros(len(u))
+    lib().ttref_qt_sample(_p(ch, C.c_int32), _p(pr), len(ch), len(u), _p(u), _p(uv), _p(pdf))
+    return uv, pdf
+
+
+def qt_pdf(children, probs, uv):
+    """Densities on the square."""
+    ch = np.ascontiguousarray(children, dtype=np.int32).reshape(-1, 4)
+    pr = np.ascontiguousarray(probs, dtype=np.float64)
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    pdf = np.zeros(len(uv))
+    lib().ttref_qt_pdf(_p(ch, C.c_int32), _p(pr), len(ch), len(uv), _p(uv), _p(pdf))
+    return pdf
+
+
+def qt_propagate(children, weights):
+    ch = np.ascontiguousarray(children, dtype=np.int32).reshape(-1, 4)
+    w = np.array(weights, dtype=np.float64)
+    lib().ttref_qt_propagate(_p(ch, C.c_int32), len(ch), _p(w))
+    return w
+
+
+class QuadModel:
+    """Oracle of one cell's learned / sampled quadtree distributions."""
+
+    def __init__(self):
+        self._m = lib().ttref_qm_create()
+
+    def __del__(self):
+        if getattr(self, "_m", None):
+            lib().ttref_qm_free(self._m)
+            self._m = None
+
+    def initial_fit(self, dirs, weights):
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        lib().ttref_qm_initial_fit(self._m, _p(d), _p(w), len(w))
+
+    def bake(self):
+        lib().ttref_qm_bake(self._m)
+
+    def incremental_fit(self, dirs, weights, original, round_):
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        o = np.ascontiguousarray(original, dtype=np.uint8)
+        lib().ttref_qm_incremental_fit(self._m, _p(d), _p(w), _p(o, C.c_uint8), len(w), int(round_))
+
+    def update_and_bake(self):
+        lib().ttref_qm_update_and_bake(self._m)
+
+    def learned(self):
+        n = lib().ttref_qm_num_nodes(self._m, 0)
+        ch = np.zeros((n, 4), np.int32)
+        cnt, mean, m2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        lib().ttref_qm_get_learned(self._m, _p(ch, C.c_int32), _p(cnt), _p(mean), _p(m2))
+        return {"children": ch, "counts": cnt, "means": mean, "m2": m2}
+
+    def sampled(self):
+        n = lib().ttref_qm_num_nodes(self._m, 1)
+        ch = np.zeros((n, 4), np.int32)
+        means, sd, probs, prior = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        lib().ttref_qm_get_sampled(self._m, _p(ch, C.c_int32), _p(means), _p(sd), _p(probs), _p(prior))
+        return {"children": ch, "means": means, "stddev": sd, "probs": probs, "prior": prior}
+
+    def filter_angle(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(3)
+        return lib().ttref_qm_filter_angle(self._m, _p(d))

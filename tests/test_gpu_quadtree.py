@@ -1,0 +1,144 @@
+"""Quadtree guide (guide_kind TTGPU_GUIDE_QUADTREE) on the GPU, through the C ABI.
+
+* The GPU's hierarchical warping equals the oracle's on the same uniforms (square
+  points and densities), its leaf frequencies pass a chi-square test against
+  w_leaf / w_root, and its density entry point equals the oracle's
+  (src/path_guiding_quadtree.cxx:168-259, src/path_guiding.hxx:248-254).
+* Unbiasedness of ptg with the quadtree guide: per-pixel 3-sigma parity with the
+  oracle's converged Cornell render.
+* Training: the per-pass refits and per-sweep adaptations run (more than one cell,
+  records every pass), and on the Cornell box with a hidden light the guided render
+  has lower relMSE than pt2 at equal samples per pixel.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from naivetrace_amd import abi, configs
+from naivetrace_amd.renderer import GpuRenderer, quadtree_pdf_gpu, quadtree_sample_gpu
+import oracle_lib as O
+import stats_util as S
+from test_gpu_guiding import _relmse
+from test_gpu_parity import GOLDEN, _zscores
+from test_quadtree import _leaves, _random_tree, _regions, _unit
+
+pytestmark = pytest.mark.gpu
+
+INV_4PI = 1.0 / (4.0 * np.pi)
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_gpu_quadtree_warp_and_pdf_match_oracle(gpu_lib, oracle, seed):
+    ch, probs = _random_tree(O, seed)
+    p32 = probs.astype(np.float32)
+    # the GPU's weights are the float32 ones; propagate them as the library does
+    p32 = O.qt_propagate(ch, p32.astype(np.float64)).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    n = 400_000
+    u = rng.random((n, 2)).astype(np.float32)
+    print(f"quadtree warp seed {seed}: {len(ch)} nodes, {n} samples", flush=True)
+    uv, d, pdf = quadtree_sample_gpu(ch, p32, u)
+    ouv, opdf = O.qt_sample(ch, p32.astype(np.float64), u.astype(np.float64))
+    same = np.all(np.abs(uv - ouv) < 2e-5, axis=1)
+    print(f"  warped points equal to the oracle's: {same.mean():.6f}", flush=True)
+    assert same.mean() > 0.999
+    assert np.allclose(d, O.qt_square_to_sphere(uv.astype(np.float64)), atol=3e-6)
+    # density of the sampled direction: the oracle's density at the same point (away from leaf edges)
+    np.testing.assert_allclose(pdf[same], O.qt_pdf(ch, p32, O.qt_sphere_to_square(d[same])) * INV_4PI, rtol=2e-4)
+    # leaf frequencies
+    leaves, reg = _leaves(ch), _regions(ch)
+    idx = np.full(n, -1)
+    for k, leaf in enumerate(leaves):
+        lo_u, lo_v, size, _ = reg[leaf]
+        m = (uv[:, 0] >= lo_u) & (uv[:, 0] < lo_u + size) & (uv[:, 1] >= lo_v) & (uv[:, 1] < lo_v + size)
+        idx[m] = k
+    assert (idx >= 0).all()
+    p_leaf = p32[leaves].astype(np.float64) / p32[0]
+    counts = np.bincount(idx, minlength=len(leaves)).astype(float)
+    assert counts[p_leaf == 0].sum() == 0
+    pv, chi2, dof = S.chi_square_pvalue(counts, p_leaf, n)
+    print(f"  leaf chi2 {chi2:.1f} dof {dof} p {pv:.3g}", flush=True)
+    assert pv > 1e-4
+    # the density entry point on arbitrary directions
+    dirs = _unit(rng, 100_000).astype(np.float32)
+    pg = quadtree_pdf_gpu(ch, p32, dirs)
+    po = O.qt_pdf(ch, p32, O.qt_sphere_to_square(dirs.astype(np.float64))) * INV_4PI
+    agree = np.isclose(pg, po, rtol=2e-4, atol=0)
+    assert agree.mean() > 0.999
+
+
+def test_gpu_quadtree_degenerate_weights_are_uniform(gpu_lib):
+    rng = np.random.default_rng(3)
+    dirs = _unit(rng, 1000).astype(np.float32)
+    single = np.array([[-1, -1, -1, -1]], np.int32)
+    assert np.allclose(quadtree_pdf_gpu(single, np.array([2.5], np.float32), dirs), INV_4PI, rtol=1e-6)
+    # weights without a positive root define no density: the guide falls back to uniform
+    ch = np.array([[1, 2, 3, 4]] + [[-1] * 4] * 4, np.int32)
+    assert np.allclose(quadtree_pdf_gpu(ch, np.zeros(5, np.float32), dirs), INV_4PI, rtol=1e-6)
+    uv, d, p = quadtree_sample_gpu(ch, np.zeros(5, np.float32), rng.random((1000, 2)).astype(np.float32))
+    assert np.allclose(p, INV_4PI, rtol=1e-6) and np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-5)
+
+
+def _qparams(w, h, depth, guide_max_spp, spp, seed=23):
+    p = abi.default_params(w, h, spp=spp, max_depth=depth, seed=seed, algo=abi.ALGO_PTG)
+    p.guide_kind = abi.GUIDE_QUADTREE
+    p.guide_max_spp = guide_max_spp
+    p.guide_subdiv_factor = 100
+    return p
+
+
+def test_gpu_ptg_quadtree_schedule(gpu_lib):
+    s = configs.cornell_hidden_light(64, 64)
+    with GpuRenderer(s, _qparams(64, 64, 8, 8, 16)) as r:
+        assert r.run() == 0
+        info = r.guide_info()
+        _, _, n = r.framebuffer()
+    print("quadtree guide info", info, flush=True)
+    assert info["rounds"] == 1 + 2 + 4 + 8
+    assert info["cells"] > 1 and info["records_last_pass"] > 0 and info["records_dropped"] == 0
+    assert np.all(n == 16)
+
+
+def test_gpu_ptg_quadtree_unbiased_cornell(gpu_lib):
+    f = np.load(os.path.join(GOLDEN, "cornell32_consistent_oracle.npz"))
+    w, h, depth, spp = int(f["width"]), int(f["height"]), int(f["max_depth"]), int(f["spp"])
+    with GpuRenderer(configs.cornell_box(w, h), _qparams(w, h, depth, 16, spp)) as r:
+        r.run()
+        rgb, sq, n = r.framebuffer()
+    assert np.all(n == spp)
+    mean = rgb / spp
+    var = np.maximum(sq / spp - mean ** 2, 0) * spp / (spp - 1)
+    z = _zscores(mean, np.sqrt(var / spp), f["mean"], f["stderr"])
+    frac3 = float(np.mean(np.abs(z) > 3))
+    npix = w * h
+    img, img_ref = mean.mean(axis=(0, 1)), f["mean"].mean(axis=(0, 1))
+    img_se = np.sqrt((var / spp).sum(axis=(0, 1)) + (f["stderr"] ** 2).sum(axis=(0, 1))) / npix
+    print(f"ptg quadtree cornell: mean z {z.mean():+.4f} |z|>3 {frac3:.4f} image {img} oracle {img_ref} +- {img_se}",
+          flush=True)
+    assert frac3 < 0.015
+    assert abs(z.mean()) < 5.0 / np.sqrt(z.size) + 0.05
+    assert np.all(np.abs(img - img_ref) < 4 * img_se + 1e-3 * np.abs(img_ref))
+
+
+def test_gpu_ptg_quadtree_reduces_error_hidden_light(gpu_lib):
+    w = h = 64
+    s = configs.cornell_hidden_light(w, h)
+    pref = abi.default_params(w, h, max_depth=8, seed=99)
+    pref.mis_mode = abi.MIS_CONSISTENT
+    with GpuRenderer(s, pref) as r:
+        r.render_spp(32768)
+        ref = r.mean_image()
+    spp = 64
+    pp = abi.default_params(w, h, spp=spp, max_depth=8, seed=5)
+    pp.mis_mode = abi.MIS_CONSISTENT
+    with GpuRenderer(s, pp) as r:
+        r.run()
+        plain = r.mean_image()
+    with GpuRenderer(s, _qparams(w, h, 8, 64, spp, seed=6)) as r:
+        r.run()
+        guided = r.mean_image()
+    e_plain, e_guided = _relmse(plain, ref), _relmse(guided, ref)
+    print(f"hidden light {spp} spp: relMSE plain {e_plain:.5f} quadtree {e_guided:.5f} ratio {e_plain / e_guided:.2f}",
+          flush=True)
+    assert e_guided < e_plain

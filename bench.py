@@ -131,7 +131,7 @@ def relmse(img, ref, eps=1e-2):
 
 def relmse_equal_time(budget_s, size, guide_max_spp, ref_spp):
     """Cornell box with the light hidden behind a baffle (indirect light dominates): relMSE of plain
-    pt2, vMF-tree guiding and learned guiding after the same wall time each (guide training
+    pt2, vMF-tree, quadtree and learned guiding after the same wall time each (guide training
     included; scene upload and BVH build excluded), against a pt2 render with `ref_spp` spp."""
     import threading
     from naivetrace_amd import abi, configs
@@ -167,12 +167,12 @@ def relmse_equal_time(budget_s, size, guide_max_spp, ref_spp):
     p = abi.default_params(size, size, max_depth=8, seed=5)
     p.mis_mode = abi.MIS_CONSISTENT
     out["plain"] = timed(p, False)
-    for name, kind in (("vmf_tree", abi.GUIDE_VMF_TREE), ("learned", abi.GUIDE_LEARNED)):
+    for name, kind in (("vmf_tree", abi.GUIDE_VMF_TREE), ("quadtree", abi.GUIDE_QUADTREE), ("learned", abi.GUIDE_LEARNED)):
         g = abi.default_params(size, size, max_depth=8, seed=6, algo=abi.ALGO_PTG)
         g.guide_kind = kind
         g.guide_max_spp = guide_max_spp
         out[name] = timed(g, True)
-    for name in ("vmf_tree", "learned"):
+    for name in ("vmf_tree", "quadtree", "learned"):
         out[f"gain_{name}"] = round(out["plain"]["relmse"] / max(out[name]["relmse"], 1e-30), 3)
     return out
 

@@ -233,7 +233,7 @@ typedef struct ttgpu_params {
   int32_t linear_output;        /* --linear-out for GenerateImage */
   int32_t num_threads;          /* CPU threads (oracle / host side) */
   /* guiding (ptg) */
-  int32_t guide_kind;           /* ttgpu_guide_kind (TTGPU_ALGO_PTG: VMF_TREE when NONE) */
+  int32_t guide_kind;           /* ttgpu_guide_kind (TTGPU_ALGO_PTG: QUADTREE when NONE) */
   int32_t guide_max_spp;        /* --guide-max-spp: training sweeps of 2, 4, .. up to this many spp (512) */
   float guide_mix_bsdf;         /* ignored: the probability of keeping the BSDF sample is the
                                    shader's (0.5; glossy dielectric with alpha > 0.05: 0.1),
@@ -345,6 +345,13 @@ int ttgpu_sample_phase(int32_t phase_kind, float g, const float wi[3], uint64_t 
  * last training pass and records dropped for lack of buffer space. */
 int ttgpu_guide_info(ttgpu_handle *h, int32_t *cells, int32_t *rounds, int64_t *records_last_pass,
                      int64_t *records_dropped);
+/* Density of the current guide (vMF-tree or quadtree handles) at positions pos
+ * (3 per query) for directions dirs (3 per query, unit length): the cell's
+ * distribution the renderer samples (RadianceDistributionSampled::Pdf,
+ * src/path_guiding.hxx:248-254; vmf_fitting::Pdf).  on_device = 0: the
+ * recorder's host copy; 1: the copy uploaded for the GPU kernels. */
+int ttgpu_guide_pdf(ttgpu_handle *h, int64_t count, const float *pos, const float *dirs, int32_t on_device,
+                    float *out_pdf);
 
 /* 8-lobe von Mises-Fisher mixture (vmf_fitting::VonMisesFischerMixture<8>,
  * src/distribution_mixture_models.hxx:17-29). */

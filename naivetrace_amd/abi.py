@@ -142,7 +142,7 @@ EXPORTS = [
     "ttgpu_nn_init_params", "ttgpu_nn_forward", "ttgpu_nn_gradient", "ttgpu_nn_train",
     "ttgpu_scene_file_load", "ttgpu_scene_file_parse", "ttgpu_scene_file_scene", "ttgpu_scene_file_resolution",
     "ttgpu_scene_file_free", "ttgpu_write_rgb_image", "ttgpu_write_image",
-    "ttgpu_quadtree_build", "ttgpu_quadtree_adapt", "ttgpu_quadtree_sample", "ttgpu_quadtree_pdf",
+    "ttgpu_guide_pdf", "ttgpu_quadtree_build", "ttgpu_quadtree_adapt", "ttgpu_quadtree_sample", "ttgpu_quadtree_pdf",
     "ttgpu_qmodel_create", "ttgpu_qmodel_free", "ttgpu_qmodel_initial_fit", "ttgpu_qmodel_bake",
     "ttgpu_qmodel_incremental_fit", "ttgpu_qmodel_update_and_bake", "ttgpu_qmodel_num_nodes",
     "ttgpu_qmodel_get_learned", "ttgpu_qmodel_get_sampled", "ttgpu_qmodel_filter_direction",
@@ -199,6 +199,7 @@ def _declare(lib):
         "ttgpu_scene_file_free": (None, [C.c_void_p]),
         "ttgpu_write_rgb_image": (C.c_int, [f32p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]),
         "ttgpu_write_image": (C.c_int, [h, C.c_char_p, C.c_int32]),
+        "ttgpu_guide_pdf": (C.c_int, [h, C.c_int64, f32p, f32p, C.c_int32, f32p]),
         "ttgpu_quadtree_build": (C.c_int, [f32p, f32p, C.c_int64, C.c_float, C.c_int32, i32p, f32p, i32p]),
         "ttgpu_quadtree_adapt": (C.c_int, [i32p, f32p, C.c_int32, C.c_float, C.c_int32, i32p, f32p, i32p, f32p, i32p]),
         "ttgpu_quadtree_sample": (C.c_int, [i32p, f32p, C.c_int32, C.c_int64, f32p, f32p, f32p, f32p]),

@@ -32,11 +32,13 @@ TT_HD float2 quad_sphere_to_square(float3 d) {
   return make_float2((atan2f(d.y, d.x) / kPi + 1.f) * 0.5f, (d.z + 1.f) * 0.5f);
 }
 
+// rho = sqrt(1 - z^2) = 2 sqrt(v (1 - v)) avoids the cancellation in 1 - z^2 near the poles, where
+// z = 2v - 1 rounds to a coarse float; sin(pi x) / cos(pi x) avoid rounding pi x.
 TT_HD float3 quad_square_to_sphere(float u, float v) {
   const float z = v * 2.f - 1.f;
-  const float rho = sqrtf(fmaxf(0.f, 1.f - z * z));
+  const float rho = 2.f * sqrtf(fmaxf(0.f, v * (1.f - v)));
   float sn, cs;
-  sincosf((u * 2.f - 1.f) * kPi, &sn, &cs);
+  sincospif(u * 2.f - 1.f, &sn, &cs);
   return normalize(mk(rho * cs, rho * sn, z));
 }
 

@@ -1211,6 +1211,17 @@ __global__ void k_quad_pdf(QuadGuide g, int64_t n, const float *dirs, float *out
   for (int64_t i = gtid(); i < n; i += grid_stride()) out_pdf[i] = guide_pdf(g, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
 }
 
+// Density of the handle's uploaded guide at positions / directions (cell lookup + the cell's
+// vMF mixture or quadtree), as the guided shade kernel evaluates it.
+__global__ void k_guide_pdf_at(DeviceScene sc, int64_t n, const float *pos, const float *dirs, float *out_pdf) {
+  for (int64_t i = gtid(); i < n; i += grid_stride()) {
+    const float3 p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    const float3 d = mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+    const int cell = guide_lookup(sc, p);
+    out_pdf[i] = sc.guide_qcells ? guide_pdf(quad_guide(sc, cell), d) : guide_pdf(sc.guide_cells[cell], d);
+  }
+}
+
 // ---- host-side launchers ------------------------------------------------------------------
 static int g_trace_blocks = 0;
 
@@ -1320,6 +1331,9 @@ void LaunchQuadSample(const QuadNode *nodes, int root, float w_root, int64_t n, 
 }
 void LaunchQuadPdf(const QuadNode *nodes, int root, float w_root, int64_t n, const float *dirs, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_quad_pdf, dim3(Blocks(n, 256, 4096)), dim3(256), 0, st, QuadGuide{nodes, root, w_root}, n, dirs, p);
+}
+void LaunchGuidePdfAt(const DeviceScene &sc, int64_t n, const float *pos, const float *dirs, float *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_guide_pdf_at, dim3(Blocks(n, 256, 4096)), dim3(256), 0, st, sc, n, pos, dirs, p);
 }
 void LaunchGuidePdf(const GuideCell &c, int64_t n, const float *dirs, float *p, hipStream_t st) {
   hipLaunchKernelGGL(k_guide_pdf, dim3(Blocks(n, 256, 8192)), dim3(256), 0, st, c, n, dirs, p);

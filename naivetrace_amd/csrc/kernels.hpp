@@ -33,6 +33,7 @@ void LaunchGuideSample(const GuideCell &c, uint64_t seed, int64_t n, float *d, f
 void LaunchQuadSample(const QuadNode *nodes, int root, float w_root, int64_t n, const float *u, float *uv, float *d, float *p,
                       hipStream_t st);
 void LaunchQuadPdf(const QuadNode *nodes, int root, float w_root, int64_t n, const float *dirs, float *p, hipStream_t st);
+void LaunchGuidePdfAt(const DeviceScene &sc, int64_t n, const float *pos, const float *dirs, float *p, hipStream_t st);
 void LaunchGuidePdf(const GuideCell &c, int64_t n, const float *dirs, float *p, hipStream_t st);
 void LaunchSamplePhase(int kind, float g, float3 wi, uint64_t seed, int64_t n, float *d, float *v, float *p, hipStream_t st);
 

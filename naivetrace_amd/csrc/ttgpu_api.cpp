@@ -1095,8 +1095,9 @@ int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_han
       h->net = std::make_unique<nn::NeuralGuide>(nn::NeuralGuideConfig{}, flo, fhi, params->seed);
     } else if (h->guided) {
       const int subdiv = params->guide_subdiv_factor > 0 ? params->guide_subdiv_factor : 100;
-      h->recorder = std::make_unique<guide::Recorder>(subdiv, params->seed, params->num_threads,
-                                                      params->guide_kind == TTGPU_GUIDE_QUADTREE);
+      // The quadtree is the reference's active directional model, and the default.
+      const bool quadtree = params->guide_kind == TTGPU_GUIDE_QUADTREE || params->guide_kind == TTGPU_GUIDE_NONE;
+      h->recorder = std::make_unique<guide::Recorder>(subdiv, params->seed, params->num_threads, quadtree);
       UploadGuide(h);
     }
     auto t1 = std::chrono::steady_clock::now();
@@ -1543,6 +1544,50 @@ int ttgpu_guide_info(ttgpu_handle *h, int32_t *cells, int32_t *rounds, int64_t *
   return TTGPU_OK;
 }
 
+namespace {
+// Device scratch of `count` floats per column, freed on scope exit.
+struct DeviceFloats {
+  std::vector<float *> p;
+  ~DeviceFloats() {
+    for (float *x : p)
+      if (x) (void)hipFree(x);
+  }
+  float *Get(size_t n) {
+    float *x = nullptr;
+    Check(hipMalloc(&x, std::max<size_t>(1, n) * sizeof(float)), "hipMalloc");
+    p.push_back(x);
+    return x;
+  }
+};
+}  // namespace
+
+int ttgpu_guide_pdf(ttgpu_handle *h, int64_t count, const float *pos, const float *dirs, int32_t on_device, float *out_pdf) {
+  if (!h || count < 0 || (count > 0 && (!pos || !dirs || !out_pdf))) return TTGPU_ERR_INVALID_ARGUMENT;
+  if (!h->guided || h->learned)
+    return Fail(h, TTGPU_ERR_INVALID_ARGUMENT, "not a vMF-tree or quadtree guided handle");
+  TTGPU_GUARD(h, {
+    if (on_device) {
+      if (count == 0) return TTGPU_OK;
+      DeviceFloats f;
+      float *dp = f.Get(3 * count), *dd = f.Get(3 * count), *out = f.Get(count);
+      Check(hipMemcpy(dp, pos, 3 * count * sizeof(float), hipMemcpyHostToDevice), "upload");
+      Check(hipMemcpy(dd, dirs, 3 * count * sizeof(float), hipMemcpyHostToDevice), "upload");
+      LaunchGuidePdfAt(h->dsc, count, dp, dd, out, h->stream);
+      Check(hipGetLastError(), "launch");
+      Check(hipStreamSynchronize(h->stream), "sync");
+      Check(hipMemcpy(out_pdf, out, count * sizeof(float), hipMemcpyDeviceToHost), "download");
+      return TTGPU_OK;
+    }
+    const guide::Recorder &rec = *h->recorder;
+    for (int64_t i = 0; i < count; ++i) {
+      const double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+      const guide::CellData &c = rec.cells()[rec.tree().Lookup(p)];
+      out_pdf[i] = rec.quadtree() ? (float)c.qcurrent.Pdf(dirs + 3 * i) : guide::Pdf(c.current, dirs + 3 * i);
+    }
+    return TTGPU_OK;
+  })
+}
+
 static void ToVmf(const ttgpu_vmf8 &a, guide::Vmf &m) {
   for (int k = 0; k < guide::kLobes; ++k) {
     for (int c = 0; c < 3; ++c) m.mu[k][c] = a.mu[k][c];
@@ -1689,6 +1734,28 @@ static void TreeToArrays(const guide::quad::Tree &t, int32_t *children) {
     for (int k = 0; k < 4; ++k) children[4 * (size_t)n + k] = t.child[n][k];
 }
 
+namespace {
+// Device copy of one quadtree for the sampling and pdf entry points.
+struct DeviceQuad {
+  QuadNode *nodes = nullptr;
+  QuadCell cell{-1, 1.f};
+  ~DeviceQuad() {
+    if (nodes) (void)hipFree(nodes);
+  }
+};
+void UploadQuad(const int32_t *children, const float *probs, int32_t num_nodes, DeviceQuad &d) {
+  guide::quad::Tree t;
+  TreeFromArrays(children, num_nodes, t);
+  if (!probs) throw std::invalid_argument("null weights");
+  std::vector<float> w(probs, probs + num_nodes);
+  std::vector<QuadNode> nodes;
+  d.cell = FlattenQuad(t, w, nodes);
+  nodes.resize(std::max<size_t>(1, nodes.size()));
+  Check(hipMalloc(&d.nodes, nodes.size() * sizeof(QuadNode)), "hipMalloc");
+  Check(hipMemcpy(d.nodes, nodes.data(), nodes.size() * sizeof(QuadNode), hipMemcpyHostToDevice), "upload");
+}
+}  // namespace
+
 int ttgpu_quadtree_build(const float *uv, const float *weights, int64_t n, float fraction, int32_t capacity,
                          int32_t *children, float *node_weights, int32_t *num_nodes) {
   if (n < 0 || (n > 0 && (!uv || !weights)) || !children || !node_weights || !num_nodes) return TTGPU_ERR_INVALID_ARGUMENT;
@@ -1728,41 +1795,6 @@ int ttgpu_quadtree_adapt(const int32_t *children, const float *weights, int32_t 
   })
 }
 
-namespace {
-// Device copy of one quadtree for the sampling and pdf entry points.
-struct DeviceQuad {
-  QuadNode *nodes = nullptr;
-  QuadCell cell{-1, 1.f};
-  ~DeviceQuad() {
-    if (nodes) (void)hipFree(nodes);
-  }
-};
-void UploadQuad(const int32_t *children, const float *probs, int32_t num_nodes, DeviceQuad &d) {
-  guide::quad::Tree t;
-  TreeFromArrays(children, num_nodes, t);
-  if (!probs) throw std::invalid_argument("null weights");
-  std::vector<float> w(probs, probs + num_nodes);
-  std::vector<QuadNode> nodes;
-  d.cell = FlattenQuad(t, w, nodes);
-  nodes.resize(std::max<size_t>(1, nodes.size()));
-  Check(hipMalloc(&d.nodes, nodes.size() * sizeof(QuadNode)), "hipMalloc");
-  Check(hipMemcpy(d.nodes, nodes.data(), nodes.size() * sizeof(QuadNode), hipMemcpyHostToDevice), "upload");
-}
-// Device scratch of `count` floats per column, freed on scope exit.
-struct DeviceFloats {
-  std::vector<float *> p;
-  ~DeviceFloats() {
-    for (float *x : p)
-      if (x) (void)hipFree(x);
-  }
-  float *Get(size_t n) {
-    float *x = nullptr;
-    Check(hipMalloc(&x, std::max<size_t>(1, n) * sizeof(float)), "hipMalloc");
-    p.push_back(x);
-    return x;
-  }
-};
-}  // namespace
 
 int ttgpu_quadtree_sample(const int32_t *children, const float *probs, int32_t num_nodes, int64_t count, const float *u,
                           float *out_uv, float *out_dir, float *out_pdf) {

@@ -121,6 +121,17 @@ class GpuRenderer:
         self._check(self._lib.ttgpu_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def guide_pdf(self, pos, dirs, on_device=False):
+        """Density of the current guide at positions `pos` [n, 3] for unit directions `dirs` [n, 3]:
+        the recorder's host copy, or the copy the GPU kernels sample (on_device)."""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros(len(d), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self._lib.ttgpu_guide_pdf(self._h, len(d), p.ctypes.data_as(fp), d.ctypes.data_as(fp),
+                                              1 if on_device else 0, out.ctypes.data_as(fp)))
+        return out
+
     def guide_info(self):
         """Guided handles: kd-tree cells, training rounds, records of the last pass, dropped records."""
         cells, rounds = C.c_int32(), C.c_int32()

@@ -7,8 +7,9 @@
 * Unbiasedness of ptg with the quadtree guide: per-pixel 3-sigma parity with the
   oracle's converged Cornell render.
 * Training: the per-pass refits and per-sweep adaptations run (more than one cell,
-  records every pass), and on the Cornell box with a hidden light the guided render
-  has lower relMSE than pt2 at equal samples per pixel.
+  records every pass); the trained quadtree of floor cells concentrates on the
+  ceiling light; the uploaded copy of the guide (quadtree and vMF) has the host
+  recorder's density.
 """
 import os
 
@@ -19,7 +20,6 @@ from naivetrace_amd import abi, configs
 from naivetrace_amd.renderer import GpuRenderer, quadtree_pdf_gpu, quadtree_sample_gpu
 import oracle_lib as O
 import stats_util as S
-from test_gpu_guiding import _relmse
 from test_gpu_parity import GOLDEN, _zscores
 from test_quadtree import _leaves, _random_tree, _regions, _unit
 
@@ -43,7 +43,9 @@ def test_gpu_quadtree_warp_and_pdf_match_oracle(gpu_lib, oracle, seed):
     same = np.all(np.abs(uv - ouv) < 2e-5, axis=1)
     print(f"  warped points equal to the oracle's: {same.mean():.6f}", flush=True)
     assert same.mean() > 0.999
-    assert np.allclose(d, O.qt_square_to_sphere(uv.astype(np.float64)), atol=3e-6)
+    err = np.abs(d - O.qt_square_to_sphere(uv.astype(np.float64))).max()
+    print(f"  max |direction - oracle's map of the point| {err:.2e}", flush=True)
+    assert err < 2e-6
     # density of the sampled direction: the oracle's density at the same point (away from leaf edges)
     np.testing.assert_allclose(pdf[same], O.qt_pdf(ch, p32, O.qt_sphere_to_square(d[same])) * INV_4PI, rtol=2e-4)
     # leaf frequencies
@@ -121,24 +123,56 @@ def test_gpu_ptg_quadtree_unbiased_cornell(gpu_lib):
     assert np.all(np.abs(img - img_ref) < 4 * img_se + 1e-3 * np.abs(img_ref))
 
 
-def test_gpu_ptg_quadtree_reduces_error_hidden_light(gpu_lib):
-    w = h = 64
-    s = configs.cornell_hidden_light(w, h)
-    pref = abi.default_params(w, h, max_depth=8, seed=99)
-    pref.mis_mode = abi.MIS_CONSISTENT
-    with GpuRenderer(s, pref) as r:
-        r.render_spp(32768)
-        ref = r.mean_image()
-    spp = 64
-    pp = abi.default_params(w, h, spp=spp, max_depth=8, seed=5)
-    pp.mis_mode = abi.MIS_CONSISTENT
-    with GpuRenderer(s, pp) as r:
-        r.run()
-        plain = r.mean_image()
-    with GpuRenderer(s, _qparams(w, h, 8, 64, spp, seed=6)) as r:
-        r.run()
-        guided = r.mean_image()
-    e_plain, e_guided = _relmse(plain, ref), _relmse(guided, ref)
-    print(f"hidden light {spp} spp: relMSE plain {e_plain:.5f} quadtree {e_guided:.5f} ratio {e_plain / e_guided:.2f}",
-          flush=True)
-    assert e_guided < e_plain
+def test_gpu_ptg_quadtree_learns_incident_light(gpu_lib):
+    """After training on the Cornell box, the quadtree of floor cells puts far more probability
+    toward the ceiling light than a cosine lobe would, little below the floor, and the copy the
+    GPU samples has the same density as the recorder's host copy."""
+    s = configs.cornell_box(64, 64)
+    rng = np.random.default_rng(9)
+    n = 256
+    zc = (np.arange(n) + 0.5) / n * 2 - 1
+    phi = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * np.pi
+    Z, P = np.meshgrid(zc, phi, indexing="ij")
+    rr = np.sqrt(1 - Z ** 2)
+    D = np.stack([rr * np.cos(P), rr * np.sin(P), Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    dA = (2.0 / n) * (2 * np.pi / (2 * n))
+    light = np.array([0.0, 1.995, 0.0])
+    p = _qparams(64, 64, 8, 64, 1, seed=6)
+    p.guide_kind = abi.GUIDE_NONE  # the default guide of ptg is the quadtree
+    with GpuRenderer(s, p) as r:
+        assert r.run() == 0
+        gains = []
+        for pos in ([0.5, 0.001, 0.8], [-0.6, 0.001, 0.7], [0.8, 0.001, -0.6]):
+            pdf = r.guide_pdf(np.repeat(np.array([pos], np.float32), len(D), 0), D)
+            assert pdf.sum() * dA == pytest.approx(1.0, abs=2e-3)
+            tl = light - np.array(pos)
+            tl /= np.linalg.norm(tl)
+            cone = D @ tl > 0.98
+            mass = pdf[cone].sum() * dA
+            cos_mass = (np.maximum(D[cone, 1], 0) / np.pi).sum() * dA
+            below = pdf[D[:, 1] < 0].sum() * dA
+            print(f"floor {pos}: guide mass toward the light {mass:.4f} (cosine lobe {cos_mass:.4f}), "
+                  f"below the floor {below:.4f}", flush=True)
+            gains.append(mass / cos_mass)
+            assert below < 0.35
+        assert np.median(gains) > 4.0
+        # host copy vs uploaded copy at random positions in the room and random directions
+        pos = (rng.random((50_000, 3)) * [2, 2, 2] - [1, 0, 1]).astype(np.float32)
+        dirs = _unit(rng, 50_000).astype(np.float32)
+        host = r.guide_pdf(pos, dirs)
+        dev = r.guide_pdf(pos, dirs, on_device=True)
+        agree = np.isclose(dev, host, rtol=2e-4, atol=1e-7)
+        print(f"device / host guide density agreement {agree.mean():.6f}", flush=True)
+        assert agree.mean() > 0.999
+
+
+def test_gpu_ptg_vmf_guide_upload_matches_host(gpu_lib):
+    s = configs.cornell_box(32, 32)
+    p = _qparams(32, 32, 8, 8, 1, seed=4)
+    p.guide_kind = abi.GUIDE_VMF_TREE
+    rng = np.random.default_rng(10)
+    with GpuRenderer(s, p) as r:
+        assert r.run() == 0
+        pos = (rng.random((20_000, 3)) * [2, 2, 2] - [1, 0, 1]).astype(np.float32)
+        dirs = _unit(rng, 20_000).astype(np.float32)
+        np.testing.assert_allclose(r.guide_pdf(pos, dirs, on_device=True), r.guide_pdf(pos, dirs), rtol=2e-4, atol=1e-7)

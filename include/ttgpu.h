@@ -202,7 +202,12 @@ typedef struct ttgpu_scene {
 } ttgpu_scene;
 
 /* ---- parameters (src/scene.hxx:24-43) ---------------------------------- */
-enum ttgpu_algo { TTGPU_ALGO_PT2 = 0, TTGPU_ALGO_PTG = 1 };
+/* PT2: spectral null-path tracing with NEE (pt2); PTG: guided (ptg); PT: the
+ * forward path tracer of the random-walk framework (pt, the reference's default
+ * --algo): power-heuristic MIS, Russian roulette beyond 5 nodes, light points
+ * connected to the camera and splatted to the pixel they project to; surface
+ * scenes only. */
+enum ttgpu_algo { TTGPU_ALGO_PT2 = 0, TTGPU_ALGO_PTG = 1, TTGPU_ALGO_PT = 2 };
 enum ttgpu_sample_mode {     /* --pt-sample-mode */
   TTGPU_SAMPLE_BOTH = 0, TTGPU_SAMPLE_BSDF = 1, TTGPU_SAMPLE_LIGHTS = 2
 };

@@ -973,6 +973,239 @@ template <int NW> struct Engine {
     return MaybeScatter(ia, ps, smp);
   }
 
+  // ---- pt: the forward path tracer on the random-walk framework -------------------------
+  // PathTracingWorker::RenderPixel (src/renderingalgorithms_pathtracing.hxx:731-801) with
+  // RadianceEstimatorBase (src/rendering_randomwalk_impl.hxx:228-806), surfaces in vacuum.
+
+  static double Power(double a, double b) { return a * a / (a * a + b * b); }  // PowerHeuristic
+
+  // Pinhole PixelPdfWrtSolidAngle (camera.hxx:88-96) at image-plane point (x, y).
+  double CameraPdf(double x, double y) const {
+    const double l = std::sqrt(x * x + y * y + 1.);
+    return l * l * l / (xper * yper);
+  }
+
+  // PerspectiveCamera::Evaluate (camera.hxx:122-148): pixel hit by `dir` from the camera and the
+  // direction pdf; -1 when behind the camera or outside the image.  With TTGPU_MIS_REFERENCE the
+  // pixel index truncates toward zero as the reference's static_cast does, so directions up to one
+  // pixel beyond the left and bottom edges land in the first column and row; TTGPU_MIS_CONSISTENT
+  // rounds down.
+  int CameraEvaluate(V3 dir, double &pdf) const {
+    pdf = 0.;
+    const double z = Dot(dir, cam_frame.c2);
+    if (!(z > 0.)) return -1;
+    const double x = Dot(dir, cam_frame.c0) / z, y = Dot(dir, cam_frame.c1) / z;
+    const double fx = (x - xmin) / xper, fy = (y - ymin) / yper;
+    const bool trunc = prm.mis_mode != TTGPU_MIS_CONSISTENT;
+    const int pxi = trunc ? (int)fx : (int)std::floor(fx), pyi = trunc ? (int)fy : (int)std::floor(fy);
+    if (pxi < 0 || pxi >= W || pyi < 0 || pyi >= H) return -1;
+    pdf = CameraPdf(x, y);
+    return pyi * W + pxi;
+  }
+
+  // SampleEmissiveEnd + the emitter side of CalculateConnection: direction and distance from
+  // `here` to a sampled light point, value = radiance x |n_light . dir| / (distance^2 pdf_area
+  // pick) (radiance / (pdf pick) for the environment, intensity / (distance^2 pick) for points),
+  // and the light pdf over solid angle including the pick probability.
+  bool SampleLightPt(V3 here, const PathContext<NW> &ctx, Sampler &smp, V3 &dir, double &len, Spec<NW> &value,
+                     double &pdf_sa, bool &delta) const {
+    if (counts[0] + counts[1] + counts[2] + counts[3] == 0) return false;
+    const int kind = TowerSampling<4>(type_probs, smp.Uniform01());
+    const double u2 = smp.Uniform01();
+    delta = false;
+    auto toward = [&](V3 target) {
+      V3 d = target - here;
+      len = Length(d);
+      if (len > 0.) dir = d / len;
+      else { len = Epsilon; dir = V3{1., 0., 0.}; }
+    };
+    if (kind == IDX_AREA) {
+      const long li = TowerSamplingBisection(area_cmf.data(), counts[IDX_AREA], u2);
+      const double prob = type_probs[IDX_AREA] * ProbabilityFromCmf(area_cmf.data(), li);
+      const int prim = sc.area_lights[li];
+      double r[2];
+      smp.UniformUnitSquare(r);
+      Hit lh = sc.SampleUniformPosition(prim, r);
+      V3 lpos, lgn, lsn, lpb;
+      sc.LocalGeometry(lh, lpos, lgn, lsn, lpb);
+      toward(lpos);
+      const auto &em = sc.emitters[sc.materials[sc.prims[prim].material].emitter];
+      const double area_pdf = 1. / sc.PrimArea(prim);
+      const double cos_l = std::abs(Dot(lgn, dir));  // DFactorOf at the light: geometry normal
+      Spec<NW> L = SpecConst<NW>(0.);
+      if (em.kind == TTGPU_AREA_UNIFORM && Dot(lgn, -dir) > 0.)
+        L = (1. / UnitHalfSphereSurfaceArea) * Take<NW>(em.spectrum, ctx);
+      value = (cos_l / (Sqr(len) * area_pdf * prob)) * L;
+      pdf_sa = area_pdf * prob * AreaToSolidAngle(len, dir, lgn);
+      return true;
+    }
+    if (kind == IDX_ENV) {
+      const double prob = type_probs[IDX_ENV];
+      V3 dir_out;
+      Spec<NW> val;
+      Pdf pdf;
+      EnvSample(smp, ctx, dir_out, val, pdf);
+      dir = -dir_out;
+      len = 10. * bb_diag;
+      value = (1. / (pdf.val() * prob)) * val;
+      pdf_sa = pdf.val() * prob;
+      delta = pdf.IsDelta();
+      return true;
+    }
+    if (kind == IDX_POINT) {
+      const long li = TowerSamplingBisection(point_cmf.data(), counts[IDX_POINT], u2);
+      const double prob = type_probs[IDX_POINT] * ProbabilityFromCmf(point_cmf.data(), li);
+      const auto &pl = sc.points[li];
+      toward(V3{pl.pos[0], pl.pos[1], pl.pos[2]});
+      value = (1. / (Sqr(len) * prob * UnitSphereSurfaceArea)) * Take<NW>(pl.spectrum, ctx);
+      pdf_sa = 1.;
+      delta = true;
+      return true;
+    }
+    return false;
+  }
+
+  void RecordTo(const Spec<NW> &m, const PathContext<NW> &ctx, double out[3]) const {
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < NW; ++j) out[c] += m[j] * kSpecToRGB[c][ctx.bins[j]];
+  }
+
+  // One camera sample of pt: the path's own rgb, and the rgb (already divided by the number of
+  // pixels) splatted to *splat_pixel by the light connection at the camera node (-1: none).
+  void RenderPathPt(int px, int py, uint64_t sample_index, double rgb[3], int *splat_pixel, double splat[3]) const {
+    Sampler smp;
+    smp.rng.Seed(prm.seed, sample_index);
+    rgb[0] = rgb[1] = rgb[2] = 0.;
+    splat[0] = splat[1] = splat[2] = 0.;
+    *splat_pixel = -1;
+    PathState ps;
+    // Wavelengths and the camera ray as pt2's InitializePath; pt keeps the camera's direction pdf.
+    constexpr int strata = NBINS / NW;
+    int main_idx = smp.UniformInt(0, strata - 1);
+    int idx[NW];
+    for (int i = 0; i < NW; ++i) idx[i] = main_idx + i * strata;
+    std::swap(idx[0], idx[smp.UniformInt(0, NW - 1)]);
+    for (int i = 0; i < NW; ++i) {
+      ps.ctx.bins[i] = idx[i];
+      double c = kBinCenters[idx[i]], hw = 0.5 * (LambdaMax - LambdaMin) / NBINS;
+      ps.ctx.wavelengths[i] = (c - hw) + smp.Uniform01() * (2. * hw);
+    }
+    Spec<NW> beta = SpecConst<NW>((double)strata);
+    const bool mis = use_nee && use_emission;
+    const bool pinhole = sc.camera.kind == TTGPU_CAMERA_PINHOLE;
+    // NEE at the camera node: a light point connected to the camera, splatted to the pixel it
+    // projects to (renderingalgorithms_pathtracing.hxx:752-779).
+    if (use_nee && pinhole) {
+      V3 dir;
+      double len, pdf_light;
+      Spec<NW> value;
+      bool delta;
+      if (SampleLightPt(cam_pos, ps.ctx, smp, dir, len, value, pdf_light, delta)) {
+        double pdf_cam;
+        const int pix = CameraEvaluate(dir, pdf_cam);
+        if (pix >= 0 && !sc.IsOccluded(cam_pos, dir, 0., len * 0.9999)) {
+          const double w = (!delta && mis) ? Power(pdf_light, pdf_cam) : 1.;
+          Spec<NW> m;
+          for (int k = 0; k < NW; ++k) m[k] = w * pdf_cam * value[k] * beta[k] / (double)(W * H);
+          RecordTo(m, ps.ctx, splat);
+          *splat_pixel = pix;
+        }
+      }
+    }
+    double r[2];
+    smp.UniformUnitSquare(r);
+    V3 org = cam_pos, dir;
+    Pdf last;
+    if (pinhole) {
+      const double x = xmin + (r[0] + px) * xper, y = ymin + (r[1] + py) * yper;
+      dir = cam_frame * Normalized(V3{x, y, 1.0});
+      last = Pdf(CameraPdf(x, y));
+    } else {
+      double u = (r[0] + px) * fish_delta - 1.0, vv = (r[1] + py) * fish_delta - 1.0;
+      double rho = u * u + vv * vv;
+      V3 w{2. * u, 2. * vv, rho - 1.};
+      w = w * (1. / (1. + rho));
+      w.z = -w.z;
+      if (!(rho <= 1.)) return;
+      dir = cam_frame * w;
+      last = Pdf::Delta(1.);  // the fisheye's direction sample is flagged as a pmf
+    }
+    int node_count = 1;
+    for (;;) {
+      Hit hit;
+      const bool has_hit = sc.FirstIntersection(org, dir, 0., LargeNumber, hit);
+      ++node_count;
+      if (!has_hit) {
+        if (use_emission && !sc.envs.empty()) {
+          const Spec<NW> L = EnvEval(-dir, ps.ctx);
+          const double w = (!last.IsDelta() && mis) ? Power(last.val(), EnvPdf(-dir) * PmfEnv()) : 1.;
+          Spec<NW> m;
+          for (int k = 0; k < NW; ++k) m[k] = w * L[k] * beta[k];
+          RecordTo(m, ps.ctx, rgb);
+        }
+        return;
+      }
+      const SurfaceInteraction si = MakeSurfaceInteraction(hit, dir);
+      const auto &mat = sc.materials[si.material];
+      if (use_emission && mat.emitter >= 0) {
+        const auto &em = sc.emitters[mat.emitter];
+        if (em.kind == TTGPU_AREA_UNIFORM && Dot(si.geometry_normal, -dir) > 0.) {
+          const Spec<NW> L = (1. / UnitHalfSphereSurfaceArea) * Take<NW>(em.spectrum, ps.ctx);
+          double w = 1.;
+          if (!last.IsDelta() && mis) {
+            const double p_direct = PmfArea() / sc.PrimArea(hit.prim) * AreaToSolidAngle(hit.t, dir, si.normal);
+            w = Power(last.val(), p_direct);
+          }
+          Spec<NW> m;
+          for (int k = 0; k < NW; ++k) m[k] = w * L[k] * beta[k];
+          RecordTo(m, ps.ctx, rgb);
+        }
+      }
+      // SurvivalAtNthScatterNode (rendering_randomwalk_impl.hxx:729-756): at most max_ray_depth + 1
+      // nodes; Russian roulette with survival probability min(0.9, max beta) beyond 5 nodes.
+      if (node_count >= prm.max_ray_depth + 1) return;
+      if (node_count > 5) {
+        double mx = 0.;
+        for (int k = 0; k < NW; ++k) mx = std::max(mx, beta[k]);
+        const double p_survive = std::min(0.9, mx);
+        if (smp.Uniform01() > p_survive) return;
+        beta = (1. / p_survive) * beta;
+      }
+      const auto &shd = sc.shaders[mat.shader];
+      if (use_nee) {
+        V3 ldir;
+        double len, pdf_light;
+        Spec<NW> value;
+        bool delta;
+        if (SampleLightPt(si.pos, ps.ctx, smp, ldir, len, value, pdf_light, delta)) {
+          double bsdf_pdf = 0.;
+          const Spec<NW> f = sh.Evaluate(shd, -dir, si, ps.ctx, 0., ldir, &bsdf_pdf);
+          const V3 o = si.pos + AntiSelfIntersectionOffset(si, ldir);
+          bool nonzero = false;
+          for (int k = 0; k < NW; ++k) nonzero |= f[k] != 0. && value[k] != 0.;
+          if (nonzero && !sc.IsOccluded(o, ldir, 0., len * 0.9999)) {
+            const double w = (!delta && mis) ? Power(pdf_light, bsdf_pdf) : 1.;
+            const double d = DFactorPBRT(si, ldir);
+            Spec<NW> m;
+            for (int k = 0; k < NW; ++k) m[k] = w * d * f[k] * value[k] * beta[k];
+            RecordTo(m, ps.ctx, rgb);
+          }
+        }
+      }
+      const ScatterSample<NW> s = sh.Sample(shd, -dir, si, ps.ctx, 0., smp);
+      const double d = DFactorPBRT(si, s.dir);
+      bool zero = true;
+      for (int k = 0; k < NW; ++k) {
+        beta[k] *= s.value[k] * d / s.pdf.val();
+        zero &= beta[k] == 0.;
+      }
+      if (zero || !(s.pdf.val() > 0.)) return;
+      last = s.pdf;
+      dir = s.dir;
+      org = si.pos + AntiSelfIntersectionOffset(si, s.dir);
+    }
+  }
+
   void RenderPath(int px, int py, uint64_t sample_index, double rgb[3]) const {
     Sampler smp;
     smp.rng.Seed(prm.seed, sample_index);
@@ -1006,8 +1239,16 @@ static void RenderImpl(const Scene &sc, const ttgpu_params &p, int spp, int spp_
   Engine<NW> eng(sc, p);
   const int W = p.width, H = p.height;
   const int64_t npix = (int64_t)W * H;
+  const bool pt = p.algo == TTGPU_ALGO_PT;
   std::atomic<int> next_row{0};
-  auto worker = [&]() {
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  // pt's light connections at the camera land on other pixels: per-thread splat buffers.
+  std::vector<std::vector<double>> splat(pt ? threads : 0), splat_sq(pt ? threads : 0);
+  auto worker = [&](int tid) {
+    if (pt) {
+      splat[tid].assign(3 * npix, 0.);
+      splat_sq[tid].assign(3 * npix, 0.);
+    }
     for (;;) {
       int y = next_row.fetch_add(1);
       if (y >= H) break;
@@ -1016,7 +1257,18 @@ static void RenderImpl(const Scene &sc, const ttgpu_params &p, int spp, int spp_
         for (int s = 0; s < spp; ++s) {
           uint64_t gidx = (uint64_t)(s + spp_offset) * (uint64_t)npix + (uint64_t)(y * W + x);
           double c[3];
-          eng.RenderPath(x, y, gidx, c);
+          if (pt) {
+            double sp[3];
+            int sp_pix;
+            eng.RenderPathPt(x, y, gidx, c, &sp_pix, sp);
+            if (sp_pix >= 0)
+              for (int k = 0; k < 3; ++k) {
+                splat[tid][3 * sp_pix + k] += sp[k];
+                splat_sq[tid][3 * sp_pix + k] += sp[k] * sp[k];
+              }
+          } else {
+            eng.RenderPath(x, y, gidx, c);
+          }
           for (int k = 0; k < 3; ++k) { acc[k] += c[k]; acc2[k] += c[k] * c[k]; }
         }
         int64_t pi = (int64_t)y * W + x;
@@ -1027,10 +1279,14 @@ static void RenderImpl(const Scene &sc, const ttgpu_params &p, int spp, int spp_
       }
     }
   };
-  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
   std::vector<std::thread> ts;
-  for (int i = 0; i < threads; ++i) ts.emplace_back(worker);
+  for (int i = 0; i < threads; ++i) ts.emplace_back(worker, i);
   for (auto &t : ts) t.join();
+  for (int t = 0; t < (int)splat.size(); ++t)
+    for (int64_t i = 0; i < 3 * npix; ++i) {
+      rgb[i] += splat[t][i];
+      if (rgb_sq) rgb_sq[i] += splat_sq[t][i];
+    }
 }
 
 extern "C" int ttref_render(const ttgpu_scene *scene, const ttgpu_params *params, int32_t spp, int32_t spp_offset,
@@ -1039,6 +1295,8 @@ extern "C" int ttref_render(const ttgpu_scene *scene, const ttgpu_params *params
     if (!scene || !params || !rgb || spp < 0) throw std::invalid_argument("null argument");
     if (params->width <= 0 || params->height <= 0) throw std::invalid_argument("bad resolution");
     Scene sc(*scene);
+    if (params->algo == TTGPU_ALGO_PT && !sc.media.empty() && (scene->empty_space_medium >= 0 || !sc.volumes.empty()))
+      throw std::invalid_argument("pt runs on surface scenes (no media)");
     const auto t0 = std::chrono::steady_clock::now();
     if (params->num_wavelengths == 3) RenderImpl<3>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);
     else if (params->num_wavelengths == 4) RenderImpl<4>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);

@@ -166,6 +166,7 @@ struct DeviceScene {
   uint64_t cam_tracker;            // packed medium tracker at the camera
   DevCamera cam;
   int sample_mode, mis_mode, max_depth;
+  int algo_pt;                     // 1: pt semantics (power heuristic, Russian roulette, camera connections)
   int persistent;                  // 1: persistent traversal kernels with dynamic ray fetch
   // guiding (guided integrator only)
   const GuideNode *guide_nodes;

@@ -48,10 +48,20 @@ __global__ void k_fill_f64(double *p, size_t n, double v) {
 }
 
 // ---- camera (src/camera.hxx:105-124, 176-210) ----------------------------
-__device__ __forceinline__ bool camera_ray(const DevCamera &c, int px, int py, float r0, float r1, float3 &dir) {
+// Pinhole direction pdf over solid angle at image-plane point (x, y): |(x, y, 1)|^3 / pixel area
+// (PixelPdfWrtSolidAngle, camera.hxx:88-96).
+__device__ __forceinline__ float camera_pdf(const DevCamera &c, float x, float y) {
+  const float l = sqrtf(x * x + y * y + 1.f);
+  return l * l * l / (c.xper * c.yper);
+}
+
+__device__ __forceinline__ bool camera_ray(const DevCamera &c, int px, int py, float r0, float r1, float3 &dir,
+                                           float *pdf = nullptr) {
   if (c.kind == TTGPU_CAMERA_PINHOLE) {
-    float3 v = normalize(mk(c.xmin + (r0 + px) * c.xper, c.ymin + (r1 + py) * c.yper, 1.f));
+    const float x = c.xmin + (r0 + px) * c.xper, y = c.ymin + (r1 + py) * c.yper;
+    float3 v = normalize(mk(x, y, 1.f));
     dir = c.right * v.x + c.up * v.y + c.dir * v.z;
+    if (pdf) *pdf = camera_pdf(c, x, y);
     return true;
   }
   float u = (r0 + px) * c.fish_delta - 1.f, v = (r1 + py) * c.fish_delta - 1.f;
@@ -84,7 +94,8 @@ __device__ __forceinline__ void init_camera_path(const DeviceScene &sc, const Pa
   const int px = pixel % sc.cam.width, py = pixel / sc.cam.width;
   float r0 = rng.uniform(), r1 = rng.uniform();
   float3 dir;
-  bool ok = camera_ray(sc.cam, px, py, r0, r1, dir);
+  float cam_pdf = 0.f;
+  bool ok = camera_ray(sc.cam, px, py, r0, r1, dir, &cam_pdf);
   q.org[i] = make_float4(sc.cam.pos.x, sc.cam.pos.y, sc.cam.pos.z, lambda0);
   q.dir[i] = make_float4(dir.x, dir.y, dir.z, 0.f);
   q.thru[i] = ok ? f4((float)kStrata) : f4(0.f);
@@ -95,6 +106,12 @@ __device__ __forceinline__ void init_camera_path(const DeviceScene &sc, const Pa
   q.rng_inc[i] = make_uint2((uint32_t)rng.inc, (uint32_t)(rng.inc >> 32));
   q.flags[i] = 2u;  // node count 2: camera position and direction nodes (pt2.cxx:388-406)
   q.last_pdf[i] = 0.f;
+  if (sc.algo_pt) {
+    // pt weighs emission seen by camera rays against light connections to the camera with the
+    // camera's direction pdf; the fisheye's direction sample is flagged as a pmf (camera.hxx:191-209).
+    q.flags[i] = 2u | kFlagHasLastPdf | (sc.cam.kind == TTGPU_CAMERA_PINHOLE ? 0u : kFlagLastPdfDelta);
+    q.last_pdf[i] = cam_pdf;
+  }
   q.roughness[i] = 0.f;
   q.tracker[i] = make_uint2((uint32_t)sc.cam_tracker, (uint32_t)(sc.cam_tracker >> 32));
   if (q.wt_r) {
@@ -417,6 +434,9 @@ struct LightSample {
 };
 
 // `fallback_dir` is used when the sampled light point coincides with `here` (SegmentToPoint).
+// pt: the pdf always includes the selection probability and the cosine at an area light uses its
+// geometry normal (DFactorOf, rendering_randomwalk_impl.hxx:758-769).
+template <bool kPt = false>
 __device__ __forceinline__ LightSample sample_light(const DeviceScene &sc, float3 here, float3 fallback_dir, uint32_t bins, Pcg &rng) {
   LightSample ls;
   ls.dir = mk(0.f, 0.f, 1.f);
@@ -465,10 +485,10 @@ __device__ __forceinline__ LightSample sample_light(const DeviceScene &sc, float
     const DevEmitter &em = sc.emitters[sc.materials[sc.prim_material[lp]].emitter];
     float4 Le = (em.kind == TTGPU_AREA_UNIFORM && dot(lng, -ls.dir) > 0.f) ? take(em.radiance, bins) : f4(0.f);
     const float area_pdf = 1.f / sc.prim_area[lp];
-    const float dfac = fabsf(dot(lns, ls.dir));
+    const float dfac = fabsf(dot(kPt ? lng : lns, ls.dir));
     ls.weight = (dfac / (area_pdf * sc.area_pick_prob * ls.len * ls.len)) * Le;
     ls.pdf = area_to_solid_angle(ls.len, ls.dir, lng) * area_pdf;
-    if (sc.mis_mode == TTGPU_MIS_CONSISTENT) ls.pdf *= sc.area_pick_prob;
+    if (kPt || sc.mis_mode == TTGPU_MIS_CONSISTENT) ls.pdf *= sc.area_pick_prob;
   } else if (kind == 0) {
     float3 dout;
     float4 val;
@@ -477,7 +497,7 @@ __device__ __forceinline__ LightSample sample_light(const DeviceScene &sc, float
     ls.dir = -dout;
     ls.len = 10.f * sc.bb_diag;
     ls.weight = (1.f / (p * sc.type_probs[0])) * val;
-    ls.pdf = p * (sc.mis_mode == TTGPU_MIS_CONSISTENT ? sc.type_probs[0] : 1.f);
+    ls.pdf = p * ((kPt || sc.mis_mode == TTGPU_MIS_CONSISTENT) ? sc.type_probs[0] : 1.f);
   } else {
     const uint32_t li = min((uint32_t)(u_idx * sc.num_points), (uint32_t)sc.num_points - 1u);
     const DevPointLight &pl = sc.points[li];
@@ -493,7 +513,15 @@ __device__ __forceinline__ LightSample sample_light(const DeviceScene &sc, float
   return ls;
 }
 
+__device__ __forceinline__ float mis_weight(bool power, float a, float b) {
+  return power ? a * a / (a * a + b * b) : a / (a + b);
+}
+
 // ---- shade: emission, NEE, scattering (pt2.cxx:410-443, 479-556, 605-684) --------
+// kPt: pt's step instead (renderingalgorithms_pathtracing.hxx:749-794): power heuristic with the
+// selection probability in the light pdf, no mollification, and the survival test (at most
+// max_depth + 1 nodes, Russian roulette beyond 5) before NEE and scattering.
+template <bool kPt>
 __global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
                                                const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                uint32_t *count_out, double *fb, double *fb_sq) {
@@ -529,7 +557,7 @@ __global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc
           float penv;
           float4 L = env_eval(sc, -dir, bins, &penv);
           float w = 1.f;
-          if (has_last && !last_delta && mis) w = last_pdf / (last_pdf + penv * sc.type_probs[0]);
+          if (has_last && !last_delta && mis) w = mis_weight(kPt, last_pdf, penv * sc.type_probs[0]);
           rad = add_rgb(rad, spectral_to_rgb(w * (L * thru), bins));
         }
         finalize_path(q, fb, fb_sq, i, rad);
@@ -544,23 +572,34 @@ __global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc
             float w = 1.f;
             if (has_last && !last_delta && mis) {
               float p_light = sc.area_pick_prob / sc.prim_area[prim] * area_to_solid_angle(hitv.x, dir, h.n);
-              w = last_pdf / (last_pdf + p_light);
+              w = mis_weight(kPt, last_pdf, p_light);
             }
             rad = add_rgb(rad, spectral_to_rgb(w * (L * thru), bins));
           }
         }
         const DevShader &shd = sc.shaders[mat.shader];
-        float rough = q.roughness[i];
+        float rough = kPt ? 0.f : q.roughness[i];
         const float3 wi = -dir;
+        const uint32_t node_here = flags & kFlagNodeMask;
+        bool survive = true;
+        if constexpr (kPt) {
+          // SurvivalAtNthScatterNode (rendering_randomwalk_impl.hxx:729-756)
+          survive = node_here < (uint32_t)sc.max_depth + 1u;
+          if (survive && node_here > 5u) {
+            const float p_survive = fminf(0.9f, fmaxf(fmaxf(thru.x, thru.y), fmaxf(thru.z, thru.w)));
+            if (rng.uniform() > p_survive) survive = false;
+            else thru = thru * (1.f / p_survive);
+          }
+        }
         // AddDirectLighting (pt2.cxx:479-556), vacuum transmission.
-        if (use_nee && (sc.num_area_lights + sc.num_envs + sc.num_points) > 0) {
-          const LightSample ls = sample_light(sc, h.pos, h.n, bins, rng);
+        if (survive && use_nee && (sc.num_area_lights + sc.num_envs + sc.num_points) > 0) {
+          const LightSample ls = sample_light<kPt>(sc, h.pos, h.n, bins, rng);
           if (ls.valid) {
             float bsdf_pdf;
             float4 f = eval_bsdf(shd, h, wi, bins, rough, lambda0, ls.dir, &bsdf_pdf);
             const float dfac = fabsf(dot(h.ns, ls.dir));
             float w = 1.f;
-            if (!ls.delta && mis) w = ls.pdf / (ls.pdf + bsdf_pdf);
+            if (!ls.delta && mis) w = mis_weight(kPt, ls.pdf, bsdf_pdf);
             float4 c = (w * dfac) * (thru * f * ls.weight);
             float3 crgb = spectral_to_rgb(c, bins);
             if (crgb.x != 0.f || crgb.y != 0.f || crgb.z != 0.f) {
@@ -573,10 +612,11 @@ __global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc
           }
         }
         // MaybeScatter (pt2.cxx:605-622) and PrepareStateForAfterScattering (:583-591)
-        rough = fmaxf(my_roughness(shd), rough);
-        Scatter s = sample_bsdf(shd, h, wi, bins, rough, lambda0, rng);
-        uint32_t node = (flags & kFlagNodeMask) + 1u;
-        bool alive = node <= (uint32_t)sc.max_depth;
+        if constexpr (!kPt) rough = fmaxf(my_roughness(shd), rough);
+        Scatter s;
+        uint32_t node = node_here + 1u;
+        bool alive = kPt ? survive : node <= (uint32_t)sc.max_depth;
+        if (alive) s = sample_bsdf(shd, h, wi, bins, rough, lambda0, rng);
         if (alive) {
           thru = (fabsf(dot(h.ns, s.dir)) / s.pdf) * (thru * s.value);
           alive = !all_zero(thru) && isfinite(thru.x + thru.y + thru.z + thru.w);
@@ -626,6 +666,62 @@ __global__ void __launch_bounds__(kTraceBlock, TTGPU_TRACE_WAVES) k_shadow(Devic
     else q.rad[i] = rad;
   }
   if (gtid() == 0) atomicAdd(stats + kStatShadow, count);
+}
+
+// ---- pt: light points connected to the camera (renderingalgorithms_pathtracing.hxx:752-779) ----
+// One connection per camera sample g of the pass (its own RNG stream and wavelengths): a light
+// point, the pixel it projects to (PerspectiveCamera::Evaluate, camera.hxx:122-148) and, when the
+// segment is unoccluded, MIS weight x camera importance (= direction pdf) x light value x
+// wavelength weights / pixel count added to that pixel.  With TTGPU_MIS_REFERENCE the pixel index
+// truncates toward zero as the reference does.
+__global__ void __launch_bounds__(kTraceBlock) k_camera_connect(DeviceScene sc, uint64_t first, uint64_t count, uint32_t npix,
+                                                                 uint32_t spp_base, uint64_t seed, double *fb, double *fb_sq,
+                                                                 uint32_t *stats) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  const bool mis = sc.sample_mode == TTGPU_SAMPLE_BOTH;
+  const DevCamera &c = sc.cam;
+  uint32_t rays = 0;
+  for (uint64_t g = first + gtid(); g < first + count; g += grid_stride()) {
+    const uint32_t pixel = (uint32_t)(g % npix);
+    const uint64_t sample = spp_base + g / npix;
+    Pcg rng;
+    rng.seed(seed ^ 0xC2B2AE3D27D4EB4Full, sample * npix + pixel);
+    const int main_idx = (int)rng.bounded(kStrata);
+    int idx[4] = {main_idx, main_idx + kStrata, main_idx + 2 * kStrata, main_idx + 3 * kStrata};
+    const int rot = (int)rng.bounded(4);
+    const int t = idx[0]; idx[0] = idx[rot]; idx[rot] = t;
+    const uint32_t bins = (uint32_t)idx[0] | ((uint32_t)idx[1] << 8) | ((uint32_t)idx[2] << 16) | ((uint32_t)idx[3] << 24);
+    const LightSample ls = sample_light<true>(sc, c.pos, c.dir, bins, rng);
+    if (!ls.valid) continue;
+    const float z = dot(ls.dir, c.dir);
+    if (!(z > 0.f)) continue;
+    const float x = dot(ls.dir, c.right) / z, y = dot(ls.dir, c.up) / z;
+    const float fx = (x - c.xmin) / c.xper, fy = (y - c.ymin) / c.yper;
+    const bool trunc = sc.mis_mode != TTGPU_MIS_CONSISTENT;
+    const float ix = trunc ? truncf(fx) : floorf(fx), iy = trunc ? truncf(fy) : floorf(fy);
+    if (!(ix >= 0.f && ix < (float)c.width && iy >= 0.f && iy < (float)c.height)) continue;
+    const float pdf_cam = camera_pdf(c, x, y);
+    const float w = (!ls.delta && mis) ? mis_weight(true, ls.pdf, pdf_cam) : 1.f;
+    const float3 crgb = spectral_to_rgb(((float)kStrata * w * pdf_cam / (float)npix) * ls.weight, bins);
+    if (crgb.x == 0.f && crgb.y == 0.f && crgb.z == 0.f) continue;
+    Ray r;
+    r.org = c.pos;
+    r.dir = ls.dir;
+    r.tmax = ls.len * 0.9999f;
+    HitRec hr;
+    ++rays;
+    if (traverse<true>(sc.nodes, sc.prims, r, lds_stack + threadIdx.x, hr)) continue;
+    const size_t p = (size_t)iy * c.width + (size_t)ix;
+    atomicAdd(fb + 3 * p + 0, (double)crgb.x);
+    atomicAdd(fb + 3 * p + 1, (double)crgb.y);
+    atomicAdd(fb + 3 * p + 2, (double)crgb.z);
+    if (fb_sq) {
+      atomicAdd(fb_sq + 3 * p + 0, (double)crgb.x * crgb.x);
+      atomicAdd(fb_sq + 3 * p + 1, (double)crgb.y * crgb.y);
+      atomicAdd(fb_sq + 3 * p + 2, (double)crgb.z * crgb.z);
+    }
+  }
+  if (rays) atomicAdd(stats + kStatShadow, rays);
 }
 
 // ---- media: shade with null-path tracking (pt2.cxx:410-443 with media_integrator.cxx) ------
@@ -1279,7 +1375,10 @@ void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
 }
 void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
                  uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
-  if (sc.guide_cells)
+  if (sc.algo_pt)
+    hipLaunchKernelGGL(k_shade<true>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out,
+                       count_out, fb, fb_sq);
+  else if (sc.guide_cells)
     hipLaunchKernelGGL(k_shade_guided<0>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in,
                        queue_out, count_out, fb, fb_sq);
   else if (sc.guide_qcells)
@@ -1292,8 +1391,14 @@ void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
     hipLaunchKernelGGL(k_shade_media, dim3(Blocks(max_count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, q, queue_in, count_in,
                        queue_out, count_out, fb, fb_sq);
   else
-    hipLaunchKernelGGL(k_shade, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out, count_out,
+    hipLaunchKernelGGL(k_shade<false>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out, count_out,
                        fb, fb_sq);
+}
+void LaunchCameraConnect(const DeviceScene &sc, uint64_t first, uint64_t count, uint32_t npix, uint32_t spp_base, uint64_t seed,
+                         double *fb, double *fb_sq, uint32_t *stats, hipStream_t st) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_camera_connect, dim3(Blocks(count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, first, count, npix,
+                     spp_base, seed, fb, fb_sq, stats);
 }
 void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st) {
   if (sc.guide_cells || q.gmix)

@@ -193,7 +193,7 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (p.max_ray_depth < 2 || p.max_ray_depth > 255) bad("max_ray_depth must be in [2, 255]");
   if (p.num_wavelengths != 0 && p.num_wavelengths != 4) bad("the GPU path traces 4 wavelengths per path");
   if (p.sample_mode < 0 || p.sample_mode > 2) bad("bad sample_mode");
-  if (p.algo != TTGPU_ALGO_PT2 && p.algo != TTGPU_ALGO_PTG) bad("unknown algorithm");
+  if (p.algo != TTGPU_ALGO_PT2 && p.algo != TTGPU_ALGO_PTG && p.algo != TTGPU_ALGO_PT) bad("unknown algorithm");
   if (p.guide_kind < TTGPU_GUIDE_NONE || p.guide_kind > TTGPU_GUIDE_QUADTREE) bad("unknown guide kind");
   if (s.num_triangles < 0 || s.num_spheres < 0 || s.num_vertices < 0) bad("negative counts");
   if (s.num_triangles > 0 && (!s.vertices || !s.indices || !s.tri_material)) bad("triangles need vertices, indices, materials");
@@ -224,10 +224,12 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (s.num_media > kMaxMedia) bad("too many media");
   if (s.num_media > 0 && !s.media) bad("media table missing");
   if (s.empty_space_medium >= s.num_media || s.empty_space_medium < -1) bad("empty_space_medium out of range");
-  if (p.algo == TTGPU_ALGO_PTG) {
+  if (p.algo == TTGPU_ALGO_PTG || p.algo == TTGPU_ALGO_PT) {
     bool any_medium = s.empty_space_medium >= 0;
     for (int i = 0; i < s.num_materials; ++i) any_medium |= s.materials[i].medium >= 0 || s.materials[i].outer_medium >= 0;
-    if (any_medium) bad("the guided integrator guides surface scattering only; scenes with media use pt2");
+    if (any_medium && p.algo == TTGPU_ALGO_PTG)
+      bad("the guided integrator guides surface scattering only; scenes with media use pt2");
+    if (any_medium) bad("pt runs on surface scenes (no media); scenes with media use pt2");
   }
   for (int i = 0; i < s.num_media; ++i) {
     const auto &m = s.media[i];
@@ -688,6 +690,7 @@ void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
   h->dsc.sample_mode = h->params.sample_mode;
   h->dsc.mis_mode = h->params.mis_mode;
   h->dsc.max_depth = h->params.max_ray_depth;
+  h->dsc.algo_pt = h->params.algo == TTGPU_ALGO_PT ? 1 : 0;
 }
 
 void AllocateQueues(ttgpu_handle *h) {
@@ -981,10 +984,17 @@ bool RenderPass(ttgpu_handle *h, int spp) {
       if (h->poll[2 * prev + 1] == 0) break;
     }
   }
-  Check(hipEventRecord(ev1, h->stream), "hipEventRecord");
   Check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
   const uint64_t issued = std::max(h->poll[0], h->poll[2]);
   const bool complete = issued == total;
+  // pt: the light connection at the camera node of every issued camera sample.
+  if (h->dsc.algo_pt && h->dsc.sample_mode != TTGPU_SAMPLE_BSDF && h->dsc.cam.kind == TTGPU_CAMERA_PINHOLE)
+    h->Timed("camera", [&] {
+      LaunchCameraConnect(h->dsc, 0, issued, (uint32_t)npix, spp_base, h->params.seed, h->fb, h->fb_sq, q.counters, h->stream);
+    });
+  Check(hipGetLastError(), "kernel launch");
+  Check(hipEventRecord(ev1, h->stream), "hipEventRecord");
+  Check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
   // Per-pixel sample counts: path g of the pass belongs to pixel g % npix.
   {
     const uint64_t full = issued / npix, rem = issued % npix;

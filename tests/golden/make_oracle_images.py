@@ -17,6 +17,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle_lib  # noqa: E402
 from naivetrace_amd import abi, configs, scene_file  # noqa: E402
+from naivetrace_amd.scene import flat  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -46,7 +47,51 @@ FIXTURES = {
 }
 
 
+def pointlit_dome(w, h):
+    """The Cornell box seen with a wide lens (the sky around it visible), a point light inside the
+    view and a uniform dome: point-light and environment connections to the camera, environment
+    NEE and emission MIS."""
+    s = configs.cornell_box(w, h)
+    s.add_point_light([0.35, 1.45, 0.4], flat(6.0))
+    s.add_dome([0, 1, 0], flat(0.3))
+    s.set_camera([0.0, 1.0, 3.4], [0.0, 1.0, 0.0], [0.0, 1.0, 0.0], 75.0)
+    return s
+
+
+# pt (the random-walk forward path tracer): light connections at the camera land in other pixels
+# than their camera sample, so the standard error comes from independent batches.
+PT_FIXTURES = {
+    # name: (scene builder, width, height, batches, spp per batch, max_depth, mis_mode, sample_mode)
+    "cornell32_pt": (lambda w, h: configs.cornell_box(w, h), 32, 32, 8, 1024, 8, abi.MIS_CONSISTENT, abi.SAMPLE_BOTH),
+    "cornell32_pt_lights": (lambda w, h: configs.cornell_box(w, h), 32, 32, 8, 1024, 8, abi.MIS_CONSISTENT, abi.SAMPLE_LIGHTS),
+    # the reference's truncating pixel lookup (TTGPU_MIS_REFERENCE)
+    "pointlit_dome32_pt": (pointlit_dome, 32, 32, 8, 1024, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+}
+
+
+def make_pt(name):
+    build, w, h, batches, spp, depth, mis, mode = PT_FIXTURES[name]
+    scene = build(w, h)
+    p = abi.default_params(w, h, max_depth=depth, seed=7, algo=abi.ALGO_PT)
+    p.mis_mode = mis
+    p.sample_mode = mode
+    t = time.time()
+    means = []
+    for b in range(batches):
+        rgb, _ = oracle_lib.render(scene, p, spp, spp_offset=b * spp, with_sq=False)
+        means.append(rgb / spp)
+    means = np.array(means)
+    mean = means.mean(axis=0)
+    stderr = means.std(axis=0, ddof=1) / np.sqrt(batches)
+    out = os.path.join(HERE, f"{name}_oracle.npz")
+    np.savez_compressed(out, mean=mean, stderr=stderr, spp=spp * batches, batches=batches, width=w, height=h,
+                        max_depth=depth, mis_mode=mis, sample_mode=mode, seed=7, scene=name, algo=abi.ALGO_PT)
+    print(f"{name}: {time.time() - t:.1f}s -> {out}")
+
+
 def make(name):
+    if name in PT_FIXTURES:
+        return make_pt(name)
     build, w, h, spp, depth, mis, mode = FIXTURES[name]
     scene = build(w, h)
     p = abi.default_params(w, h, max_depth=depth, seed=7)
@@ -64,6 +109,6 @@ def make(name):
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(FIXTURES)
+    names = sys.argv[1:] or list(FIXTURES) + list(PT_FIXTURES)
     for n in names:
         make(n)

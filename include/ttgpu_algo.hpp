@@ -4,7 +4,7 @@
 // RenderingAlgo follows src/renderingalgorithms_interface.hxx:13-30 (Run,
 // RequestInterrupt, RequestFullStop, GenerateImage, SetInterruptCallback,
 // InitializeScene, a noexcept(false) destructor); RenderAlgorithmFactory follows
-// src/renderingalgorithms.cxx:27-42 for the keys "pt2" and "ptg".  Errors the
+// src/renderingalgorithms.cxx:27-42 for the keys "pt", "pt2" and "ptg".  Errors the
 // library reports as status codes are thrown as the exceptions the reference
 // throws: std::invalid_argument for bad scenes and parameters,
 // std::runtime_error otherwise.
@@ -51,7 +51,7 @@ inline void ThrowOnError(int rc, const char *msg) {
   throw std::runtime_error(m);
 }
 
-// pt2 / ptg on the GPU.  The scene description and its arrays are borrowed: they must outlive
+// pt / pt2 / ptg on the GPU.  The scene description and its arrays are borrowed: they must outlive
 // InitializeScene (the library copies what it needs to the device there).
 class GpuRenderingAlgo : public RenderingAlgo {
  public:
@@ -112,13 +112,15 @@ class GpuRenderingAlgo : public RenderingAlgo {
   bool stop_requested_ = false;
 };
 
-// RenderAlgorithmFactory: "pt2" (forward path tracing with NEE and null-path tracking) and "ptg"
-// (guided; params.guide_kind selects the vMF tree or the learned guide).
+// RenderAlgorithmFactory: "pt" (the random-walk forward path tracer, surfaces), "pt2" (forward path
+// tracing with NEE and null-path tracking) and "ptg" (guided; params.guide_kind selects the
+// quadtree, the vMF tree or the learned guide).
 inline std::unique_ptr<RenderingAlgo> RenderAlgorithmFactory(const ttgpu_scene &scene, ttgpu_params params,
                                                              const std::string &name) {
   if (name == "pt2") params.algo = TTGPU_ALGO_PT2;
   else if (name == "ptg") params.algo = TTGPU_ALGO_PTG;
-  else throw std::invalid_argument("unknown or unsupported algorithm " + name + "; available: pt2, ptg");
+  else if (name == "pt") params.algo = TTGPU_ALGO_PT;
+  else throw std::invalid_argument("unknown or unsupported algorithm " + name + "; available: pt, pt2, ptg");
   return std::make_unique<GpuRenderingAlgo>(scene, params);
 }
 

@@ -513,6 +513,45 @@ __device__ __forceinline__ LightSample sample_light(const DeviceScene &sc, float
   return ls;
 }
 
+// ---- per-block compaction by material class (LDS, ballot / prefix sums) ----------------------
+// Class of a queued path for shading: 0 = the ray escaped, 1 + shader kind otherwise; inactive
+// entries take the last class.
+constexpr int kSortClasses = 10;
+__device__ __forceinline__ int shade_class(const DeviceScene &sc, const PathQueues &q, uint32_t i) {
+  const int prim = as_int(q.hit[i].w);
+  if (prim < 0) return 0;
+  return 1 + min(sc.shaders[sc.materials[sc.prim_material[prim]].shader].kind, kSortClasses - 3);
+}
+
+// Stable counting sort of the block's kWaves x 64 (active, slot) entries by class: per-wave class counts
+// by ballot and popcount, block-wide prefix sums over (class, wave), one LDS scatter.  Afterwards
+// thread t takes the t-th entry, so each wave shades (mostly) one class and the inactive entries
+// gather in the last waves.  Every thread of the block must call it.
+template <int kWaves>
+__device__ __forceinline__ void block_sort_by_class(const DeviceScene &sc, const PathQueues &q, bool &active, uint32_t &i,
+                                                    uint32_t *s_idx, uint32_t (*s_cnt)[kSortClasses]) {
+  const int cls = active ? shade_class(sc, q, i) : kSortClasses - 1;
+  const int lane = __lane_id(), wave = (int)(threadIdx.x >> 6);
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t rank = 0;
+  for (int c = 0; c < kSortClasses; ++c) {
+    const uint64_t m = __ballot(cls == c);
+    if (cls == c) rank = (uint32_t)__popcll(m & below);
+    if (lane == 0) s_cnt[wave][c] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  uint32_t off = 0;
+  for (int c = 0; c < cls; ++c)
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) off += s_cnt[w][c];
+  for (int w = 0; w < wave; ++w) off += s_cnt[w][cls];
+  s_idx[off + rank] = active ? i : 0xFFFFFFFFu;
+  __syncthreads();
+  i = s_idx[threadIdx.x];
+  active = i != 0xFFFFFFFFu;
+  __syncthreads();  // the next iteration rewrites s_idx and s_cnt
+}
+
 __device__ __forceinline__ float mis_weight(bool power, float a, float b) {
   return power ? a * a / (a * a + b * b) : a / (a + b);
 }
@@ -525,6 +564,8 @@ template <bool kPt>
 __global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
                                                const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                uint32_t *count_out, double *fb, double *fb_sq) {
+  __shared__ uint32_t s_idx[256];
+  __shared__ uint32_t s_cnt[4][kSortClasses];
   const uint32_t count = *count_in;
   const bool ident = count_in[kIdentOffset] != 0;
   const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
@@ -536,8 +577,9 @@ __global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc
     bool active = k < count;
     bool push = false, push_shadow = false;
     uint32_t i = 0;
+    if (active) i = ident ? k : queue_in[k];
+    if (sc.sort_shade) block_sort_by_class<4>(sc, q, active, i, s_idx, s_cnt);
     if (active) {
-      i = ident ? k : queue_in[k];
       const float4 hitv = q.hit[i];
       const float3 dir = xyz(q.dir[i]);
       const float4 org4 = q.org[i];
@@ -731,6 +773,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_connect(DeviceScene sc, 
 __global__ void __launch_bounds__(kTraceBlock) k_shade_media(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
                                                              const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                              uint32_t *count_out, double *fb, double *fb_sq) {
+  __shared__ uint32_t s_idx[kTraceBlock];
+  __shared__ uint32_t s_cnt[kTraceBlock / 64][kSortClasses];
   const uint32_t count = *count_in;
   const bool ident = count_in[kIdentOffset] != 0;
   const bool use_nee = sc.sample_mode != TTGPU_SAMPLE_BSDF;
@@ -739,11 +783,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_shade_media(DeviceScene sc, Pat
   const uint32_t n_iter = (count + grid_stride() - 1) / grid_stride();
   for (uint32_t it = 0; it < n_iter; ++it) {
     const uint32_t k = it * grid_stride() + gtid();
-    const bool active = k < count;
+    bool active = k < count;
     bool push = false, push_shadow = false;
     uint32_t i = 0;
+    if (active) i = ident ? k : queue_in[k];
+    if (sc.sort_shade) block_sort_by_class<kTraceBlock / 64>(sc, q, active, i, s_idx, s_cnt);
     if (active) {
-      i = ident ? k : queue_in[k];
       const float4 hitv = q.hit[i];
       const float3 dir = xyz(q.dir[i]);
       const float4 org4 = q.org[i];
@@ -975,6 +1020,8 @@ template <int kGuide>
 __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues q, const uint32_t *__restrict__ queue_in,
                                                       const uint32_t *__restrict__ count_in, uint32_t *__restrict__ queue_out,
                                                       uint32_t *count_out, double *fb, double *fb_sq) {
+  __shared__ uint32_t s_idx[256];
+  __shared__ uint32_t s_cnt[4][kSortClasses];
   constexpr bool kLearned = kGuide == 1, kQuad = kGuide == 2;
   const uint32_t count = *count_in;
   const bool ident = count_in[kIdentOffset] != 0;
@@ -986,11 +1033,12 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
   const uint32_t n_iter = (count + grid_stride() - 1) / grid_stride();
   for (uint32_t it = 0; it < n_iter; ++it) {
     const uint32_t k = it * grid_stride() + gtid();
-    const bool active = k < count;
+    bool active = k < count;
     bool push = false, push_shadow = false;
     uint32_t i = 0;
+    if (active) i = ident ? k : queue_in[k];
+    if (sc.sort_shade) block_sort_by_class<4>(sc, q, active, i, s_idx, s_cnt);
     if (active) {
-      i = ident ? k : queue_in[k];
       const float4 hitv = q.hit[i];
       const float3 dir = xyz(q.dir[i]);
       const float4 org4 = q.org[i];

@@ -687,6 +687,9 @@ void BuildTables(ttgpu_handle *h, const ttgpu_scene &s, double bb_diag) {
   // Persistent traversal with dynamic ray fetch unless TTGPU_PERSISTENT_TRAVERSAL=0 (one ray per thread).
   const char *pt = std::getenv("TTGPU_PERSISTENT_TRAVERSAL");
   h->dsc.persistent = (pt && pt[0] == '0') ? 0 : 1;
+  // Shade kernels sort each block's paths by shader class unless TTGPU_SORT_SHADE=0.
+  const char *ss = std::getenv("TTGPU_SORT_SHADE");
+  h->dsc.sort_shade = (ss && ss[0] == '0') ? 0 : 1;
   h->dsc.sample_mode = h->params.sample_mode;
   h->dsc.mis_mode = h->params.mis_mode;
   h->dsc.max_depth = h->params.max_ray_depth;

@@ -207,7 +207,7 @@ typedef struct ttgpu_scene {
  * --algo): power-heuristic MIS, Russian roulette beyond 5 nodes, light points
  * connected to the camera and splatted to the pixel they project to; surface
  * scenes only. */
-enum ttgpu_algo { TTGPU_ALGO_PT2 = 0, TTGPU_ALGO_PTG = 1, TTGPU_ALGO_PT = 2 };
+enum ttgpu_algo { TTGPU_ALGO_PT2 = 0, TTGPU_ALGO_PTG = 1, TTGPU_ALGO_PT = 2, TTGPU_ALGO_BDPT = 3 };
 enum ttgpu_sample_mode {     /* --pt-sample-mode */
   TTGPU_SAMPLE_BOTH = 0, TTGPU_SAMPLE_BSDF = 1, TTGPU_SAMPLE_LIGHTS = 2
 };

@@ -23,7 +23,7 @@ MEDIUM_VACUUM, MEDIUM_HOMOGENEOUS, MEDIUM_ATMOSPHERE, MEDIUM_ATMOSPHERE_TABULATE
 AREA_UNIFORM, AREA_PARALLEL = range(2)
 ENV_SUN, ENV_DOME, ENV_DIRECTIONAL, ENV_MAP = range(4)
 CAMERA_PINHOLE, CAMERA_FISHEYE = range(2)
-ALGO_PT2, ALGO_PTG, ALGO_PT = range(3)
+ALGO_PT2, ALGO_PTG, ALGO_PT, ALGO_BDPT = range(4)
 SAMPLE_BOTH, SAMPLE_BSDF, SAMPLE_LIGHTS = range(3)
 MIS_REFERENCE, MIS_CONSISTENT = range(2)
 GUIDE_NONE, GUIDE_VMF_TREE, GUIDE_LEARNED, GUIDE_QUADTREE = range(4)

@@ -1206,6 +1206,463 @@ template <int NW> struct Engine {
     }
   }
 
+  // ---- bdpt: bidirectional path tracing ----------------------------------------------------
+  // BdptWorker (src/renderingalgorithms_pathtracing.hxx:358-700) on the random-walk framework,
+  // surfaces in vacuum, pinhole camera.  Strategies: every camera / scatter vertex of the eye
+  // subpath connects to a fresh light sample (t = 1) and to the light subpath's vertices 1.. (t >= 2),
+  // eye vertices on emitters count as light hits; connections from the camera are splatted.  MIS:
+  // power heuristic over all strategies of the full path, strategies with >= 2 eye vertices
+  // weighted by 1 / pixel count (BdptMis, :220-335), strategies next to a delta vertex excluded.
+  // The densities of every strategy are computed afresh for each connection.
+  enum { BV_CAMERA, BV_SURF, BV_AREA, BV_ENV, BV_POINT };
+  using Spec3d = std::array<double, 3>;
+  struct BV {
+    int type = BV_SURF;
+    V3 pos;                  // BV_ENV: direction the light travels
+    SurfaceInteraction si;   // BV_SURF
+    V3 ng;                   // BV_AREA: geometry normal at the emitter point
+    int prim = -1;           // BV_SURF / BV_AREA
+    V3 wi;                   // BV_SURF: direction of the ray that arrived
+    Spec<NW> beta, Le;       // path weight; BV_AREA / BV_ENV radiance, BV_POINT intensity
+    bool delta = false;      // the walk left this vertex through a delta lobe
+    double pdf_walk = 0.;    // density of this vertex from its own subpath's side (area / solid angle for ENV)
+    double pdf_other = 0.;   // density from the opposite side (set by the walk for inner vertices)
+  };
+
+  bool IsSurface(const BV &v) const { return v.type == BV_SURF || v.type == BV_AREA; }
+  V3 GeomNormal(const BV &v) const { return v.type == BV_SURF ? v.si.geometry_normal : v.ng; }
+  // PdfConversionFactor (rendering_randomwalk_impl.hxx:495-515): solid angle at src -> measure at dst.
+  double Conv(const BV &src, const BV &dst, V3 dir, double len) const {
+    double f = IsSurface(dst) ? std::abs(Dot(GeomNormal(dst), dir)) : 1.;
+    if (src.type != BV_ENV && dst.type != BV_ENV) f /= Sqr(len) + Epsilon;
+    return f;
+  }
+  double DFactorOf(const BV &v, V3 dir) const { return IsSurface(v) ? std::abs(Dot(GeomNormal(v), dir)) : 1.; }
+  // BsdfCorrectionFactor (:651-657) x DFactorOf for a scatter vertex.
+  double Corr(const BV &v, V3 out, bool importance) const {
+    const V3 l = importance ? -v.wi : out;
+    return std::abs(Dot(v.si.shading_normal, l)) / (std::abs(Dot(v.si.geometry_normal, l)) + Epsilon) *
+           std::abs(Dot(v.si.geometry_normal, out));
+  }
+  // Scatter density (solid angle) at a scatter vertex from `in` (reverse incident) to `out`.
+  double ScatterPdf(const BV &v, V3 in, V3 out, const PathContext<NW> &ctx) const {
+    double pdf = 0.;
+    sh.Evaluate(sc.shaders[sc.materials[v.si.material].shader], in, v.si, ctx, 0., out, &pdf);
+    return pdf;
+  }
+  // Density at an emitter / camera vertex of emitting toward `out` (solid angle; disc area for ENV).
+  double EmissionPdf(const BV &v, V3 out) const {
+    if (v.type == BV_AREA) return Dot(v.ng, out) > 0. ? 1. / UnitHalfSphereSurfaceArea : 0.;
+    if (v.type == BV_POINT) return 1. / UnitSphereSurfaceArea;
+    if (v.type == BV_ENV) return 1. / (Pi * 0.25 * Sqr(bb_diag));
+    return 0.;
+  }
+
+  // SampleEmissiveEnd: light start vertex with its density (pick x area pdf; pick x direction pdf
+  // for the environment; pick for point lights, a delta position).
+  bool SampleLightVertex(const PathContext<NW> &ctx, Sampler &smp, BV &v, bool &pos_delta) const {
+    if (counts[0] + counts[1] + counts[2] + counts[3] == 0) return false;
+    const int kind = TowerSampling<4>(type_probs, smp.Uniform01());
+    const double u2 = smp.Uniform01();
+    pos_delta = false;
+    if (kind == IDX_AREA) {
+      const long li = TowerSamplingBisection(area_cmf.data(), counts[IDX_AREA], u2);
+      const double prob = type_probs[IDX_AREA] * ProbabilityFromCmf(area_cmf.data(), li);
+      const int prim = sc.area_lights[li];
+      double r[2];
+      smp.UniformUnitSquare(r);
+      Hit lh = sc.SampleUniformPosition(prim, r);
+      V3 lsn, lpb;
+      v.type = BV_AREA;
+      sc.LocalGeometry(lh, v.pos, v.ng, lsn, lpb);
+      v.si.pos = v.pos;
+      v.si.geometry_normal = v.si.normal = v.ng;
+      v.si.pos_bounds = lpb;
+      v.prim = prim;
+      const auto &em = sc.emitters[sc.materials[sc.prims[prim].material].emitter];
+      v.Le = em.kind == TTGPU_AREA_UNIFORM ? (1. / UnitHalfSphereSurfaceArea) * Take<NW>(em.spectrum, ctx) : SpecConst<NW>(0.);
+      v.pdf_walk = prob / sc.PrimArea(prim);
+      return true;
+    }
+    if (kind == IDX_ENV) {
+      const double prob = type_probs[IDX_ENV];
+      V3 dir_out;
+      Pdf pdf;
+      EnvSample(smp, ctx, dir_out, v.Le, pdf);
+      v.type = BV_ENV;
+      v.pos = dir_out;
+      v.pdf_walk = prob * pdf.val();
+      pos_delta = pdf.IsDelta();
+      return true;
+    }
+    if (kind == IDX_POINT) {
+      const long li = TowerSamplingBisection(point_cmf.data(), counts[IDX_POINT], u2);
+      const auto &pl = sc.points[li];
+      v.type = BV_POINT;
+      v.pos = V3{pl.pos[0], pl.pos[1], pl.pos[2]};
+      v.Le = (1. / UnitSphereSurfaceArea) * Take<NW>(pl.spectrum, ctx);
+      v.pdf_walk = type_probs[IDX_POINT] * ProbabilityFromCmf(point_cmf.data(), li);
+      pos_delta = true;
+      return true;
+    }
+    return false;
+  }
+
+  // Segment between two vertices with the anti-self-intersection offsets at surface ends
+  // (CalculateSegmentToTarget, :661-716): origin, direction from a to b, length.
+  void Segment(const BV &a, const BV &b, V3 &org, V3 &dir, double &len) const {
+    const double far = 10. * bb_diag;
+    if (b.type == BV_ENV) {
+      dir = -b.pos;
+      len = far;
+      org = a.pos;
+      if (a.type == BV_SURF) org = org + AntiSelfIntersectionOffset(a.si, dir);
+      return;
+    }
+    if (a.type == BV_ENV) {  // not used: eye vertices are never ENV in connections
+      dir = a.pos;
+      len = far;
+      org = b.pos - dir * len;
+      return;
+    }
+    V3 pa = a.pos, pb = b.pos;
+    V3 d = pb - pa;
+    double l = Length(d);
+    d = l > 0. ? d / l : V3{1., 0., 0.};
+    if (a.type == BV_SURF) pa = pa + AntiSelfIntersectionOffset(a.si, d);
+    if (IsSurface(b)) pb = pb + AntiSelfIntersectionOffset(b.si, -d);
+    d = pb - pa;
+    len = Length(d);
+    dir = len > 0. ? d / len : V3{1., 0., 0.};
+    org = pa;
+  }
+
+  // BdptForwardTrace (:598-646): random walk from the last vertex of `path`, whose outgoing
+  // direction `dir` (density `pdf`, delta or not) and step weight `factor` are given.
+  // Russian roulette: the reference scales the step weight by 1 / p when the walk survives and keeps
+  // the new vertex with the unscaled weight when it ends, which weighs that vertex (2 - p) times too
+  // much on average (TTGPU_MIS_REFERENCE); TTGPU_MIS_CONSISTENT carries 1 / p to the vertices after
+  // the decision instead.
+  void Walk(std::vector<BV> &path, V3 org, V3 dir, Pdf pdf, Spec<NW> factor, const PathContext<NW> &ctx, Sampler &smp) const {
+    const int max_nodes = prm.max_ray_depth + 1;
+    const bool rr_reference = prm.mis_mode != TTGPU_MIS_CONSISTENT;
+    double carry = 1.;  // pending 1 / p of the last roulette (consistent placement)
+    for (;;) {
+      BV &cur = path.back();
+      bool zero = true;
+      for (int k = 0; k < NW; ++k) zero &= factor[k] == 0.;
+      if (zero || !(pdf.val() > 0.)) return;
+      Hit hit;
+      const bool has_hit = sc.FirstIntersection(org, dir, 0., LargeNumber, hit);
+      if (!has_hit && (ctx.importance || sc.envs.empty())) return;  // light paths end at the environment
+      cur.delta = pdf.IsDelta();
+      const int n = (int)path.size();
+      // RR on the step weight (SurvivalAtNthScatterNode with node count n + 1).
+      factor = carry * factor;
+      carry = 1.;
+      bool survive = n + 1 < max_nodes;
+      if (survive && n + 1 > 5) {
+        double mx = 0.;
+        for (int k = 0; k < NW; ++k) mx = std::max(mx, factor[k]);
+        const double ps = std::min(0.9, mx);
+        if (smp.Uniform01() > ps) survive = false;
+        else if (rr_reference) factor = (1. / ps) * factor;
+        else carry = 1. / ps;
+      }
+      BV v;
+      v.beta = cur.beta * factor;
+      if (!has_hit) {
+        v.type = BV_ENV;
+        v.pos = -dir;
+        v.Le = EnvEval(-dir, ctx);
+        v.pdf_walk = pdf.val();
+      } else {
+        v.type = BV_SURF;
+        v.si = MakeSurfaceInteraction(hit, dir);
+        v.pos = v.si.pos;
+        v.prim = hit.prim;
+        v.wi = dir;
+        v.pdf_walk = pdf.val() * Conv(cur, v, dir, hit.t);
+      }
+      // Density of the vertex before `cur` seen from `cur` (ReverseScatterPdf, :680-697).
+      if (n >= 2 && cur.type == BV_SURF) {
+        BV &prev = path[n - 2];
+        const double rp = pdf.IsDelta() ? pdf.val() : ScatterPdf(cur, dir, -cur.wi, ctx);
+        V3 o, d;
+        double len;
+        Segment(cur, prev, o, d, len);
+        prev.pdf_other = rp * Conv(cur, prev, d, len);
+      }
+      path.push_back(v);
+      if (!survive || v.type == BV_ENV) return;
+      // Next direction at the new vertex.
+      BV &nv = path.back();
+      const auto &shd = sc.shaders[sc.materials[nv.si.material].shader];
+      const ScatterSample<NW> s = sh.Sample(shd, -nv.wi, nv.si, ctx, 0., smp);
+      const double c = Corr(nv, s.dir, ctx.importance);
+      for (int k = 0; k < NW; ++k) factor[k] = s.value[k] * c / s.pdf.val();
+      pdf = s.pdf;
+      dir = s.dir;
+      org = nv.si.pos + AntiSelfIntersectionOffset(nv.si, s.dir);
+    }
+  }
+
+  // Power-heuristic weight of the strategy with `s` eye vertices over the full path z (eye
+  // vertices first): pe / pl = densities from the eye / light side, spec = delta vertices,
+  // cam_delta / light_delta = delta positions of the ends.
+  double MisBdpt(const std::vector<double> &pe, const std::vector<double> &pl, const std::vector<char> &spec, bool light_delta,
+                 int s) const {
+    const int N = (int)pe.size();
+    const double inv_npix = 1. / ((double)W * H);
+    auto P = [&](int j) {
+      double p = j >= 2 ? inv_npix : 1.;
+      for (int k = 0; k < j; ++k) p *= pe[k];
+      for (int k = j; k < N; ++k) p *= pl[k];
+      return p;
+    };
+    auto valid = [&](int j) {
+      const bool a = j == 0 ? true : spec[j - 1] != 0;  // the camera position is a delta
+      const bool b = j == N ? light_delta : spec[j] != 0;
+      return !(a || b);
+    };
+    const double ps = P(s);
+    double den = 0.;
+    for (int j = 0; j <= N; ++j)
+      if (valid(j)) den += Sqr(P(j));
+    return den > 0. ? Sqr(ps) / den : 0.;
+  }
+
+  // Eye vertex e connected to light vertex l of `light` (the light start when l == 0); returns the
+  // unweighted contribution and the MIS weight.
+  bool ConnectBdpt(const std::vector<BV> &eye, int e, const std::vector<BV> &light, int l, bool light_delta,
+                   const PathContext<NW> &ctx, const PathContext<NW> &lctx, Spec<NW> &contrib, double &w, int &pixel) const {
+    const BV &a = eye[e], &b = light[l];
+    V3 org, dir;
+    double len;
+    Segment(a, b, org, dir, len);
+    // eye side: camera importance or BSDF
+    Spec<NW> fa;
+    double pa_fwd = 0., pa_rev = 0.;  // density at a toward b; at a toward eye[e-1] given from b
+    pixel = -1;
+    if (a.type == BV_CAMERA) {
+      pixel = CameraEvaluate(dir, pa_fwd);
+      if (pixel < 0) return false;
+      fa = SpecConst<NW>(pa_fwd);
+    } else {
+      const auto &shd = sc.shaders[sc.materials[a.si.material].shader];
+      fa = sh.Evaluate(shd, -a.wi, a.si, ctx, 0., dir, &pa_fwd);
+      fa = Corr(a, dir, false) * fa;
+      pa_rev = ScatterPdf(a, dir, -a.wi, ctx);
+    }
+    // light side: emission or BSDF (importance)
+    Spec<NW> fb;
+    double pb_fwd = 0., pb_rev = 0.;  // density at b toward a; at b toward light[l-1] given from a
+    if (b.type == BV_SURF) {
+      const auto &shd = sc.shaders[sc.materials[b.si.material].shader];
+      fb = sh.Evaluate(shd, -b.wi, b.si, lctx, 0., -dir, &pb_fwd);
+      fb = Corr(b, -dir, true) * fb;
+      pb_rev = ScatterPdf(b, -dir, -b.wi, lctx);
+    } else {
+      pb_fwd = EmissionPdf(b, -dir);
+      if (b.type == BV_AREA) fb = (Dot(b.ng, -dir) > 0. ? std::abs(Dot(b.ng, dir)) : 0.) * b.Le;
+      else fb = b.Le;
+    }
+    Spec<NW> f = fa * fb;
+    bool zero = true;
+    for (int k = 0; k < NW; ++k) zero &= f[k] == 0.;
+    if (zero) return false;
+    const double G = (b.type == BV_ENV) ? 1. : 1. / Sqr(len);
+    if (sc.IsOccluded(org, dir, 0., len * 0.9999)) return false;
+    contrib = G * (a.beta * b.beta * f);
+    // densities of the full path z = eye[0..e], light[l..0]
+    const int N = e + 1 + l + 1;
+    std::vector<double> pe(N), pl(N);
+    std::vector<char> spec(N, 0);
+    for (int k = 0; k <= e; ++k) {
+      pe[k] = k == 0 ? 1. : eye[k].pdf_walk;
+      pl[k] = eye[k].pdf_other;
+      spec[k] = k < e ? eye[k].delta : 0;
+    }
+    for (int j = 0; j <= l; ++j) {
+      const int k = N - 1 - j;
+      pl[k] = light[j].pdf_walk;
+      pe[k] = light[j].pdf_other;
+      spec[k] = j < l ? light[j].delta : 0;
+    }
+    pe[e + 1] = pa_fwd * Conv(a, b, dir, len);
+    pl[e] = pb_fwd * Conv(b, a, -dir, len);
+    if (e >= 1) {
+      V3 o2, d2;
+      double l2;
+      Segment(a, eye[e - 1], o2, d2, l2);
+      pl[e - 1] = pa_rev * Conv(a, eye[e - 1], d2, l2);
+    }
+    if (l >= 1) {
+      V3 o2, d2;
+      double l2;
+      Segment(b, light[l - 1], o2, d2, l2);
+      pe[e + 2] = pb_rev * Conv(b, light[l - 1], d2, l2);
+    }
+    w = MisBdpt(pe, pl, spec, light_delta, e + 1);
+    return true;
+  }
+
+  void RenderPathBdpt(int px, int py, uint64_t sample_index, double rgb[3], std::vector<std::pair<int, Spec3d>> &splats) const {
+    PathContext<NW> ctx;
+    Sampler smp;
+    smp.rng.Seed(prm.seed, sample_index);
+    rgb[0] = rgb[1] = rgb[2] = 0.;
+    constexpr int strata = NBINS / NW;
+    int main_idx = smp.UniformInt(0, strata - 1);
+    int idx[NW];
+    for (int i = 0; i < NW; ++i) idx[i] = main_idx + i * strata;
+    std::swap(idx[0], idx[smp.UniformInt(0, NW - 1)]);
+    for (int i = 0; i < NW; ++i) {
+      ctx.bins[i] = idx[i];
+      double c = kBinCenters[idx[i]], hw = 0.5 * (LambdaMax - LambdaMin) / NBINS;
+      ctx.wavelengths[i] = (c - hw) + smp.Uniform01() * (2. * hw);
+    }
+    ctx.importance = false;
+    PathContext<NW> lctx = ctx;
+    lctx.importance = true;
+    const int max_nodes = prm.max_ray_depth + 1;
+    // eye subpath
+    std::vector<BV> eye;
+    {
+      BV c;
+      c.type = BV_CAMERA;
+      c.pos = cam_pos;
+      c.beta = SpecConst<NW>((double)strata);
+      c.pdf_walk = 1.;
+      eye.push_back(c);
+      double r[2];
+      smp.UniformUnitSquare(r);
+      const double x = xmin + (r[0] + px) * xper, y = ymin + (r[1] + py) * yper;
+      const V3 dir = cam_frame * Normalized(V3{x, y, 1.0});
+      if (max_nodes > 1) Walk(eye, cam_pos, dir, Pdf(CameraPdf(x, y)), SpecConst<NW>(1.), ctx, smp);
+    }
+    // light subpath
+    std::vector<BV> light;
+    bool light_delta = false;
+    {
+      BV l0;
+      if (SampleLightVertex(lctx, smp, l0, light_delta)) {
+        l0.beta = SpecConst<NW>(1. / l0.pdf_walk);
+        light.push_back(l0);
+        double r[2];
+        smp.UniformUnitSquare(r);
+        V3 org, dir;
+        Pdf pdf;
+        Spec<NW> factor;
+        if (l0.type == BV_AREA) {
+          dir = OrthogonalSystemZAligned(l0.ng) * ToUniformHemisphere(r);
+          pdf = Pdf(1. / UnitHalfSphereSurfaceArea);
+          factor = (std::abs(Dot(l0.ng, dir)) * UnitHalfSphereSurfaceArea) * l0.Le;
+          org = l0.pos + AntiSelfIntersectionOffset(l0.si, dir);
+        } else if (l0.type == BV_POINT) {
+          dir = ToUniformSphere(r);
+          pdf = Pdf(1. / UnitSphereSurfaceArea);
+          factor = UnitSphereSurfaceArea * l0.Le;
+          org = l0.pos;
+        } else {
+          // EnvLightPointSamplingBeyondScene (light.cxx:201-219): a disc across the scene's box.
+          const V3 centre{0.5 * (sc.bb_lo[0] + sc.bb_hi[0]), 0.5 * (sc.bb_lo[1] + sc.bb_hi[1]), 0.5 * (sc.bb_lo[2] + sc.bb_hi[2])};
+          dir = l0.pos;
+          org = centre - dir * (10. * bb_diag) + OrthogonalSystemZAligned(dir) * (ToUniformDisc(r) * (0.5 * bb_diag));
+          pdf = Pdf(EmissionPdf(l0, dir));
+          factor = (1. / pdf.val()) * l0.Le;
+        }
+        if (max_nodes > 1) Walk(light, org, dir, pdf, factor, lctx, smp);
+      }
+    }
+    // strategies
+    auto add_rgb = [&](const Spec<NW> &m, double out[3]) {
+      for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < NW; ++j) out[c] += m[j] * kSpecToRGB[c][ctx.bins[j]];
+    };
+    const double inv_npix = 1. / ((double)W * H);
+    for (int e = 0; e < (int)eye.size(); ++e) {
+      const BV &ev = eye[e];
+      if (e == 0 || ev.type == BV_SURF) {
+        if (e + 1 < max_nodes) {
+          std::vector<BV> direct(1);
+          bool dl_delta;
+          if (SampleLightVertex(lctx, smp, direct[0], dl_delta)) {
+            direct[0].beta = SpecConst<NW>(1. / direct[0].pdf_walk);
+            Spec<NW> c;
+            double w;
+            int pix;
+            if (ConnectBdpt(eye, e, direct, 0, dl_delta, ctx, lctx, c, w, pix)) {
+              if (e == 0) {
+                double sp[3] = {0, 0, 0};
+                add_rgb((w * inv_npix) * c, sp);
+                splats.push_back({pix, Spec3d{sp[0], sp[1], sp[2]}});
+              } else {
+                add_rgb(w * c, rgb);
+              }
+            }
+          }
+        }
+        const int lmax = std::min((int)light.size(), max_nodes - e - 1);
+        for (int l = 1; l < lmax; ++l) {
+          Spec<NW> c;
+          double w;
+          int pix;
+          if (!ConnectBdpt(eye, e, light, l, light_delta, ctx, lctx, c, w, pix)) continue;
+          if (e == 0) {
+            double sp[3] = {0, 0, 0};
+            add_rgb((w * inv_npix) * c, sp);
+            splats.push_back({pix, Spec3d{sp[0], sp[1], sp[2]}});
+          } else {
+            add_rgb(w * c, rgb);
+          }
+        }
+      }
+      // the eye subpath hit an emitter (HandleLightHit, :526-556)
+      if (e > 0) {
+        Spec<NW> Le;
+        double p_start = 0., p_emit = 0.;
+        bool is_light = false;
+        if (ev.type == BV_ENV) {
+          Le = ev.Le;
+          p_start = EnvPdf(ev.pos) * PmfEnv();
+          p_emit = 1. / (Pi * 0.25 * Sqr(bb_diag));
+          is_light = true;
+        } else if (ev.type == BV_SURF && sc.materials[ev.si.material].emitter >= 0) {
+          const auto &em = sc.emitters[sc.materials[ev.si.material].emitter];
+          const bool facing = Dot(ev.si.geometry_normal, -ev.wi) > 0.;
+          Le = (em.kind == TTGPU_AREA_UNIFORM && facing) ? (1. / UnitHalfSphereSurfaceArea) * Take<NW>(em.spectrum, ctx)
+                                                          : SpecConst<NW>(0.);
+          p_start = PmfArea() / sc.PrimArea(ev.prim);
+          p_emit = facing ? 1. / UnitHalfSphereSurfaceArea : 0.;
+          is_light = true;
+        }
+        if (is_light) {
+          const int N = e + 1;
+          std::vector<double> pe(N), pl(N);
+          std::vector<char> spec(N, 0);
+          for (int k = 0; k <= e; ++k) {
+            pe[k] = k == 0 ? 1. : eye[k].pdf_walk;
+            pl[k] = eye[k].pdf_other;
+            spec[k] = k < e ? eye[k].delta : 0;
+          }
+          pl[e] = p_start;
+          V3 o2, d2;
+          double l2;
+          BV as_light = ev;
+          if (ev.type == BV_SURF) {
+            as_light.type = BV_AREA;
+            as_light.ng = ev.si.geometry_normal;
+          }
+          Segment(as_light, eye[e - 1], o2, d2, l2);
+          pl[e - 1] = p_emit * Conv(as_light, eye[e - 1], d2, l2);
+          const double w = MisBdpt(pe, pl, spec, false, N);
+          add_rgb(w * (ev.beta * Le), rgb);
+        }
+      }
+    }
+  }
+
   void RenderPath(int px, int py, uint64_t sample_index, double rgb[3]) const {
     Sampler smp;
     smp.rng.Seed(prm.seed, sample_index);
@@ -1239,7 +1696,8 @@ static void RenderImpl(const Scene &sc, const ttgpu_params &p, int spp, int spp_
   Engine<NW> eng(sc, p);
   const int W = p.width, H = p.height;
   const int64_t npix = (int64_t)W * H;
-  const bool pt = p.algo == TTGPU_ALGO_PT;
+  const bool pt = p.algo == TTGPU_ALGO_PT || p.algo == TTGPU_ALGO_BDPT;
+  const bool bdpt = p.algo == TTGPU_ALGO_BDPT;
   std::atomic<int> next_row{0};
   if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
   // pt's light connections at the camera land on other pixels: per-thread splat buffers.
@@ -1257,7 +1715,16 @@ static void RenderImpl(const Scene &sc, const ttgpu_params &p, int spp, int spp_
         for (int s = 0; s < spp; ++s) {
           uint64_t gidx = (uint64_t)(s + spp_offset) * (uint64_t)npix + (uint64_t)(y * W + x);
           double c[3];
-          if (pt) {
+          if (bdpt) {
+            thread_local std::vector<std::pair<int, typename Engine<NW>::Spec3d>> sps;
+            sps.clear();
+            eng.RenderPathBdpt(x, y, gidx, c, sps);
+            for (const auto &sp : sps)
+              for (int k = 0; k < 3; ++k) {
+                splat[tid][3 * sp.first + k] += sp.second[k];
+                splat_sq[tid][3 * sp.first + k] += sp.second[k] * sp.second[k];
+              }
+          } else if (pt) {
             double sp[3];
             int sp_pix;
             eng.RenderPathPt(x, y, gidx, c, &sp_pix, sp);
@@ -1295,8 +1762,16 @@ extern "C" int ttref_render(const ttgpu_scene *scene, const ttgpu_params *params
     if (!scene || !params || !rgb || spp < 0) throw std::invalid_argument("null argument");
     if (params->width <= 0 || params->height <= 0) throw std::invalid_argument("bad resolution");
     Scene sc(*scene);
-    if (params->algo == TTGPU_ALGO_PT && !sc.media.empty() && (scene->empty_space_medium >= 0 || !sc.volumes.empty()))
-      throw std::invalid_argument("pt runs on surface scenes (no media)");
+    if ((params->algo == TTGPU_ALGO_PT || params->algo == TTGPU_ALGO_BDPT) && !sc.media.empty() &&
+        (scene->empty_space_medium >= 0 || !sc.volumes.empty()))
+      throw std::invalid_argument("pt and bdpt run on surface scenes (no media)");
+    if (params->algo == TTGPU_ALGO_BDPT) {
+      if (scene->camera.kind != TTGPU_CAMERA_PINHOLE) throw std::invalid_argument("bdpt needs the pinhole camera");
+      for (int i = 0; i < scene->num_area_emitters; ++i)
+        if (scene->area_emitters[i].kind != TTGPU_AREA_UNIFORM) throw std::invalid_argument("bdpt supports uniform area emitters only");
+      for (int i = 0; i < scene->num_env_lights; ++i)
+        if (scene->env_lights[i].kind == TTGPU_ENV_DIRECTIONAL) throw std::invalid_argument("bdpt does not support directional lights");
+    }
     const auto t0 = std::chrono::steady_clock::now();
     if (params->num_wavelengths == 3) RenderImpl<3>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);
     else if (params->num_wavelengths == 4) RenderImpl<4>(sc, *params, spp, spp_offset, threads, rgb, rgb_sq);

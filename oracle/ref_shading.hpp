@@ -22,6 +22,7 @@ template <int NW> inline double MaxCoeff(const Spec<NW> &a) { double s = a[0]; f
 template <int NW> struct PathContext {
   int bins[NW];
   double wavelengths[NW];
+  bool importance = false;  // TransportType::IMPORTANCE (light subpaths): no radiance scaling on refraction
 };
 
 template <int NW> inline Spec<NW> Take(const float *spectrum, const PathContext<NW> &ctx) {
@@ -281,7 +282,7 @@ template <int NW> struct Shading {
   ScatterSample<NW> DielectricSample(const ttgpu_shader &s, V3 wi, const SurfaceInteraction &si, const PathContext<NW> &ctx, Sampler &smp) const {
     double abs_shn_dot_i = std::abs(Dot(si.shading_normal, wi));
     double eta = EtaIOverT(s, wi, si, ctx);
-    double radiance_weight = Sqr(eta);
+    double radiance_weight = ctx.importance ? 1. : Sqr(eta);  // shader.cxx:425, 498, 566
     double fr = 1.;
     V3 wt;
     bool has_wt = Refracted(wi, si.shading_normal, eta, wt);
@@ -313,7 +314,7 @@ template <int NW> struct Shading {
     const double open = 1. - rough;
     double abs_shn_dot_i = std::abs(Dot(si.shading_normal, wi));
     double eta = EtaIOverT(s, wi, si, ctx);
-    double radiance_weight = Sqr(eta);
+    double radiance_weight = ctx.importance ? 1. : Sqr(eta);  // shader.cxx:425, 498, 566
     double fr = 1.;
     V3 wt;
     bool has_wt = Refracted(wi, si.shading_normal, eta, wt);
@@ -349,7 +350,7 @@ template <int NW> struct Shading {
                                    double rough, V3 wo, double *pdf) const {
     const double open = 1. - rough;
     double eta = EtaIOverT(s, wi, si, ctx);
-    double radiance_weight = Sqr(eta);
+    double radiance_weight = ctx.importance ? 1. : Sqr(eta);  // shader.cxx:425, 498, 566
     const double abs_shn_dot_i = std::abs(Dot(si.shading_normal, wi));
     double fr = 1.;
     V3 wt;

@@ -161,6 +161,7 @@ struct DeviceScene {
   float type_probs[4];             // env, area, point, volume
   float area_pick_prob;            // probability to pick one given area light (incl. type)
   float bb_diag;
+  float3 bb_center;                // centre of the scene's bounding box
   int empty_space_medium;          // kVacuum or medium id
   int has_media;
   uint64_t cam_tracker;            // packed medium tracker at the camera

@@ -1366,6 +1366,484 @@ __global__ void k_guide_pdf_at(DeviceScene sc, int64_t n, const float *pos, cons
   }
 }
 
+// ---- bdpt: bidirectional path tracing (renderingalgorithms_pathtracing.hxx:341-700) ----------
+// One thread per camera sample: eye and light subpaths on the random-walk framework in private
+// arrays, then every strategy of BdptWorker::RenderPixel: each camera / scatter vertex of the eye
+// subpath connects to a fresh light sample and to the light subpath's vertices 1..; eye vertices on
+// emitters count as light hits; connections from the camera are splatted to their pixel.  MIS: the
+// power heuristic over all strategies of the full path, strategies with >= 2 eye vertices weighted
+// by 1 / pixel count, strategies next to a delta vertex excluded; the densities of every strategy
+// are computed afresh per connection (FP64 products).  Surfaces in vacuum, pinhole camera.
+#ifndef TTGPU_BDPT_MAX_VERTS
+#define TTGPU_BDPT_MAX_VERTS 17
+#endif
+constexpr int kBdptMaxVerts = TTGPU_BDPT_MAX_VERTS;  // max_ray_depth + 1 <= this (checked on the host)
+static_assert(kBdptMaxVerts == kBdptMaxVertices, "the host check and the kernel's arrays must agree");
+enum { kBvCamera = 0, kBvSurf = 1, kBvArea = 2, kBvEnv = 3, kBvPoint = 4 };
+
+struct BdptVertex {
+  SurfHit h;     // kBvSurf / kBvArea: surface; kBvEnv: h.pos = direction the light travels; kBvPoint: h.pos
+  float3 wi;     // kBvSurf: direction of the ray that arrived
+  float4 beta;   // path weight
+  float4 Le;     // kBvArea / kBvEnv radiance, kBvPoint intensity
+  float pdf_walk;   // density of this vertex from its own subpath's side
+  float pdf_other;  // density from the opposite side (inner vertices, set by the walk)
+  int type;
+  int delta;     // the walk left this vertex through a delta lobe
+};
+
+struct BdptCtx {
+  const DeviceScene &sc;
+  uint32_t bins;
+  float lambda0;
+  bool consistent_rr;
+  float inv_npix;
+};
+
+__device__ __forceinline__ bool bv_surface(const BdptVertex &v) { return v.type == kBvSurf || v.type == kBvArea; }
+// PdfConversionFactor (rendering_randomwalk_impl.hxx:495-515)
+__device__ __forceinline__ float bv_conv(const BdptVertex &src, const BdptVertex &dst, float3 dir, float len) {
+  float f = bv_surface(dst) ? fabsf(dot(dst.h.ng, dir)) : 1.f;
+  if (src.type != kBvEnv && dst.type != kBvEnv) f /= len * len;
+  return f;
+}
+// BsdfCorrectionFactor (:651-657) times the cosine at the vertex.
+__device__ __forceinline__ float bv_corr(const BdptVertex &v, float3 out, bool importance) {
+  const float3 l = importance ? -v.wi : out;
+  return fabsf(dot(v.h.ns, l)) / (fabsf(dot(v.h.ng, l)) + 1e-20f) * fabsf(dot(v.h.ng, out));
+}
+__device__ __forceinline__ float bv_emission_pdf(const BdptCtx &c, const BdptVertex &v, float3 out) {
+  if (v.type == kBvArea) return dot(v.h.ng, out) > 0.f ? 1.f / (2.f * kPi) : 0.f;
+  if (v.type == kBvPoint) return 1.f / (4.f * kPi);
+  if (v.type == kBvEnv) return 1.f / (kPi * 0.25f * c.sc.bb_diag * c.sc.bb_diag);
+  return 0.f;
+}
+// Segment between vertices with anti-self-intersection offsets at surface ends.
+__device__ __forceinline__ void bv_segment(const BdptCtx &c, const BdptVertex &a, const BdptVertex &b, float3 &org, float3 &dir,
+                                           float &len) {
+  const float far = 10.f * c.sc.bb_diag;
+  if (b.type == kBvEnv) {
+    dir = -b.h.pos;
+    len = far;
+    org = a.type == kBvSurf ? offset_origin(a.h, dir) : a.h.pos;
+    return;
+  }
+  if (a.type == kBvEnv) {
+    dir = a.h.pos;
+    len = far;
+    org = b.h.pos - dir * len;
+    return;
+  }
+  float3 d = b.h.pos - a.h.pos;
+  float l = length(d);
+  d = l > 0.f ? d * (1.f / l) : mk(1.f, 0.f, 0.f);
+  const float3 pa = a.type == kBvSurf ? offset_origin(a.h, d) : a.h.pos;
+  const float3 pb = bv_surface(b) ? offset_origin(b.h, -d) : b.h.pos;
+  d = pb - pa;
+  len = length(d);
+  dir = len > 0.f ? d * (1.f / len) : mk(1.f, 0.f, 0.f);
+  org = pa;
+}
+
+// SampleEmissiveEnd: light start vertex and its density; pos_delta for point lights.
+__device__ bool bv_sample_light(const BdptCtx &c, Pcg &rng, BdptVertex &v, bool &pos_delta) {
+  const DeviceScene &sc = c.sc;
+  const float u_type = rng.uniform(), u_idx = rng.uniform();
+  int kind = 3;
+  {
+    float r = u_type;
+    while (kind > 0 && r >= sc.type_probs[kind]) { r -= sc.type_probs[kind]; --kind; }
+  }
+  pos_delta = false;
+  v.delta = 0;
+  if (kind == 1 && sc.num_area_lights > 0) {
+    const uint32_t li = min((uint32_t)(u_idx * sc.num_area_lights), (uint32_t)sc.num_area_lights - 1u);
+    const int lp = sc.area_lights[li];
+    const PrimRecord pr = sc.prims[lp];
+    const float r0 = rng.uniform(), r1 = rng.uniform();
+    if (as_int(pr.b.w) == 0) {
+      const float3 v0 = xyz(pr.a), v1 = xyz(pr.b), v2 = xyz(pr.c);
+      const float sq = sqrtf(r0);
+      const float b0 = 1.f - sq, b1 = (1.f - r1) * sq, b2 = r1 * sq;
+      v.h.pos = v0 * b0 + v1 * b1 + v2 * b2;
+      v.h.ng = normalize(cross(v1 - v0, v2 - v0));
+      const float g7 = 7.f * kFltEps * 0.5f / (1.f - 7.f * kFltEps * 0.5f);
+      const float3 e = mk(fabsf(v0.x * b0) + fabsf(v1.x * b1) + fabsf(v2.x * b2), fabsf(v0.y * b0) + fabsf(v1.y * b1) + fabsf(v2.y * b2),
+                          fabsf(v0.z * b0) + fabsf(v1.z * b1) + fabsf(v2.z * b2)) * g7;
+      v.h.offset = e.x * fabsf(v.h.ng.x) + e.y * fabsf(v.h.ng.y) + e.z * fabsf(v.h.ng.z);
+    } else {
+      v.h.ng = to_uniform_sphere(r0, r1);
+      v.h.pos = xyz(pr.a) + v.h.ng * pr.b.x;
+      const float g5 = 5.f * kFltEps * 0.5f / (1.f - 5.f * kFltEps * 0.5f);
+      v.h.offset = (fabsf(v.h.pos.x) * fabsf(v.h.ng.x) + fabsf(v.h.pos.y) * fabsf(v.h.ng.y) + fabsf(v.h.pos.z) * fabsf(v.h.ng.z)) * g5;
+    }
+    v.h.n = v.h.ng;
+    v.h.ns = v.h.ng;
+    v.h.prim = lp;
+    v.h.material = sc.prim_material[lp];
+    v.type = kBvArea;
+    const DevEmitter &em = sc.emitters[sc.materials[v.h.material].emitter];
+    v.Le = em.kind == TTGPU_AREA_UNIFORM ? take(em.radiance, c.bins) : f4(0.f);
+    v.pdf_walk = sc.area_pick_prob / sc.prim_area[lp];
+    return true;
+  }
+  if (kind == 0 && sc.num_envs > 0) {
+    float3 dout;
+    float p;
+    bool delta;
+    env_sample(sc, c.bins, rng, dout, v.Le, p, delta);
+    v.type = kBvEnv;
+    v.h.pos = dout;
+    v.pdf_walk = sc.type_probs[0] * p;
+    pos_delta = delta;
+    return true;
+  }
+  if (kind == 2 && sc.num_points > 0) {
+    const uint32_t li = min((uint32_t)(u_idx * sc.num_points), (uint32_t)sc.num_points - 1u);
+    const DevPointLight &pl = sc.points[li];
+    v.type = kBvPoint;
+    v.h.pos = pl.pos;
+    v.Le = take(pl.intensity, c.bins);
+    v.pdf_walk = sc.type_probs[2] / sc.num_points;
+    pos_delta = true;
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ float bv_scatter_pdf(const BdptCtx &c, const BdptVertex &v, float3 in, float3 out, bool importance) {
+  float pdf = 0.f;
+  eval_bsdf(c.sc.shaders[c.sc.materials[v.h.material].shader], v.h, in, c.bins, 0.f, c.lambda0, out, &pdf, importance);
+  return pdf;
+}
+
+// BdptForwardTrace (:598-646) from the last vertex of `path` with the sampled direction `dir`.
+__device__ void bv_walk(const BdptCtx &c, BdptVertex *path, int &n, float3 org, float3 dir, float pdf, bool pdf_delta, float4 factor,
+                        bool importance, Pcg &rng, int *lds_stack, uint32_t &rays) {
+  const DeviceScene &sc = c.sc;
+  const int max_nodes = min(sc.max_depth + 1, kBdptMaxVerts);
+  float carry = 1.f;
+  for (;;) {
+    if (all_zero(factor) || !(pdf > 0.f) || !isfinite(factor.x + factor.y + factor.z + factor.w)) return;
+    Ray r;
+    r.org = org;
+    r.dir = dir;
+    r.tmax = 3.0e38f;
+    HitRec hr;
+    ++rays;
+    const bool hit = traverse<false>(sc.nodes, sc.prims, r, lds_stack, hr);
+    if (!hit && (importance || sc.num_envs == 0)) return;  // light subpaths end at the environment
+    BdptVertex &cur = path[n - 1];
+    cur.delta = pdf_delta ? 1 : 0;
+    factor = carry * factor;
+    carry = 1.f;
+    bool survive = n + 1 < max_nodes;
+    if (survive && n + 1 > 5) {
+      const float ps = fminf(0.9f, fmaxf(fmaxf(factor.x, factor.y), fmaxf(factor.z, factor.w)));
+      if (rng.uniform() > ps) survive = false;
+      else if (c.consistent_rr) carry = 1.f / ps;
+      else factor = factor * (1.f / ps);
+    }
+    BdptVertex &v = path[n];
+    v.beta = cur.beta * factor;
+    v.delta = 0;
+    v.pdf_other = 0.f;
+    if (!hit) {
+      v.type = kBvEnv;
+      v.h.pos = -dir;
+      float pe;
+      v.Le = env_eval(sc, -dir, c.bins, &pe);
+      v.pdf_walk = pdf;
+    } else {
+      v.type = kBvSurf;
+      v.h = make_hit(sc, hr.prim, hr.u, hr.v, hr.t, org, dir);
+      v.wi = dir;
+      v.pdf_walk = pdf * bv_conv(cur, v, dir, hr.t);
+    }
+    if (n >= 2 && cur.type == kBvSurf) {
+      BdptVertex &prev = path[n - 2];
+      const float rp = pdf_delta ? pdf : bv_scatter_pdf(c, cur, dir, -cur.wi, importance);
+      float3 o, d;
+      float len;
+      bv_segment(c, cur, prev, o, d, len);
+      prev.pdf_other = rp * bv_conv(cur, prev, d, len);
+    }
+    ++n;
+    if (!survive || v.type == kBvEnv) return;
+    const DevShader &shd = sc.shaders[sc.materials[v.h.material].shader];
+    const Scatter s = sample_bsdf(shd, v.h, -v.wi, c.bins, 0.f, c.lambda0, rng, importance);
+    factor = (bv_corr(v, s.dir, importance) / s.pdf) * s.value;
+    pdf = s.pdf;
+    pdf_delta = s.delta;
+    dir = s.dir;
+    org = offset_origin(v.h, s.dir);
+  }
+}
+
+// MIS weight of the strategy with s eye vertices over the full path of N vertices.
+__device__ float bv_mis(const double *pe, const double *pl, const bool *spec, int N, bool light_delta, int s, double inv_npix) {
+  double den = 0., ps = 0.;
+  for (int j = 0; j <= N; ++j) {
+    const bool a = j == 0 ? true : spec[j - 1];
+    const bool b = j == N ? light_delta : spec[j];
+    if ((a || b) && j != s) continue;
+    double p = j >= 2 ? inv_npix : 1.;
+    for (int k = 0; k < j; ++k) p *= pe[k];
+    for (int k = j; k < N; ++k) p *= pl[k];
+    if (j == s) ps = p;
+    if (!(a || b)) den += p * p;
+  }
+  return den > 0. ? (float)(ps * ps / den) : 0.f;
+}
+
+// Connection of eye vertex e with light vertex l of `light`; the contribution (MIS weighted) and
+// the splat pixel (camera connections) or -1.
+__device__ bool bv_connect(const BdptCtx &c, const BdptVertex *eye, int e, const BdptVertex *light, int l, bool light_delta,
+                           int *lds_stack, uint32_t &rays, float4 &contrib, int &pixel) {
+  const DeviceScene &sc = c.sc;
+  const BdptVertex &a = eye[e], &b = light[l];
+  float3 org, dir;
+  float len;
+  bv_segment(c, a, b, org, dir, len);
+  float4 fa;
+  float pa_fwd = 0.f, pa_rev = 0.f;
+  pixel = -1;
+  if (a.type == kBvCamera) {
+    const DevCamera &cam = sc.cam;
+    const float z = dot(dir, cam.dir);
+    if (!(z > 0.f)) return false;
+    const float x = dot(dir, cam.right) / z, y = dot(dir, cam.up) / z;
+    const float fx = (x - cam.xmin) / cam.xper, fy = (y - cam.ymin) / cam.yper;
+    const bool trunc = sc.mis_mode != TTGPU_MIS_CONSISTENT;
+    const float ix = trunc ? truncf(fx) : floorf(fx), iy = trunc ? truncf(fy) : floorf(fy);
+    if (!(ix >= 0.f && ix < (float)cam.width && iy >= 0.f && iy < (float)cam.height)) return false;
+    pixel = (int)iy * cam.width + (int)ix;
+    pa_fwd = camera_pdf(cam, x, y);
+    fa = f4(pa_fwd);
+  } else {
+    const DevShader &shd = sc.shaders[sc.materials[a.h.material].shader];
+    fa = bv_corr(a, dir, false) * eval_bsdf(shd, a.h, -a.wi, c.bins, 0.f, c.lambda0, dir, &pa_fwd);
+    pa_rev = bv_scatter_pdf(c, a, dir, -a.wi, false);
+  }
+  float4 fb;
+  float pb_fwd = 0.f, pb_rev = 0.f;
+  if (b.type == kBvSurf) {
+    const DevShader &shd = sc.shaders[sc.materials[b.h.material].shader];
+    fb = bv_corr(b, -dir, true) * eval_bsdf(shd, b.h, -b.wi, c.bins, 0.f, c.lambda0, -dir, &pb_fwd, true);
+    pb_rev = bv_scatter_pdf(c, b, -dir, -b.wi, true);
+  } else {
+    pb_fwd = bv_emission_pdf(c, b, -dir);
+    fb = b.type == kBvArea ? (dot(b.h.ng, -dir) > 0.f ? fabsf(dot(b.h.ng, dir)) : 0.f) * b.Le : b.Le;
+  }
+  const float4 f = fa * fb;
+  if (all_zero(f)) return false;
+  const float G = b.type == kBvEnv ? 1.f : 1.f / (len * len);
+  Ray r;
+  r.org = org;
+  r.dir = dir;
+  r.tmax = len * 0.9999f;
+  HitRec hr;
+  ++rays;
+  if (traverse<true>(sc.nodes, sc.prims, r, lds_stack, hr)) return false;
+  const int N = e + 1 + l + 1;
+  double pe[2 * kBdptMaxVerts], pl[2 * kBdptMaxVerts];
+  bool spec[2 * kBdptMaxVerts];
+  for (int k = 0; k <= e; ++k) {
+    pe[k] = k == 0 ? 1. : (double)eye[k].pdf_walk;
+    pl[k] = eye[k].pdf_other;
+    spec[k] = k < e && eye[k].delta;
+  }
+  for (int j = 0; j <= l; ++j) {
+    const int k = N - 1 - j;
+    pl[k] = light[j].pdf_walk;
+    pe[k] = light[j].pdf_other;
+    spec[k] = j < l && light[j].delta;
+  }
+  pe[e + 1] = (double)pa_fwd * bv_conv(a, b, dir, len);
+  pl[e] = (double)pb_fwd * bv_conv(b, a, -dir, len);
+  if (e >= 1) {
+    float3 o2, d2;
+    float l2;
+    bv_segment(c, a, eye[e - 1], o2, d2, l2);
+    pl[e - 1] = (double)pa_rev * bv_conv(a, eye[e - 1], d2, l2);
+  }
+  if (l >= 1) {
+    float3 o2, d2;
+    float l2;
+    bv_segment(c, b, light[l - 1], o2, d2, l2);
+    pe[e + 2] = (double)pb_rev * bv_conv(b, light[l - 1], d2, l2);
+  }
+  const float w = bv_mis(pe, pl, spec, N, light_delta, e + 1, c.inv_npix);
+  contrib = (w * G) * (a.beta * b.beta * f);
+  return true;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_bdpt(DeviceScene sc, uint64_t first, uint64_t count, uint32_t npix,
+                                                       uint32_t spp_base, uint64_t seed, double *fb, double *fb_sq, uint32_t *stats) {
+  __shared__ int lds_stack[kLdsStack * kTraceBlock];
+  uint32_t rays = 0;
+  for (uint64_t g = first + gtid(); g < first + count; g += grid_stride()) {
+    const uint32_t pixel = (uint32_t)(g % npix);
+    const uint64_t sample = spp_base + g / npix;
+    Pcg rng;
+    rng.seed(seed, sample * npix + pixel);
+    const int main_idx = (int)rng.bounded(kStrata);
+    int idx[4] = {main_idx, main_idx + kStrata, main_idx + 2 * kStrata, main_idx + 3 * kStrata};
+    const int rot = (int)rng.bounded(4);
+    const int t = idx[0]; idx[0] = idx[rot]; idx[rot] = t;
+    const float lambda0 = 370.f + 10.f * idx[0] + 10.f * rng.uniform();
+    rng.uniform(); rng.uniform(); rng.uniform();
+    const uint32_t bins = (uint32_t)idx[0] | ((uint32_t)idx[1] << 8) | ((uint32_t)idx[2] << 16) | ((uint32_t)idx[3] << 24);
+    const BdptCtx c{sc, bins, lambda0, sc.mis_mode == TTGPU_MIS_CONSISTENT, 1.f / (float)npix};
+    const int max_nodes = min(sc.max_depth + 1, kBdptMaxVerts);
+    // eye subpath
+    BdptVertex eye[kBdptMaxVerts];
+    int ne = 1;
+    eye[0].type = kBvCamera;
+    eye[0].h.pos = sc.cam.pos;
+    eye[0].beta = f4((float)kStrata);
+    eye[0].pdf_walk = 1.f;
+    eye[0].pdf_other = 0.f;
+    eye[0].delta = 0;
+    {
+      const int px = pixel % sc.cam.width, py = pixel / sc.cam.width;
+      const float r0 = rng.uniform(), r1 = rng.uniform();
+      float3 dir;
+      float pdf_cam;
+      camera_ray(sc.cam, px, py, r0, r1, dir, &pdf_cam);
+      if (max_nodes > 1) bv_walk(c, eye, ne, sc.cam.pos, dir, pdf_cam, false, f4(1.f), false, rng, lds_stack + threadIdx.x, rays);
+    }
+    // light subpath
+    BdptVertex light[kBdptMaxVerts];
+    int nl = 0;
+    bool light_delta = false;
+    if (bv_sample_light(c, rng, light[0], light_delta)) {
+      light[0].beta = f4(1.f / light[0].pdf_walk);
+      light[0].pdf_other = 0.f;
+      nl = 1;
+      const float r0 = rng.uniform(), r1 = rng.uniform();
+      float3 org, dir;
+      float pdf;
+      float4 factor;
+      if (light[0].type == kBvArea) {
+        const float3 l = to_uniform_hemisphere(r0, r1);
+        const Frame fr = make_frame(light[0].h.ng);
+        dir = fr.to_world(l);
+        pdf = 1.f / (2.f * kPi);
+        factor = (fabsf(dot(light[0].h.ng, dir)) * 2.f * kPi) * light[0].Le;
+        org = offset_origin(light[0].h, dir);
+      } else if (light[0].type == kBvPoint) {
+        dir = to_uniform_sphere(r0, r1);
+        pdf = 1.f / (4.f * kPi);
+        factor = (4.f * kPi) * light[0].Le;
+        org = light[0].h.pos;
+      } else {
+        // EnvLightPointSamplingBeyondScene (light.cxx:201-219): a disc across the scene's box.
+        dir = light[0].h.pos;
+        const float rr = sqrtf(r0), ph = 2.f * kPi * r1;
+        float sn, cs;
+        fast_sincos(ph, &sn, &cs);
+        const Frame fr = make_frame(dir);
+        org = sc.bb_center - dir * (10.f * sc.bb_diag) + fr.to_world(mk(rr * cs, rr * sn, 0.f)) * (0.5f * sc.bb_diag);
+        pdf = bv_emission_pdf(c, light[0], dir);
+        factor = (1.f / pdf) * light[0].Le;
+      }
+      if (max_nodes > 1) bv_walk(c, light, nl, org, dir, pdf, false, factor, true, rng, lds_stack + threadIdx.x, rays);
+    }
+    // strategies
+    float4 acc = f4(0.f);
+    for (int e = 0; e < ne; ++e) {
+      const BdptVertex &ev = eye[e];
+      if (e == 0 || ev.type == kBvSurf) {
+        if (e + 1 < max_nodes) {
+          BdptVertex direct[1];
+          bool dl_delta;
+          if (bv_sample_light(c, rng, direct[0], dl_delta)) {
+            direct[0].beta = f4(1.f / direct[0].pdf_walk);
+            direct[0].pdf_other = 0.f;
+            float4 contrib;
+            int pix;
+            if (bv_connect(c, eye, e, direct, 0, dl_delta, lds_stack + threadIdx.x, rays, contrib, pix)) {
+              if (e == 0) {
+                const float3 rgb = spectral_to_rgb(c.inv_npix * contrib, bins);
+                atomicAdd(fb + 3 * (size_t)pix + 0, (double)rgb.x);
+                atomicAdd(fb + 3 * (size_t)pix + 1, (double)rgb.y);
+                atomicAdd(fb + 3 * (size_t)pix + 2, (double)rgb.z);
+              } else {
+                acc = acc + contrib;
+              }
+            }
+          }
+        }
+        const int lmax = min(nl, max_nodes - e - 1);
+        for (int l = 1; l < lmax; ++l) {
+          float4 contrib;
+          int pix;
+          if (!bv_connect(c, eye, e, light, l, light_delta, lds_stack + threadIdx.x, rays, contrib, pix)) continue;
+          if (e == 0) {
+            const float3 rgb = spectral_to_rgb(c.inv_npix * contrib, bins);
+            atomicAdd(fb + 3 * (size_t)pix + 0, (double)rgb.x);
+            atomicAdd(fb + 3 * (size_t)pix + 1, (double)rgb.y);
+            atomicAdd(fb + 3 * (size_t)pix + 2, (double)rgb.z);
+          } else {
+            acc = acc + contrib;
+          }
+        }
+      }
+      // HandleLightHit (:526-556)
+      if (e > 0) {
+        float4 Le = f4(0.f);
+        float p_start = 0.f, p_emit = 0.f;
+        bool is_light = false;
+        BdptVertex as_light = ev;
+        if (ev.type == kBvEnv) {
+          float pe;
+          env_eval(sc, ev.h.pos, bins, &pe);
+          Le = ev.Le;
+          p_start = pe * sc.type_probs[0];
+          p_emit = bv_emission_pdf(c, ev, ev.h.pos);
+          is_light = true;
+        } else if (ev.type == kBvSurf && sc.materials[ev.h.material].emitter >= 0) {
+          const DevEmitter &em = sc.emitters[sc.materials[ev.h.material].emitter];
+          const bool facing = dot(ev.h.ng, -ev.wi) > 0.f;
+          Le = (em.kind == TTGPU_AREA_UNIFORM && facing) ? take(em.radiance, bins) : f4(0.f);
+          p_start = sc.area_pick_prob / sc.prim_area[ev.h.prim];
+          as_light.type = kBvArea;
+          p_emit = facing ? 1.f / (2.f * kPi) : 0.f;
+          is_light = true;
+        }
+        if (is_light && !all_zero(Le)) {
+          const int N = e + 1;
+          double pe[kBdptMaxVerts], pl[kBdptMaxVerts];
+          bool spec[kBdptMaxVerts];
+          for (int k = 0; k <= e; ++k) {
+            pe[k] = k == 0 ? 1. : (double)eye[k].pdf_walk;
+            pl[k] = eye[k].pdf_other;
+            spec[k] = k < e && eye[k].delta;
+          }
+          pl[e] = p_start;
+          float3 o2, d2;
+          float l2;
+          bv_segment(c, as_light, eye[e - 1], o2, d2, l2);
+          pl[e - 1] = (double)p_emit * bv_conv(as_light, eye[e - 1], d2, l2);
+          const float w = bv_mis(pe, pl, spec, N, false, N, c.inv_npix);
+          acc = acc + w * (ev.beta * Le);
+        }
+      }
+    }
+    const float3 rgb = spectral_to_rgb(acc, bins);
+    atomicAdd(fb + 3 * (size_t)pixel + 0, (double)rgb.x);
+    atomicAdd(fb + 3 * (size_t)pixel + 1, (double)rgb.y);
+    atomicAdd(fb + 3 * (size_t)pixel + 2, (double)rgb.z);
+    if (fb_sq) {
+      atomicAdd(fb_sq + 3 * (size_t)pixel + 0, (double)rgb.x * rgb.x);
+      atomicAdd(fb_sq + 3 * (size_t)pixel + 1, (double)rgb.y * rgb.y);
+      atomicAdd(fb_sq + 3 * (size_t)pixel + 2, (double)rgb.z * rgb.z);
+    }
+  }
+  if (rays) atomicAdd(stats + kStatClosest, rays);
+}
+
 // ---- host-side launchers ------------------------------------------------------------------
 static int g_trace_blocks = 0;
 
@@ -1441,6 +1919,12 @@ void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
   else
     hipLaunchKernelGGL(k_shade<false>, dim3(Blocks(max_count, 256, 8192)), dim3(256), 0, st, sc, q, queue_in, count_in, queue_out, count_out,
                        fb, fb_sq);
+}
+void LaunchBdpt(const DeviceScene &sc, uint64_t first, uint64_t count, uint32_t npix, uint32_t spp_base, uint64_t seed, double *fb,
+                double *fb_sq, uint32_t *stats, hipStream_t st) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_bdpt, dim3(Blocks(count, kTraceBlock, 16384)), dim3(kTraceBlock), 0, st, sc, first, count, npix, spp_base,
+                     seed, fb, fb_sq, stats);
 }
 void LaunchCameraConnect(const DeviceScene &sc, uint64_t first, uint64_t count, uint32_t npix, uint32_t spp_base, uint64_t seed,
                          double *fb, double *fb_sq, uint32_t *stats, hipStream_t st) {

@@ -20,6 +20,9 @@ void LaunchTrace(const DeviceScene &sc, const PathQueues &q, const uint32_t *que
                  hipStream_t st);
 void LaunchShade(const DeviceScene &sc, const PathQueues &q, const uint32_t *queue_in, const uint32_t *count_in, uint32_t *queue_out,
                  uint32_t *count_out, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st);
+constexpr int kBdptMaxVertices = 17;  // BDPT subpath vertices per thread: max_ray_depth <= 16
+void LaunchBdpt(const DeviceScene &sc, uint64_t first, uint64_t count, uint32_t npix, uint32_t spp_base, uint64_t seed, double *fb,
+                double *fb_sq, uint32_t *stats, hipStream_t st);
 void LaunchCameraConnect(const DeviceScene &sc, uint64_t first, uint64_t count, uint32_t npix, uint32_t spp_base, uint64_t seed,
                          double *fb, double *fb_sq, uint32_t *stats, hipStream_t st);
 void LaunchShadow(const DeviceScene &sc, const PathQueues &q, uint32_t max_count, double *fb, double *fb_sq, hipStream_t st);

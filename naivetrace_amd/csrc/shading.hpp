@@ -218,7 +218,10 @@ TT_HD float eta_i_over_t(const DevShader &s, const SurfHit &h, float3 wi, float 
   float ior = s.ior_lambda_coeff != 0.f ? s.ior_ratio + s.ior_lambda_coeff * lambda0 : s.ior_ratio;
   return entering ? 1.f / ior : ior;
 }
-TT_HD Scatter dielectric_sample(const DevShader &s, const SurfHit &h, float3 wi, float rough, float lambda0, Pcg &rng) {
+// importance: transport of importance (light subpaths), without the 1 / eta^2 radiance scaling
+// on refraction (shader.cxx:425, 498, 566).
+TT_HD Scatter dielectric_sample(const DevShader &s, const SurfHit &h, float3 wi, float rough, float lambda0, Pcg &rng,
+                                bool importance = false) {
   float abs_ni = fabsf(dot(h.ns, wi));
   float eta = eta_i_over_t(s, h, wi, lambda0);
   float fr = 1.f;
@@ -245,13 +248,13 @@ TT_HD Scatter dielectric_sample(const DevShader &s, const SurfHit &h, float3 wi,
     val = do_refl ? fr : 1.f - fr;
   }
   val /= fabsf(dot(out.dir, h.ns));
-  if (dot(out.dir, h.n) < 0.f) val *= eta * eta;
+  if (dot(out.dir, h.n) < 0.f && !importance) val *= eta * eta;
   out.value = f4(val);
   if (s.ior_lambda_coeff != 0.f) out.value.y = out.value.z = out.value.w = 0.f;
   return out;
 }
 TT_HD float4 dielectric_eval(const DevShader &s, const SurfHit &h, float3 wi, float rough, float lambda0, float3 wo,
-                                                  float *pdf) {
+                                                  float *pdf, bool importance = false) {
   if (!(rough > 0.f)) { *pdf = 0.f; return f4(0.f); }
   const float open = 1.f - rough;
   float eta = eta_i_over_t(s, h, wi, lambda0);
@@ -269,7 +272,7 @@ TT_HD float4 dielectric_eval(const DevShader &s, const SurfHit &h, float3 wi, fl
     tv += (1.f - fr) * sp;
     tp += (1.f - prob_refl) * sp;
   }
-  if (dot(wo, h.n) < 0.f) tv *= eta * eta;
+  if (dot(wo, h.n) < 0.f && !importance) tv *= eta * eta;
   tv /= fabsf(dot(wo, h.ns));
   *pdf = tp;
   return f4(tv);
@@ -337,7 +340,8 @@ TT_HD Scatter glossy_diel_sample(const DevShader &s, const SurfHit &h, float3 wi
 
 TT_HD float average_projected_schlick(float ks) { return ks + (1.f - ks) * 2.f / 42.f; }
 
-TT_HD Scatter sample_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, Pcg &rng) {
+TT_HD Scatter sample_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, Pcg &rng,
+                          bool importance = false) {
   switch (s.kind) {
     case TTGPU_SHADER_DIFFUSE: return diffuse_sample(s, h, bins, rng);
     case TTGPU_SHADER_MIRROR: {
@@ -348,7 +352,7 @@ TT_HD Scatter sample_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint3
       o.value = dot(h.n, o.dir) < 0.f ? f4(0.f) : (1.f / dot(h.ns, o.dir)) * take(s.reflectance, bins);
       return o;
     }
-    case TTGPU_SHADER_DIELECTRIC: return dielectric_sample(s, h, wi, rough, lambda0, rng);
+    case TTGPU_SHADER_DIELECTRIC: return dielectric_sample(s, h, wi, rough, lambda0, rng, importance);
     case TTGPU_SHADER_BECKMANN_METAL:
     case TTGPU_SHADER_GGX_METAL: return metal_sample(s, h, wi, bins, rough, rng);
     case TTGPU_SHADER_INVISIBLE: {
@@ -384,10 +388,11 @@ TT_HD Scatter sample_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint3
   return o;
 }
 
-TT_HD float4 eval_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, float3 wo, float *pdf) {
+TT_HD float4 eval_bsdf(const DevShader &s, const SurfHit &h, float3 wi, uint32_t bins, float rough, float lambda0, float3 wo, float *pdf,
+                       bool importance = false) {
   switch (s.kind) {
     case TTGPU_SHADER_DIFFUSE: return diffuse_eval(s, h, bins, wo, pdf);
-    case TTGPU_SHADER_DIELECTRIC: return dielectric_eval(s, h, wi, rough, lambda0, wo, pdf);
+    case TTGPU_SHADER_DIELECTRIC: return dielectric_eval(s, h, wi, rough, lambda0, wo, pdf, importance);
     case TTGPU_SHADER_BECKMANN_METAL:
     case TTGPU_SHADER_GGX_METAL: return metal_eval(s, h, wi, bins, rough, wo, pdf);
     case TTGPU_SHADER_GLOSSY_DIELECTRIC: return glossy_diel_eval(s, h, wi, rough, wo, pdf);

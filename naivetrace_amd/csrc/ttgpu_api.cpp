@@ -193,7 +193,16 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (p.max_ray_depth < 2 || p.max_ray_depth > 255) bad("max_ray_depth must be in [2, 255]");
   if (p.num_wavelengths != 0 && p.num_wavelengths != 4) bad("the GPU path traces 4 wavelengths per path");
   if (p.sample_mode < 0 || p.sample_mode > 2) bad("bad sample_mode");
-  if (p.algo != TTGPU_ALGO_PT2 && p.algo != TTGPU_ALGO_PTG && p.algo != TTGPU_ALGO_PT) bad("unknown algorithm");
+  if (p.algo != TTGPU_ALGO_PT2 && p.algo != TTGPU_ALGO_PTG && p.algo != TTGPU_ALGO_PT && p.algo != TTGPU_ALGO_BDPT)
+    bad("unknown algorithm");
+  if (p.algo == TTGPU_ALGO_BDPT) {
+    if (p.max_ray_depth + 1 > kBdptMaxVertices) bad("bdpt supports max_ray_depth <= 16");
+    if (s.camera.kind != TTGPU_CAMERA_PINHOLE) bad("bdpt needs the pinhole camera");
+    for (int i = 0; i < s.num_area_emitters; ++i)
+      if (s.area_emitters[i].kind != TTGPU_AREA_UNIFORM) bad("bdpt supports uniform area emitters only");
+    for (int i = 0; i < s.num_env_lights; ++i)
+      if (s.env_lights[i].kind == TTGPU_ENV_DIRECTIONAL) bad("bdpt does not support directional lights");
+  }
   if (p.guide_kind < TTGPU_GUIDE_NONE || p.guide_kind > TTGPU_GUIDE_QUADTREE) bad("unknown guide kind");
   if (s.num_triangles < 0 || s.num_spheres < 0 || s.num_vertices < 0) bad("negative counts");
   if (s.num_triangles > 0 && (!s.vertices || !s.indices || !s.tri_material)) bad("triangles need vertices, indices, materials");
@@ -224,12 +233,12 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (s.num_media > kMaxMedia) bad("too many media");
   if (s.num_media > 0 && !s.media) bad("media table missing");
   if (s.empty_space_medium >= s.num_media || s.empty_space_medium < -1) bad("empty_space_medium out of range");
-  if (p.algo == TTGPU_ALGO_PTG || p.algo == TTGPU_ALGO_PT) {
+  if (p.algo == TTGPU_ALGO_PTG || p.algo == TTGPU_ALGO_PT || p.algo == TTGPU_ALGO_BDPT) {
     bool any_medium = s.empty_space_medium >= 0;
     for (int i = 0; i < s.num_materials; ++i) any_medium |= s.materials[i].medium >= 0 || s.materials[i].outer_medium >= 0;
     if (any_medium && p.algo == TTGPU_ALGO_PTG)
       bad("the guided integrator guides surface scattering only; scenes with media use pt2");
-    if (any_medium) bad("pt runs on surface scenes (no media); scenes with media use pt2");
+    if (any_medium) bad("pt and bdpt run on surface scenes (no media); scenes with media use pt2");
   }
   for (int i = 0; i < s.num_media; ++i) {
     const auto &m = s.media[i];
@@ -929,8 +938,55 @@ void ResetFramebuffer(ttgpu_handle *h) {
 // the slots of finished paths take the next camera paths, so launches stay full until the pass's
 // last paths drain.  The host runs one iteration ahead of a 16-byte progress read-back.  A stop
 // request ends the issuing of new paths; the paths in flight finish.  Returns false after a stop.
+// bdpt: one thread per camera sample (eye and light subpaths and all their connections), in
+// launches of at most 2^20 samples so that a stop request ends the pass after the launch in flight.
+bool RenderPassBdpt(ttgpu_handle *h, int spp) {
+  const uint64_t npix = (uint64_t)h->npix;
+  const uint64_t total = npix * (uint64_t)spp;
+  const uint32_t spp_base = (uint32_t)h->total_spp;
+  hipEvent_t ev0 = h->NextEvent(), ev1 = h->NextEvent();
+  Check(hipEventRecord(ev0, h->stream), "hipEventRecord");
+  constexpr uint64_t kChunk = 1ull << 20;
+  uint64_t issued = 0;
+  while (issued < total && !h->stop.load()) {
+    const uint64_t n = std::min(kChunk, total - issued);
+    h->Timed("bdpt", [&] { LaunchBdpt(h->dsc, issued, n, (uint32_t)npix, spp_base, h->params.seed, h->fb, h->fb_sq, h->q.counters, h->stream); });
+    Check(hipGetLastError(), "kernel launch");
+    Check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    issued += n;
+    if (h->interrupt.exchange(false) && h->cb) h->cb(h->cb_user, 0);
+  }
+  Check(hipEventRecord(ev1, h->stream), "hipEventRecord");
+  Check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  const bool complete = issued == total;
+  {
+    const uint64_t full = issued / npix, rem = issued % npix;
+    if (full)
+      for (auto &c : h->pixel_spp) c += full;
+    for (uint64_t k = 0; k < rem; ++k) h->pixel_spp[k] += 1;
+  }
+  h->stats.paths_traced += issued;
+  float ms = 0.f;
+  Check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+  h->stats.render_ms = ms;
+  h->CollectTimers();
+  h->event_next = 0;
+  uint32_t counters[kNumCounters];
+  Check(hipMemcpy(counters, h->q.counters, sizeof(counters), hipMemcpyDeviceToHost), "read counters");
+  h->stats.rays_closest = counters[kStatClosest];
+  h->stats.rays_shadow = counters[kStatShadow];
+  if (complete) {
+    h->total_spp += spp;
+    h->sched_total += spp;
+  }
+  h->stats.total_spp = h->total_spp;
+  h->stats.passes += 1;
+  return complete;
+}
+
 bool RenderPass(ttgpu_handle *h, int spp) {
   if (h->stop.load()) return false;
+  if (h->params.algo == TTGPU_ALGO_BDPT) return RenderPassBdpt(h, spp);
   const uint64_t npix = (uint64_t)h->npix;
   const uint64_t total = npix * (uint64_t)spp;
   const uint32_t cap = h->capacity;
@@ -1090,6 +1146,8 @@ int ttgpu_create(const ttgpu_scene *scene, const ttgpu_params *params, ttgpu_han
     double diag = nprim ? std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]))
                         : 1.0;
     BuildTables(h, s, diag);
+    h->dsc.bb_center = nprim ? make_float3((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])), (float)(0.5 * (lo[2] + hi[2])))
+                             : make_float3(0.f, 0.f, 0.f);
     h->guided = params->algo == TTGPU_ALGO_PTG;
     h->learned = h->guided && params->guide_kind == TTGPU_GUIDE_LEARNED;
     if (h->guided) {

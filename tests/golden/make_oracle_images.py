@@ -69,10 +69,23 @@ PT_FIXTURES = {
 }
 
 
+# bdpt (bidirectional path tracing), same batch layout; TTGPU_MIS_CONSISTENT (unbiased Russian
+# roulette placement) and TTGPU_MIS_REFERENCE (the reference's).
+BDPT_FIXTURES = {
+    "cornell32_bdpt": (lambda w, h: configs.cornell_box(w, h), 32, 32, 8, 512, 8, abi.MIS_CONSISTENT, abi.SAMPLE_BOTH),
+    "pointlit_dome32_bdpt": (pointlit_dome, 32, 32, 8, 512, 8, abi.MIS_REFERENCE, abi.SAMPLE_BOTH),
+}
+
+
 def make_pt(name):
-    build, w, h, batches, spp, depth, mis, mode = PT_FIXTURES[name]
+    if name in BDPT_FIXTURES:
+        build, w, h, batches, spp, depth, mis, mode = BDPT_FIXTURES[name]
+        algo = abi.ALGO_BDPT
+    else:
+        build, w, h, batches, spp, depth, mis, mode = PT_FIXTURES[name]
+        algo = abi.ALGO_PT
     scene = build(w, h)
-    p = abi.default_params(w, h, max_depth=depth, seed=7, algo=abi.ALGO_PT)
+    p = abi.default_params(w, h, max_depth=depth, seed=7, algo=algo)
     p.mis_mode = mis
     p.sample_mode = mode
     t = time.time()
@@ -85,12 +98,12 @@ def make_pt(name):
     stderr = means.std(axis=0, ddof=1) / np.sqrt(batches)
     out = os.path.join(HERE, f"{name}_oracle.npz")
     np.savez_compressed(out, mean=mean, stderr=stderr, spp=spp * batches, batches=batches, width=w, height=h,
-                        max_depth=depth, mis_mode=mis, sample_mode=mode, seed=7, scene=name, algo=abi.ALGO_PT)
+                        max_depth=depth, mis_mode=mis, sample_mode=mode, seed=7, scene=name, algo=algo)
     print(f"{name}: {time.time() - t:.1f}s -> {out}")
 
 
 def make(name):
-    if name in PT_FIXTURES:
+    if name in PT_FIXTURES or name in BDPT_FIXTURES:
         return make_pt(name)
     build, w, h, spp, depth, mis, mode = FIXTURES[name]
     scene = build(w, h)
@@ -109,6 +122,6 @@ def make(name):
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(FIXTURES) + list(PT_FIXTURES)
+    names = sys.argv[1:] or list(FIXTURES) + list(PT_FIXTURES) + list(BDPT_FIXTURES)
     for n in names:
         make(n)

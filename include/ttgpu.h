@@ -206,7 +206,11 @@ typedef struct ttgpu_scene {
  * forward path tracer of the random-walk framework (pt, the reference's default
  * --algo): power-heuristic MIS, Russian roulette beyond 5 nodes, light points
  * connected to the camera and splatted to the pixel they project to; surface
- * scenes only. */
+ * scenes only.  BDPT: bidirectional path tracing (bdpt, BdptWorker,
+ * src/renderingalgorithms_pathtracing.hxx:341-700): surface scenes, pinhole
+ * camera, uniform area, point and environment lights (no directional),
+ * max_ray_depth <= 16; mis_mode REFERENCE keeps the reference's Russian-roulette
+ * placement and pixel truncation, CONSISTENT is unbiased. */
 enum ttgpu_algo { TTGPU_ALGO_PT2 = 0, TTGPU_ALGO_PTG = 1, TTGPU_ALGO_PT = 2, TTGPU_ALGO_BDPT = 3 };
 enum ttgpu_sample_mode {     /* --pt-sample-mode */
   TTGPU_SAMPLE_BOTH = 0, TTGPU_SAMPLE_BSDF = 1, TTGPU_SAMPLE_LIGHTS = 2

@@ -4,7 +4,7 @@
 // RenderingAlgo follows src/renderingalgorithms_interface.hxx:13-30 (Run,
 // RequestInterrupt, RequestFullStop, GenerateImage, SetInterruptCallback,
 // InitializeScene, a noexcept(false) destructor); RenderAlgorithmFactory follows
-// src/renderingalgorithms.cxx:27-42 for the keys "pt", "pt2" and "ptg".  Errors the
+// src/renderingalgorithms.cxx:27-42 for the keys "pt", "pt2", "ptg" and "bdpt".  Errors the
 // library reports as status codes are thrown as the exceptions the reference
 // throws: std::invalid_argument for bad scenes and parameters,
 // std::runtime_error otherwise.
@@ -51,7 +51,7 @@ inline void ThrowOnError(int rc, const char *msg) {
   throw std::runtime_error(m);
 }
 
-// pt / pt2 / ptg on the GPU.  The scene description and its arrays are borrowed: they must outlive
+// pt / pt2 / ptg / bdpt on the GPU.  The scene description and its arrays are borrowed: they must outlive
 // InitializeScene (the library copies what it needs to the device there).
 class GpuRenderingAlgo : public RenderingAlgo {
  public:
@@ -113,14 +113,15 @@ class GpuRenderingAlgo : public RenderingAlgo {
 };
 
 // RenderAlgorithmFactory: "pt" (the random-walk forward path tracer, surfaces), "pt2" (forward path
-// tracing with NEE and null-path tracking) and "ptg" (guided; params.guide_kind selects the
-// quadtree, the vMF tree or the learned guide).
+// tracing with NEE and null-path tracking), "ptg" (guided; params.guide_kind selects the quadtree,
+// the vMF tree or the learned guide) and "bdpt" (bidirectional, surfaces).
 inline std::unique_ptr<RenderingAlgo> RenderAlgorithmFactory(const ttgpu_scene &scene, ttgpu_params params,
                                                              const std::string &name) {
   if (name == "pt2") params.algo = TTGPU_ALGO_PT2;
   else if (name == "ptg") params.algo = TTGPU_ALGO_PTG;
   else if (name == "pt") params.algo = TTGPU_ALGO_PT;
-  else throw std::invalid_argument("unknown or unsupported algorithm " + name + "; available: pt, pt2, ptg");
+  else if (name == "bdpt") params.algo = TTGPU_ALGO_BDPT;
+  else throw std::invalid_argument("unknown or unsupported algorithm " + name + "; available: pt, pt2, ptg, bdpt");
   return std::make_unique<GpuRenderingAlgo>(scene, params);
 }
 

@@ -63,7 +63,10 @@ def _pmc_means(csv_path):
     acc = collections.defaultdict(list)
     for row in csv.DictReader(open(csv_path)):
         name = row["Kernel_Name"]
+        # Template instances carry their return type ("void ttgpu::k_shade<false>(...)").
         name = name.split("(")[0]
+        if name.startswith("void "):
+            name = name[len("void "):]
         acc[name].append(float(row["Counter_Value"]))
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 

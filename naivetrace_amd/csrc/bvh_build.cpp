@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 #include <thread>
 
 namespace ttgpu {
@@ -281,6 +282,73 @@ BuiltBvh4 CollapseToBvh4(const BuiltBvh &b2) {
   Collapser c{b2, {}};
   if (b2.num_nodes > 0) c.Emit(0, 0);
   return c.out;
+}
+
+namespace {
+
+// The device's decode of a quantized plane: q * 2^e is exact, so fma and multiply-add agree.
+float DecodePlane(int q, float step, float origin) { return std::fma((float)q, step, origin); }
+
+}  // namespace
+
+std::vector<uint32_t> QuantizeBvh4(const BuiltBvh4 &b4) {
+  std::vector<uint32_t> out(16 * (size_t)b4.num_nodes, 0u);
+  for (int i = 0; i < b4.num_nodes; ++i) {
+    const float *n = &b4.nodes[32 * (size_t)i];
+    int child[4], count[4];
+    std::memcpy(child, n + 24, sizeof(child));
+    std::memcpy(count, n + 28, sizeof(count));
+    bool valid[4];
+    for (int c = 0; c < 4; ++c) valid[c] = !(child[c] < 0 && count[c] == 0);
+    uint32_t *o = &out[16 * (size_t)i];
+    uint32_t rows[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t exps = 0;
+    for (int k = 0; k < 3; ++k) {
+      const float *lo = n + 4 * (2 * k), *hi = n + 4 * (2 * k + 1);
+      float org = kInf, top = -kInf;
+      for (int c = 0; c < 4; ++c)
+        if (valid[c]) { org = std::min(org, lo[c]); top = std::max(top, hi[c]); }
+      if (!(org <= top)) org = top = 0.f;  // no child at all (empty tree)
+      const double ext = (double)top - (double)org;
+      int e = -100;
+      if (ext > 0.) e = std::max(e, (int)std::ceil(std::log2(ext / 255.0)));
+      for (;; ++e) {
+        if (e > 127) throw std::runtime_error("BVH node extent too large for 8-bit planes");
+        const float step = std::ldexp(1.f, e);
+        bool ok = true;
+        uint32_t rlo = 0, rhi = 0;
+        for (int c = 0; c < 4 && ok; ++c) {
+          if (!valid[c]) continue;
+          int ql = (int)std::max(0.0, std::min(255.0, std::floor(((double)lo[c] - org) / step)));
+          while (ql > 0 && DecodePlane(ql, step, org) > lo[c]) --ql;
+          int qh = (int)std::max(0.0, std::min(255.0, std::ceil(((double)hi[c] - org) / step)));
+          while (qh < 255 && DecodePlane(qh, step, org) < hi[c]) ++qh;
+          ok = DecodePlane(ql, step, org) <= lo[c] && DecodePlane(qh, step, org) >= hi[c];
+          rlo |= (uint32_t)ql << (8 * c);
+          rhi |= (uint32_t)qh << (8 * c);
+        }
+        if (!ok) continue;
+        rows[2 * k] = rlo;
+        rows[2 * k + 1] = rhi;
+        break;
+      }
+      std::memcpy(&o[k], &org, 4);
+      exps |= (uint32_t)(e + 127) << (8 * k);
+    }
+    uint32_t counts = 0;
+    for (int c = 0; c < 4; ++c) {
+      if (count[c] < 0 || count[c] > 255) throw std::runtime_error("BVH leaf with more than 255 primitives");
+      counts |= (uint32_t)count[c] << (8 * c);
+    }
+    o[3] = exps;
+    for (int r = 0; r < 4; ++r) o[4 + r] = rows[r];
+    o[8] = rows[4];
+    o[9] = rows[5];
+    o[10] = counts;
+    o[11] = 0;
+    std::memcpy(o + 12, child, sizeof(child));
+  }
+  return out;
 }
 
 }  // namespace ttgpu

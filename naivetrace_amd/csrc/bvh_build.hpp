@@ -33,4 +33,11 @@ struct BuiltBvh4 {
 };
 BuiltBvh4 CollapseToBvh4(const BuiltBvh &b2);
 
+// The same tree with 8-bit child planes (device layout QBvh4Node, 16 words per node): per node and
+// axis a grid origin (the lowest child plane) and a power-of-two step chosen so that 255 steps span
+// the children; lower planes round down and upper planes up, each checked after the decode the
+// device performs, so every decoded child box contains the true one.  Empty slots (leaf, count 0)
+// get q = 0.  Throws if a leaf holds more than 255 primitives or an extent exceeds 255 x 2^127.
+std::vector<uint32_t> QuantizeBvh4(const BuiltBvh4 &b4);
+
 }  // namespace ttgpu

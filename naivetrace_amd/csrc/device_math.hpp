@@ -13,6 +13,8 @@ namespace ttgpu {
 
 TT_HD int as_int(float x) { return __builtin_bit_cast(int, x); }
 TT_HD float as_float(int x) { return __builtin_bit_cast(float, x); }
+TT_HD float as_float(uint32_t x) { return __builtin_bit_cast(float, x); }
+TT_HD uint32_t as_uint(float x) { return __builtin_bit_cast(uint32_t, x); }
 // Fast transcendental instructions on the GPU; libm on the host.
 TT_HD void fast_sincos(float x, float *s, float *c) {
 #if defined(__HIP_DEVICE_COMPILE__)

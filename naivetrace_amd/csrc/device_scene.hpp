@@ -37,6 +37,30 @@ struct alignas(16) Bvh4Node {
   int4 count;   // primitives of a leaf child
 };
 
+// Quantized BVH4 node, 64 bytes (two per 128-byte cache line).  The children's planes are
+// 8-bit offsets on a per-node grid: plane = origin_k + q * 2^(e_k - 127), q in [0, 255], with
+// the host choosing q so that every decoded box contains its child's true box (the product
+// q * 2^e is exact, so the decode rounds once, identically on host and device).
+//   origin : xyz = grid origin (lo corner of the union of the children), w = exponent bytes e_x, e_y, e_z
+//   qxy    : lo.x, hi.x, lo.y, hi.y rows, byte c = child c
+//   qz     : lo.z, hi.z rows, then the four leaf counts (byte c = child c), then 0
+//   child  : >= 0 inner node; < 0 leaf ~first primitive (count 0: empty slot)
+struct alignas(64) QBvh4Node {
+  float4 origin;
+  uint4 qxy;
+  uint4 qz;
+  int4 child;
+};
+
+#ifndef TTGPU_QBVH
+#define TTGPU_QBVH 1
+#endif
+#if TTGPU_QBVH
+using SurfNode = QBvh4Node;
+#else
+using SurfNode = Bvh4Node;
+#endif
+
 constexpr int kStackEntries = 96;                   // traversal stack entries per lane
 // A BVH4 node pushes at most three children, so a tree of depth D needs 3 (D + 1) entries.
 constexpr int kMaxBvh4Depth = kStackEntries / 3 - 1;
@@ -137,7 +161,7 @@ struct QuadCell {
 };
 
 struct DeviceScene {
-  const Bvh4Node *nodes;             // surface BVH
+  const SurfNode *nodes;             // surface BVH (QBvh4Node, or Bvh4Node when built with TTGPU_QBVH=0)
   const PrimRecord *prims;         // leaf order
   const int *prim_material;        // leaf order
   const float4 *prim_normals;      // leaf order, 3 per triangle (vertex normals), or null

@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(kTraceBlock, TTGPU_TRACE_WAVES) k_trace_dyn(De
       r.org = xyz(q.org[i]);
       r.dir = xyz(q.dir[i]);
       r.tmax = 3.0e38f;
-      trav_init(t, r);
+      trav_init(t, r, sc.nodes);
       active = true;
     }
     if (__ballot(active) == 0) break;

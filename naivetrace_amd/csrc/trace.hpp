@@ -151,23 +151,29 @@ TT_HD void box2(const BvhNode &n, const RayPre &p, float tmax, float &tl, float 
 
 // ---- BVH4 traversal ----------------------------------------------------------------------
 // A ray's traversal state; one step processes one node, so the loop below and the persistent
-// kernels with dynamic ray fetch (Aila & Laine 2009) share the same code.
-struct Trav {
+// kernels with dynamic ray fetch (Aila & Laine 2009) share the same code.  `cur` holds the node
+// to process next, already fetched: a step issues the fetch of its successor before it
+// intersects its own leaf primitives, so the two memory latencies overlap.
+template <class Node>
+struct TravT {
   RayPre p;
   float3 org, dir;
   float tmax;
   HitRec hit;
-  int node, sp;
+  int sp;
+  Node cur;
 };
+using Trav = TravT<SurfNode>;
 
-TT_HD void trav_init(Trav &t, const Ray &r) {
+template <class Node>
+TT_HD void trav_init(TravT<Node> &t, const Ray &r, const Node *__restrict__ nodes) {
   t.p = precompute(r);
   t.org = r.org;
   t.dir = r.dir;
   t.tmax = r.tmax;
   t.hit.prim = -1;
-  t.node = 0;
   t.sp = 0;
+  t.cur = nodes[0];
 }
 
 // Slab test of the four children; entry distances, +inf when missed.
@@ -188,42 +194,58 @@ TT_HD void box4(const Bvh4Node &n, const RayPre &p, float tmax, float t[4]) {
   }
 }
 
-// Processes the current node: leaf children are intersected at once, inner children are visited
-// nearest first with the others pushed far to near.  Returns false when the traversal is finished
-// (stack empty, or an occluder found for any-hit queries).  `stack` is this lane's LDS column
-// (stride kTraceBlock), `spill` its private overflow; the host bounds the tree depth so that the
-// kStackEntries entries always suffice.
-template <bool kAnyHit>
-TT_HD bool trav_step(Trav &t, const Bvh4Node *__restrict__ nodes, const PrimRecord *__restrict__ prims, int *stack, int *spill) {
-  const Bvh4Node n = nodes[t.node];
+TT_HD void node_counts(const Bvh4Node &n, int count[4]) {
+  count[0] = n.count.x; count[1] = n.count.y; count[2] = n.count.z; count[3] = n.count.w;
+}
+
+// Quantized node: each plane decodes as fma(q, 2^(e - 127), origin) -- exact product, one
+// rounding, the value the host checked to enclose the child -- and then enters the same slab test
+// as a full-precision plane.
+TT_HD float qplane(uint32_t row, int c, float scale, float origin) {
+  return fmaf((float)((row >> (8 * c)) & 0xffu), scale, origin);
+}
+
+TT_HD void box4(const QBvh4Node &n, const RayPre &p, float tmax, float t[4]) {
+  const uint32_t e = as_uint(n.origin.w);
+  const float sx = as_float((e & 0xffu) << 23), sy = as_float(((e >> 8) & 0xffu) << 23), sz = as_float(((e >> 16) & 0xffu) << 23);
+  const float pad = 1.0000004f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float x0 = qplane(n.qxy.x, c, sx, n.origin.x) * p.inv.x - p.oinv.x, x1 = qplane(n.qxy.y, c, sx, n.origin.x) * p.inv.x - p.oinv.x;
+    const float y0 = qplane(n.qxy.z, c, sy, n.origin.y) * p.inv.y - p.oinv.y, y1 = qplane(n.qxy.w, c, sy, n.origin.y) * p.inv.y - p.oinv.y;
+    const float z0 = qplane(n.qz.x, c, sz, n.origin.z) * p.inv.z - p.oinv.z, z1 = qplane(n.qz.y, c, sz, n.origin.z) * p.inv.z - p.oinv.z;
+    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.f));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * pad;
+    t[c] = tn <= tf ? tn : as_float(0x7f800000);
+  }
+}
+
+TT_HD void node_counts(const QBvh4Node &n, int count[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) count[c] = (int)((n.qz.z >> (8 * c)) & 0xffu);
+}
+
+// Processes the current node: inner children are visited nearest first with the others pushed far
+// to near, and the successor's fetch is issued; then the leaf children are intersected.  Returns
+// false when the traversal is finished (stack empty, or an occluder found for any-hit queries).
+// Inner children are culled against tmax before this node's leaves can shorten it; a child that a
+// leaf hit here puts out of reach fails its own box test when visited.  `stack` is this lane's
+// LDS column (stride kTraceBlock), `spill` its private overflow; the host bounds the tree depth
+// so that the kStackEntries entries always suffice.
+template <bool kAnyHit, class Node>
+TT_HD bool trav_step(TravT<Node> &t, const Node *nodes, const PrimRecord *prims, int *stack, int *spill) {
   float tc[4];
-  box4(n, t.p, t.tmax, tc);
-  const int child[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-  const int count[4] = {n.count.x, n.count.y, n.count.z, n.count.w};
-  Ray r;
-  r.org = t.org;
-  r.dir = t.dir;
+  box4(t.cur, t.p, t.tmax, tc);
+  const int child[4] = {t.cur.child.x, t.cur.child.y, t.cur.child.z, t.cur.child.w};
+  int count[4];
+  node_counts(t.cur, count);
   float key[4];
   int id[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    key[c] = as_float(0x7f800000);
-    id[c] = -1;
-    if (!(tc[c] <= t.tmax)) continue;
-    if (child[c] < 0) {
-      const int first = ~child[c];
-      for (int k = 0; k < count[c]; ++k) {
-        float th, u, v;
-        if (intersect_prim(prims[first + k], t.p, r, t.tmax, th, u, v)) {
-          t.tmax = th;
-          t.hit.t = th; t.hit.u = u; t.hit.v = v; t.hit.prim = first + k;
-          if (kAnyHit) return false;
-        }
-      }
-    } else {
-      key[c] = tc[c];
-      id[c] = child[c];
-    }
+    const bool inner = child[c] >= 0 && tc[c] <= t.tmax;
+    key[c] = inner ? tc[c] : as_float(0x7f800000);
+    id[c] = inner ? child[c] : -1;
   }
   // Sort the (at most four) inner children by entry distance: five compare-exchanges.
 #define TT_CSWAP(a, b)                                                                  \
@@ -236,8 +258,7 @@ TT_HD bool trav_step(Trav &t, const Bvh4Node *__restrict__ nodes, const PrimReco
   int next = -1;
 #pragma unroll
   for (int c = 3; c >= 0; --c) {
-    // Children entered beyond a hit found in this node's leaves are dropped.
-    if (id[c] < 0 || !(key[c] <= t.tmax)) continue;
+    if (id[c] < 0) continue;
     if (next >= 0) {
       if (t.sp < kLdsStack) stack[t.sp * kTraceBlock] = next;
       else if (t.sp < kStackEntries) spill[t.sp - kLdsStack] = next;
@@ -246,22 +267,51 @@ TT_HD bool trav_step(Trav &t, const Bvh4Node *__restrict__ nodes, const PrimReco
     }
     next = id[c];
   }
-  if (next < 0) {
-    if (t.sp == 0) return false;
+  if (next < 0 && t.sp > 0) {
     --t.sp;
     next = t.sp < kLdsStack ? stack[t.sp * kTraceBlock] : spill[t.sp - kLdsStack];
   }
-  t.node = next;
-  return true;
+  // Leaf children of this node, while the successor's fetch is in flight.  The empty asm with a
+  // memory clobber keeps the compiler from sinking the fetch below the leaf loop (it waits for
+  // nothing: the fetched registers are first read at the next step).
+  const Node succ = nodes[next < 0 ? 0 : next];
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" ::: "memory");
+#endif
+  Ray r;
+  r.org = t.org;
+  r.dir = t.dir;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (child[c] >= 0 || !(tc[c] <= t.tmax) || count[c] == 0) continue;
+    const int first = ~child[c];
+    // Whole 48-byte records, the next one fetched while the current one is tested.
+    PrimRecord rec = prims[first];
+    for (int k = 0; k < count[c]; ++k) {
+      const PrimRecord cur = rec;
+      if (k + 1 < count[c]) rec = prims[first + k + 1];
+#if defined(__HIP_DEVICE_COMPILE__)
+      asm volatile("" ::: "memory");
+#endif
+      float th, u, v;
+      if (intersect_prim(cur, t.p, r, t.tmax, th, u, v)) {
+        t.tmax = th;
+        t.hit.t = th; t.hit.u = u; t.hit.v = v; t.hit.prim = first + k;
+        if (kAnyHit) return false;
+      }
+    }
+  }
+  t.cur = succ;
+  return next >= 0;
 }
 
 // Whole traversal of one ray (closest hit, or any hit with kAnyHit); true if something was hit.
-template <bool kAnyHit>
-TT_HD bool traverse(const Bvh4Node *__restrict__ nodes, const PrimRecord *__restrict__ prims, const Ray &r, int *stack,
+template <bool kAnyHit, class Node>
+TT_HD bool traverse(const Node *__restrict__ nodes, const PrimRecord *__restrict__ prims, const Ray &r, int *stack,
                     HitRec &hit) {
   int spill[kSpillStack];
-  Trav t;
-  trav_init(t, r);
+  TravT<Node> t;
+  trav_init(t, r, nodes);
   while (trav_step<kAnyHit>(t, nodes, prims, stack, spill)) {}
   hit = t.hit;
   return t.hit.prim >= 0;

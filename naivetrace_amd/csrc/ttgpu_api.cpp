@@ -351,9 +351,15 @@ void BuildGeometry(ttgpu_handle *h, const ttgpu_scene &s, const std::vector<int6
     const BuiltBvh4 b4 = CollapseToBvh4(bvh);
     if (b4.max_depth > kMaxBvh4Depth)
       throw std::runtime_error("BVH4 deeper than the traversal stack (" + std::to_string(kMaxBvh4Depth) + " levels)");
+#if TTGPU_QBVH
+    const std::vector<uint32_t> qnodes = QuantizeBvh4(b4);
+    h->dsc.nodes = h->Upload((const QBvh4Node *)qnodes.data(), b4.num_nodes);
+    h->stats.scene_bytes += qnodes.size() * 4;
+#else
     h->dsc.nodes = h->Upload((const Bvh4Node *)b4.nodes.data(), b4.num_nodes);
-    h->stats.bvh_nodes = b4.num_nodes;
     h->stats.scene_bytes += b4.nodes.size() * 4;
+#endif
+    h->stats.bvh_nodes = b4.num_nodes;
     h->dsc.prims = h->Upload(recs.data(), recs.size());
     h->dsc.prim_material = h->Upload(mats.data(), mats.size());
     h->dsc.prim_area = h->Upload(areas.data(), areas.size());

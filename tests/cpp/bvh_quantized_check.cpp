@@ -1,0 +1,129 @@
+// Host-side check of the surface BVH in both device layouts, running the traversal code of
+// naivetrace_amd/csrc/trace.hpp (compiled for the host here, the same source the kernels use):
+//   * every quantized child box (QBvh4Node, 8-bit planes) contains the child's true box;
+//   * the quantized tree's closest-hit distance equals a brute-force loop over all primitives, and
+//     its any-hit answer equals whether brute force hits (every ray when the scene is small,
+//     every 16th otherwise), including axis-parallel rays and a scene far from the origin;
+//   * the full-precision tree (Bvh4Node) is compared the same way.
+// Far from the origin the slab test (plane / dir - org / dir, as in Embree's non-robust mode that
+// the reference uses) cancels and loses some grazing hits in both trees; the caller decides what
+// to require.  Equal distances with different primitives are ties on shared edges, not errors.
+// Usage: bvh_quantized_check <grid> <x offset> <rays>; prints a summary; exit status 1 when a
+// decoded box does not contain its child or a node field differs.
+#include "../../naivetrace_amd/csrc/bvh_build.hpp"
+#include "../../naivetrace_amd/csrc/device_scene.hpp"
+#include "../../naivetrace_amd/csrc/trace.hpp"
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+using namespace ttgpu;
+
+int main(int argc, char **argv) {
+  const int grid = argc > 1 ? atoi(argv[1]) : 100;
+  const float off = argc > 2 ? (float)atof(argv[2]) : 0.f;
+  const int rays = argc > 3 ? atoi(argv[3]) : 50000;
+  std::mt19937 rng(7);
+  std::uniform_real_distribution<float> U(0.f, 1.f);
+  // A bumpy heightfield of grid x grid cells plus small random triangles floating above it.
+  std::vector<float3> v0, v1, v2;
+  std::vector<float> h((grid + 1) * (grid + 1));
+  for (int i = 0; i <= grid; ++i)
+    for (int j = 0; j <= grid; ++j) h[i * (grid + 1) + j] = 3.f * sinf(i * 0.13f) * cosf(j * 0.07f) + 0.3f * U(rng);
+  for (int i = 0; i < grid; ++i)
+    for (int j = 0; j < grid; ++j) {
+      const float3 a = mk(i + off, h[i * (grid + 1) + j], j), b = mk(i + 1 + off, h[(i + 1) * (grid + 1) + j], j);
+      const float3 c = mk(i + off, h[i * (grid + 1) + j + 1], j + 1), d = mk(i + 1 + off, h[(i + 1) * (grid + 1) + j + 1], j + 1);
+      v0.push_back(a); v1.push_back(b); v2.push_back(c);
+      v0.push_back(b); v1.push_back(d); v2.push_back(c);
+    }
+  for (int k = 0; k < grid * grid / 2; ++k) {
+    const float3 c = mk(U(rng) * grid + off, 4 + 10 * U(rng), U(rng) * grid);
+    v0.push_back(c); v1.push_back(c + mk(U(rng), U(rng), 0)); v2.push_back(c + mk(0, U(rng), U(rng)));
+  }
+  const int n = (int)v0.size();
+  std::vector<BuildPrim> bp(n);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      const float a = comp(v0[i], k), b = comp(v1[i], k), c = comp(v2[i], k);
+      bp[i].lo[k] = std::min(a, std::min(b, c));
+      bp[i].hi[k] = std::max(a, std::max(b, c));
+    }
+  const BuiltBvh b2 = BuildBvh(bp, 4, 0);
+  const BuiltBvh4 b4 = CollapseToBvh4(b2);
+  const std::vector<uint32_t> q = QuantizeBvh4(b4);
+  const Bvh4Node *fn = (const Bvh4Node *)b4.nodes.data();
+  const QBvh4Node *qn = (const QBvh4Node *)q.data();
+
+  long bad_box = 0, bad_field = 0;
+  double vol_full = 0, vol_q = 0;
+  for (int i = 0; i < b4.num_nodes; ++i) {
+    const float *f = &b4.nodes[32 * (size_t)i];
+    int child[4], count[4];
+    std::memcpy(child, f + 24, 16);
+    std::memcpy(count, f + 28, 16);
+    const uint32_t e = as_uint(qn[i].origin.w);
+    const float s[3] = {as_float((e & 0xffu) << 23), as_float(((e >> 8) & 0xffu) << 23), as_float(((e >> 16) & 0xffu) << 23)};
+    const uint32_t rows[6] = {qn[i].qxy.x, qn[i].qxy.y, qn[i].qxy.z, qn[i].qxy.w, qn[i].qz.x, qn[i].qz.y};
+    const float o[3] = {qn[i].origin.x, qn[i].origin.y, qn[i].origin.z};
+    const int qchild[4] = {qn[i].child.x, qn[i].child.y, qn[i].child.z, qn[i].child.w};
+    int qcount[4];
+    node_counts(qn[i], qcount);
+    for (int c = 0; c < 4; ++c) {
+      if (qchild[c] != child[c] || qcount[c] != count[c]) bad_field++;
+      if (child[c] < 0 && count[c] == 0) continue;
+      double vf = 1, vq = 1;
+      for (int k = 0; k < 3; ++k) {
+        const float lo = qplane(rows[2 * k], c, s[k], o[k]), hi = qplane(rows[2 * k + 1], c, s[k], o[k]);
+        const float tlo = f[4 * (2 * k) + c], thi = f[4 * (2 * k + 1) + c];
+        if (lo > tlo || hi < thi) bad_box++;
+        vf *= thi - tlo + 1e-3;
+        vq *= hi - lo + 1e-3;
+      }
+      vol_full += vf;
+      vol_q += vq;
+    }
+  }
+  std::vector<PrimRecord> recs(n);
+  for (int k = 0; k < n; ++k) {
+    const int p = (int)b2.order[k];
+    recs[k].a = make_float4(v0[p].x, v0[p].y, v0[p].z, 0.f);
+    recs[k].b = make_float4(v1[p].x, v1[p].y, v1[p].z, 0.f);
+    recs[k].c = make_float4(v2[p].x, v2[p].y, v2[p].z, 0.f);
+  }
+  std::vector<int> stack(kLdsStack * kTraceBlock);
+  long qdiff = 0, qany = 0, fdiff = 0, fany = 0, checked = 0, hits = 0;
+  const int every = n <= 2000 ? 1 : 16;
+  for (int r = 0; r < rays; ++r) {
+    Ray ray;
+    ray.org = mk(U(rng) * grid + off, 2 + 15 * U(rng), U(rng) * grid);
+    float3 d = mk(U(rng) - 0.5f, U(rng) - 0.7f, U(rng) - 0.5f);
+    if (r % 7 == 0) d = mk(1.f, 0.f, 0.f);
+    if (r % 11 == 0) d = mk(0.f, -1.f, 0.f);
+    ray.dir = normalize(d);
+    ray.tmax = 3e38f;
+    if (r % every) continue;
+    HitRec hf, hq, af, aq;
+    traverse<false>(fn, recs.data(), ray, stack.data(), hf);
+    traverse<false>(qn, recs.data(), ray, stack.data(), hq);
+    const bool of = traverse<true>(fn, recs.data(), ray, stack.data(), af);
+    const bool oq = traverse<true>(qn, recs.data(), ray, stack.data(), aq);
+    const RayPre pp = precompute(ray);
+    float best = 3e38f;
+    int bi = -1;
+    for (int k = 0; k < n; ++k) {
+      float th, u, v;
+      if (intersect_prim(recs[k], pp, ray, best, th, u, v)) { best = th; bi = k; }
+    }
+    checked++;
+    hits += bi >= 0;
+    qdiff += (hq.prim >= 0) != (bi >= 0) || (bi >= 0 && hq.t != best);
+    fdiff += (hf.prim >= 0) != (bi >= 0) || (bi >= 0 && hf.t != best);
+    qany += oq != (bi >= 0);
+    fany += of != (bi >= 0);
+  }
+  printf("prims %d nodes %d box_errors %ld field_errors %ld inflation %.4f rays %ld hits %ld quantized_closest_diff %ld quantized_anyhit_diff %ld "
+         "full_closest_diff %ld full_anyhit_diff %ld\n",
+         n, b4.num_nodes, bad_box, bad_field, vol_q / vol_full, checked, hits, qdiff, qany, fdiff, fany);
+  return (bad_box || bad_field) ? 1 : 0;
+}

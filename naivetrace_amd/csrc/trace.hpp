@@ -194,6 +194,10 @@ TT_HD void box4(const Bvh4Node &n, const RayPre &p, float tmax, float t[4]) {
   }
 }
 
+// Element c of a four-entry register array without dynamic indexing (selects, no scratch).
+template <class T>
+TT_HD T pick4(const T (&a)[4], int c) { return c == 0 ? a[0] : (c == 1 ? a[1] : (c == 2 ? a[2] : a[3])); }
+
 TT_HD void node_counts(const Bvh4Node &n, int count[4]) {
   count[0] = n.count.x; count[1] = n.count.y; count[2] = n.count.z; count[3] = n.count.w;
 }
@@ -281,24 +285,47 @@ TT_HD bool trav_step(TravT<Node> &t, const Node *nodes, const PrimRecord *prims,
   Ray r;
   r.org = t.org;
   r.dir = t.dir;
+  // One cursor over (leaf child, primitive) of every leaf child the ray enters, the next 48-byte
+  // record fetched while the current one is tested, also across leaf children.
+  uint32_t lmask = 0;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    if (child[c] >= 0 || !(tc[c] <= t.tmax) || count[c] == 0) continue;
-    const int first = ~child[c];
-    // Whole 48-byte records, the next one fetched while the current one is tested.
+  for (int c = 0; c < 4; ++c)
+    if (child[c] < 0 && count[c] > 0 && tc[c] <= t.tmax) lmask |= 1u << c;
+  if (lmask) {
+    int c = __builtin_ctz(lmask);
+    lmask &= lmask - 1;
+    int first = ~pick4(child, c), cnt = pick4(count, c), k = 0;
     PrimRecord rec = prims[first];
-    for (int k = 0; k < count[c]; ++k) {
+    for (;;) {
       const PrimRecord cur = rec;
-      if (k + 1 < count[c]) rec = prims[first + k + 1];
+      const int cur_id = first + k;
+      bool more = true;
+      if (++k < cnt) {
+        rec = prims[first + k];
+      } else {
+        // Next leaf child still within reach of the closest hit so far.
+        while (lmask && !(pick4(tc, __builtin_ctz(lmask)) <= t.tmax)) lmask &= lmask - 1;
+        if (lmask) {
+          c = __builtin_ctz(lmask);
+          lmask &= lmask - 1;
+          first = ~pick4(child, c);
+          cnt = pick4(count, c);
+          k = 0;
+          rec = prims[first];
+        } else {
+          more = false;
+        }
+      }
 #if defined(__HIP_DEVICE_COMPILE__)
       asm volatile("" ::: "memory");
 #endif
       float th, u, v;
       if (intersect_prim(cur, t.p, r, t.tmax, th, u, v)) {
         t.tmax = th;
-        t.hit.t = th; t.hit.u = u; t.hit.v = v; t.hit.prim = first + k;
+        t.hit.t = th; t.hit.u = u; t.hit.v = v; t.hit.prim = cur_id;
         if (kAnyHit) return false;
       }
+      if (!more) break;
     }
   }
   t.cur = succ;

@@ -161,7 +161,13 @@ struct TravT {
   float tmax;
   HitRec hit;
   int sp;
-  Node cur;
+  Node cur;             // next node to visit, already fetched (valid when has_node)
+  bool has_node;
+  int leaf_first;       // primitive to test next (valid while leaf_left > 0)
+  int leaf_left;        // primitives left in the current leaf
+  int pend[3];          // further leaves of the same node: first << 3 | count
+  int npend;
+  PrimRecord rec;       // the record of leaf_first, already fetched
 };
 using Trav = TravT<SurfNode>;
 
@@ -174,6 +180,9 @@ TT_HD void trav_init(TravT<Node> &t, const Ray &r, const Node *__restrict__ node
   t.hit.prim = -1;
   t.sp = 0;
   t.cur = nodes[0];
+  t.has_node = true;
+  t.leaf_left = 0;
+  t.npend = 0;
 }
 
 // Slab test of the four children; entry distances, +inf when missed.
@@ -193,10 +202,6 @@ TT_HD void box4(const Bvh4Node &n, const RayPre &p, float tmax, float t[4]) {
     t[c] = tn <= tf ? tn : as_float(0x7f800000);
   }
 }
-
-// Element c of a four-entry register array without dynamic indexing (selects, no scratch).
-template <class T>
-TT_HD T pick4(const T (&a)[4], int c) { return c == 0 ? a[0] : (c == 1 ? a[1] : (c == 2 ? a[2] : a[3])); }
 
 TT_HD void node_counts(const Bvh4Node &n, int count[4]) {
   count[0] = n.count.x; count[1] = n.count.y; count[2] = n.count.z; count[3] = n.count.w;
@@ -229,15 +234,52 @@ TT_HD void node_counts(const QBvh4Node &n, int count[4]) {
   for (int c = 0; c < 4; ++c) count[c] = (int)((n.qz.z >> (8 * c)) & 0xffu);
 }
 
-// Processes the current node: inner children are visited nearest first with the others pushed far
-// to near, and the successor's fetch is issued; then the leaf children are intersected.  Returns
-// false when the traversal is finished (stack empty, or an occluder found for any-hit queries).
-// Inner children are culled against tmax before this node's leaves can shorten it; a child that a
-// leaf hit here puts out of reach fails its own box test when visited.  `stack` is this lane's
-// LDS column (stride kTraceBlock), `spill` its private overflow; the host bounds the tree depth
-// so that the kStackEntries entries always suffice.
+// One iteration of a ray's traversal: either one node step or one primitive test, so the lanes of
+// a wave that are testing primitives and those that are visiting nodes advance together instead of
+// the whole wave waiting for the lane with the most primitives in its node's leaves.
+//   node step: slab test of the four children of the (prefetched) node; inner children ordered by
+//     entry distance with a five-comparator network, the nearest visited next and the others pushed
+//     far to near; the successor's fetch is issued; the leaves the ray enters are queued (the first
+//     one's record fetched) to be tested one primitive per iteration.
+//   primitive step: tests the fetched record and fetches the next one of the queued leaves.
+// Returns false when the traversal is finished (no queued primitive and no node left, or an
+// occluder found for any-hit queries).  Queued leaves are not culled against a hit found after
+// they were queued: the primitive test rejects what lies beyond it.  `stack` is this lane's LDS
+// column (stride kTraceBlock), `spill` its private overflow; the host bounds the tree depth so
+// that the kStackEntries entries always suffice; leaves hold at most 7 primitives (the host
+// builds leaves of at most 4) and primitive indices stay below 2^28.
 template <bool kAnyHit, class Node>
 TT_HD bool trav_step(TravT<Node> &t, const Node *nodes, const PrimRecord *prims, int *stack, int *spill) {
+  if (t.leaf_left > 0) {
+    const PrimRecord cur = t.rec;
+    const int cur_id = t.leaf_first;
+    ++t.leaf_first;
+    --t.leaf_left;
+    if (t.leaf_left == 0 && t.npend > 0) {
+      const int e = t.pend[0];
+      t.pend[0] = t.pend[1];
+      t.pend[1] = t.pend[2];
+      --t.npend;
+      t.leaf_first = e >> 3;
+      t.leaf_left = e & 7;
+    }
+    if (t.leaf_left > 0) t.rec = prims[t.leaf_first];
+#if defined(__HIP_DEVICE_COMPILE__)
+    // Keeps the compiler from sinking the fetch below the test it should overlap.
+    asm volatile("" ::: "memory");
+#endif
+    Ray r;
+    r.org = t.org;
+    r.dir = t.dir;
+    float th, u, v;
+    if (intersect_prim(cur, t.p, r, t.tmax, th, u, v)) {
+      t.tmax = th;
+      t.hit.t = th; t.hit.u = u; t.hit.v = v; t.hit.prim = cur_id;
+      if (kAnyHit) return false;
+    }
+    return t.leaf_left > 0 || t.has_node;
+  }
+  if (!t.has_node) return false;
   float tc[4];
   box4(t.cur, t.p, t.tmax, tc);
   const int child[4] = {t.cur.child.x, t.cur.child.y, t.cur.child.z, t.cur.child.w};
@@ -275,61 +317,23 @@ TT_HD bool trav_step(TravT<Node> &t, const Node *nodes, const PrimRecord *prims,
     --t.sp;
     next = t.sp < kLdsStack ? stack[t.sp * kTraceBlock] : spill[t.sp - kLdsStack];
   }
-  // Leaf children of this node, while the successor's fetch is in flight.  The empty asm with a
-  // memory clobber keeps the compiler from sinking the fetch below the leaf loop (it waits for
-  // nothing: the fetched registers are first read at the next step).
-  const Node succ = nodes[next < 0 ? 0 : next];
-#if defined(__HIP_DEVICE_COMPILE__)
-  asm volatile("" ::: "memory");
-#endif
-  Ray r;
-  r.org = t.org;
-  r.dir = t.dir;
-  // One cursor over (leaf child, primitive) of every leaf child the ray enters, the next 48-byte
-  // record fetched while the current one is tested, also across leaf children.
-  uint32_t lmask = 0;
+  // Queue the leaves the ray enters, in child order.
+  int nl = 0;
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
-    if (child[c] < 0 && count[c] > 0 && tc[c] <= t.tmax) lmask |= 1u << c;
-  if (lmask) {
-    int c = __builtin_ctz(lmask);
-    lmask &= lmask - 1;
-    int first = ~pick4(child, c), cnt = pick4(count, c), k = 0;
-    PrimRecord rec = prims[first];
-    for (;;) {
-      const PrimRecord cur = rec;
-      const int cur_id = first + k;
-      bool more = true;
-      if (++k < cnt) {
-        rec = prims[first + k];
-      } else {
-        // Next leaf child still within reach of the closest hit so far.
-        while (lmask && !(pick4(tc, __builtin_ctz(lmask)) <= t.tmax)) lmask &= lmask - 1;
-        if (lmask) {
-          c = __builtin_ctz(lmask);
-          lmask &= lmask - 1;
-          first = ~pick4(child, c);
-          cnt = pick4(count, c);
-          k = 0;
-          rec = prims[first];
-        } else {
-          more = false;
-        }
-      }
-#if defined(__HIP_DEVICE_COMPILE__)
-      asm volatile("" ::: "memory");
-#endif
-      float th, u, v;
-      if (intersect_prim(cur, t.p, r, t.tmax, th, u, v)) {
-        t.tmax = th;
-        t.hit.t = th; t.hit.u = u; t.hit.v = v; t.hit.prim = cur_id;
-        if (kAnyHit) return false;
-      }
-      if (!more) break;
-    }
+  for (int c = 0; c < 4; ++c) {
+    if (child[c] >= 0 || count[c] == 0 || !(tc[c] <= t.tmax)) continue;
+    const int e = (~child[c] << 3) | count[c];
+    if (nl == 0) { t.leaf_first = ~child[c]; t.leaf_left = count[c]; }
+    else if (nl == 1) t.pend[0] = e;
+    else if (nl == 2) t.pend[1] = e;
+    else t.pend[2] = e;
+    ++nl;
   }
-  t.cur = succ;
-  return next >= 0;
+  t.npend = nl > 1 ? nl - 1 : 0;
+  if (nl > 0) t.rec = prims[t.leaf_first];
+  t.has_node = next >= 0;
+  t.cur = nodes[next < 0 ? 0 : next];
+  return t.has_node || t.leaf_left > 0;
 }
 
 // Whole traversal of one ray (closest hit, or any hit with kAnyHit); true if something was hit.

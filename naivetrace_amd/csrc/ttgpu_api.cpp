@@ -207,7 +207,8 @@ void Validate(const ttgpu_scene &s, const ttgpu_params &p) {
   if (s.num_triangles < 0 || s.num_spheres < 0 || s.num_vertices < 0) bad("negative counts");
   if (s.num_triangles > 0 && (!s.vertices || !s.indices || !s.tri_material)) bad("triangles need vertices, indices, materials");
   if (s.num_spheres > 0 && (!s.spheres || !s.sphere_material)) bad("spheres need data and materials");
-  if (s.num_triangles + s.num_spheres >= ((int64_t)1 << 31) - 1) bad("too many primitives");
+  // Traversal queues leaves as first << 3 | count in an int.
+  if (s.num_triangles + s.num_spheres >= ((int64_t)1 << 28)) bad("too many primitives (at most 2^28 - 1)");
   for (int64_t i = 0; i < 3 * s.num_triangles; ++i)
     if (s.indices[i] >= (uint64_t)s.num_vertices) bad("triangle vertex index out of range");
   auto check_mat = [&](int m) {

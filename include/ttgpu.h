@@ -179,7 +179,7 @@ typedef struct ttgpu_scene {
   int64_t num_vertices;
   const float *vertices;        /* 3 x num_vertices */
   const float *normals;         /* 3 x num_vertices or NULL (flat shading) */
-  int64_t num_triangles;
+  int64_t num_triangles;        /* num_triangles + num_spheres < 2^28 */
   const uint32_t *indices;      /* 3 x num_triangles */
   const int32_t *tri_material;  /* num_triangles */
   int64_t num_spheres;

@@ -552,6 +552,36 @@ __device__ __forceinline__ void block_sort_by_class(const DeviceScene &sc, const
   __syncthreads();  // the next iteration rewrites s_idx and s_cnt
 }
 
+// Block-aggregated appends to two queues: per-wave counts through LDS, one atomic per queue and
+// block (instead of one per wave), block-wide prefix over the waves.  Reuses the sort's class
+// counters (free once the sort has finished); every thread of the block must call it.
+template <int kWaves>
+__device__ __forceinline__ void block_append2(uint32_t *ca, bool pa, uint32_t *cb, bool pb, uint32_t (*s_cnt)[kSortClasses],
+                                              uint32_t &slot_a, uint32_t &slot_b) {
+  static_assert(kSortClasses >= 4, "two counts and two bases per wave row");
+  const uint64_t ma = __ballot(pa), mb = __ballot(pb);
+  const int lane = __lane_id(), wave = (int)(threadIdx.x >> 6);
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (lane == 0) {
+    s_cnt[wave][0] = (uint32_t)__popcll(ma);
+    s_cnt[wave][1] = (uint32_t)__popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ta = 0, tb = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) { ta += s_cnt[w][0]; tb += s_cnt[w][1]; }
+    s_cnt[0][2] = ta ? atomicAdd(ca, ta) : 0u;
+    s_cnt[0][3] = tb ? atomicAdd(cb, tb) : 0u;
+  }
+  __syncthreads();
+  uint32_t oa = s_cnt[0][2], ob = s_cnt[0][3];
+  for (int w = 0; w < wave; ++w) { oa += s_cnt[w][0]; ob += s_cnt[w][1]; }
+  slot_a = oa + (uint32_t)__popcll(ma & below);
+  slot_b = ob + (uint32_t)__popcll(mb & below);
+  __syncthreads();  // the next iteration rewrites s_cnt
+}
+
 __device__ __forceinline__ float mis_weight(bool power, float a, float b) {
   return power ? a * a / (a * a + b * b) : a / (a + b);
 }
@@ -682,9 +712,9 @@ __global__ void __launch_bounds__(256, TTGPU_SHADE_WAVES) k_shade(DeviceScene sc
         store_rng(q, i, rng);
       }
     }
-    const uint32_t slot = wave_append(count_out, push);
+    uint32_t slot, sslot;
+    block_append2<4>(count_out, push, q.counters + kCountShadow, push_shadow, s_cnt, slot, sslot);
     if (push) queue_out[slot] = i;
-    const uint32_t sslot = wave_append(q.counters + kCountShadow, push_shadow);
     if (push_shadow) q.shadow_queue[sslot] = i;
   }
 }
@@ -926,9 +956,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_shade_media(DeviceScene sc, Pat
         store_rng(q, i, rng);
       }
     }
-    const uint32_t slot = wave_append(count_out, push);
+    uint32_t slot, sslot;
+    block_append2<kTraceBlock / 64>(count_out, push, q.counters + kCountShadow, push_shadow, s_cnt, slot, sslot);
     if (push) queue_out[slot] = i;
-    const uint32_t sslot = wave_append(q.counters + kCountShadow, push_shadow);
     if (push_shadow) q.shadow_queue[sslot] = i;
   }
 }
@@ -1161,9 +1191,9 @@ __global__ void __launch_bounds__(256) k_shade_guided(DeviceScene sc, PathQueues
         store_rng(q, i, rng);
       }
     }
-    const uint32_t slot = wave_append(count_out, push);
+    uint32_t slot, sslot;
+    block_append2<4>(count_out, push, q.counters + kCountShadow, push_shadow, s_cnt, slot, sslot);
     if (push) queue_out[slot] = i;
-    const uint32_t sslot = wave_append(q.counters + kCountShadow, push_shadow);
     if (push_shadow) q.shadow_queue[sslot] = i;
   }
 }

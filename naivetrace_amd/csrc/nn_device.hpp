@@ -82,8 +82,10 @@ __device__ __forceinline__ void dense_bwd(const __bf16 *__restrict__ WN, const _
 // the layer inputs X0, H1, H2, H3 for the backward pass; the 96 raw outputs of record `rec` of the
 // tile land in out_lds[rec * kPitchOut32 + o] (fp32).
 constexpr int kPitchOut32 = 97;
+// out_pitch / out_cols: row pitch of out_lds in floats and the outputs stored (the first
+// out_cols of kOut; inference keeps only the 80 mixture parameters).
 __device__ inline void forward_tile(const NetView &net, __bf16 *x, __bf16 *h0, __bf16 *h1, float *out_lds, __bf16 *acts,
-                                    int64_t n_pad, int64_t tile_base) {
+                                    int64_t n_pad, int64_t tile_base, int out_pitch = kPitchOut32, int out_cols = kOut) {
   const int l = threadIdx.x, lr = l & 15, lo = (l >> 4) * 4;
   const __bf16 *W[3] = {net.p + net.L.w1, net.p + net.L.w2, net.p + net.L.w3};
   const float *B[3] = {net.pf + net.L.b1, net.pf + net.L.b2, net.pf + net.L.b3};
@@ -111,18 +113,24 @@ __device__ inline void forward_tile(const NetView &net, __bf16 *x, __bf16 *h0, _
         }
     __syncthreads();
   }
-  floatx4 acc[kOut / 16][4];
-  dense_fwd<kOut>(net.p + net.L.w4, bufs[3], acc);
+  // Output layer in two halves of kOut / 2 rows (half the accumulator registers live at once).
+  // The last hidden layer's activations (bufs[3]) are read into registers before any output is
+  // written, so out_lds may share storage with x and h1 (not with h0 = bufs[3] across halves).
   const float *b4 = net.pf + net.L.b4;
 #pragma unroll
-  for (int og = 0; og < kOut / 16; ++og)
+  for (int half = 0; half < 2; ++half) {
+    floatx4 acc[kOut / 32][4];
+    dense_fwd<kOut / 2>(net.p + net.L.w4 + half * (kOut / 2) * kHid, bufs[3], acc);
 #pragma unroll
-    for (int rg = 0; rg < 4; ++rg)
+    for (int og = 0; og < kOut / 32; ++og)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int o = og * 16 + lo + j, rec = rg * 16 + lr;
-        out_lds[rec * kPitchOut32 + o] = acc[og][rg][j] + b4[o];
-      }
+      for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int o = half * (kOut / 2) + og * 16 + lo + j, rec = rg * 16 + lr;
+          if (o < out_cols) out_lds[rec * out_pitch + o] = acc[og][rg][j] + b4[o];
+        }
+  }
   __syncthreads();
 }
 

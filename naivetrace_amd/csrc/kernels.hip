@@ -408,9 +408,10 @@ __device__ __forceinline__ SurfHit make_hit(const DeviceScene &sc, int prim, flo
   return h;
 }
 
-// Adds a finished path's radiance to its pixel and hands its slot to path regeneration.
+// Adds a finished path's radiance to its pixel and hands its slot to path regeneration (one
+// atomic per wave on the free-slot counter: the lanes finishing here share it).
 __device__ __forceinline__ void finalize_path(const PathQueues &q, double *fb, double *fb_sq, uint32_t i, float4 rad) {
-  q.free_slots[atomicAdd(q.counters + kCountFree, 1u)] = i;
+  q.free_slots[wave_append(q.counters + kCountFree, true)] = i;
   const uint32_t p = q.pixel[i];
   atomicAdd(fb + 3 * (size_t)p + 0, (double)rad.x);
   atomicAdd(fb + 3 * (size_t)p + 1, (double)rad.y);
@@ -1030,7 +1031,7 @@ __device__ __forceinline__ void emit_records(const PathQueues &q, uint32_t i, in
     const float4 pp = q.gv_pos_pdf[(size_t)j * cap + i];
     if (!(pp.w > 0.f)) continue;
     const float4 da = q.gv_dir_acc[(size_t)j * cap + i];
-    const uint32_t slot = atomicAdd(q.counters + kCountRecords, 1u);
+    const uint32_t slot = wave_append(q.counters + kCountRecords, true);
     if (slot < q.rec_capacity) {
       const float w = da.w / pp.w;
       q.rec_pos_w[slot] = make_float4(pp.x, pp.y, pp.z, w);

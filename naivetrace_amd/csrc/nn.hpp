@@ -80,8 +80,19 @@ struct NetView {
 };
 
 // Hash of integer grid coordinates (Teschner et al. 2003 primes), or dense index when the level fits.
+// Dense levels have dim^3 <= table size <= 2^24, so every product below fits 24 bits: 24-bit
+// multiplies are exact there (and keep the GPU compiler from a 64-bit multiply-add whose unused
+// upper addend would wait on an unrelated pending load).
+__host__ __device__ inline uint32_t mul24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul24(a, b);
+#else
+  return a * b;
+#endif
+}
+
 __host__ __device__ inline uint32_t grid_index(uint32_t x, uint32_t y, uint32_t z, uint32_t dim, bool dense, uint32_t mask) {
-  if (dense) return (x + dim * (y + dim * z)) & mask;
+  if (dense) return (x + mul24(dim, y + mul24(dim, z))) & mask;
   return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & mask;
 }
 

@@ -22,20 +22,45 @@ __device__ inline void encode_row(const NetView &net, float3 pos, float3 n, floa
   const float p01[3] = {(pos.x - net.bb_lo[0]) * net.bb_inv[0], (pos.y - net.bb_lo[1]) * net.bb_inv[1],
                         (pos.z - net.bb_lo[2]) * net.bb_inv[2]};
   const int64_t T = (int64_t)1 << net.L.log2_table;
-  for (int lv = 0; lv < net.L.levels; ++lv) {
-    uint32_t idx[8];
-    float w[8];
-    grid_corners(p01, net.res[lv], net.L.log2_table, idx, w);
-    const uint32_t *g = reinterpret_cast<const uint32_t *>(net.p + net.L.grid + (int64_t)lv * T * kFeat);
-    float f0 = 0.f, f1 = 0.f;
+  // Levels in groups of four: the 32 table reads of a group (random entries of tables far larger
+  // than the caches) are all issued before the first one is used, so they cost about one memory
+  // latency per group instead of one per level.  The empty asm with a memory clobber keeps the
+  // compiler from sinking the reads back into the sums.
+  constexpr int kGroup = 4;
+  for (int lv0 = 0; lv0 < net.L.levels; lv0 += kGroup) {
+    uint32_t v[kGroup][8];
+    float w[kGroup][8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const uint32_t v = g[idx[k]];  // two bf16 features of the entry
-      f0 += w[k] * __uint_as_float(v << 16);
-      f1 += w[k] * __uint_as_float(v & 0xFFFF0000u);
+    for (int j = 0; j < kGroup; ++j) {
+      const int lv = lv0 + j;
+      uint32_t idx[8];
+      if (lv < net.L.levels) {
+        grid_corners(p01, net.res[lv], net.L.log2_table, idx, w[j]);
+        const uint32_t *g = reinterpret_cast<const uint32_t *>(net.p + net.L.grid + (int64_t)lv * T * kFeat);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[j][k] = g[idx[k]];  // two bf16 features of the entry
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { v[j][k] = 0u; w[j][k] = 0.f; }
+      }
     }
-    row[2 * lv] = (__bf16)f0;
-    row[2 * lv + 1] = (__bf16)f1;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::: "memory");
+#endif
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      const int lv = lv0 + j;
+      float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        f0 += w[j][k] * __uint_as_float(v[j][k] << 16);
+        f1 += w[j][k] * __uint_as_float(v[j][k] & 0xFFFF0000u);
+      }
+      if (lv < net.L.levels) {
+        row[2 * lv] = (__bf16)f0;
+        row[2 * lv + 1] = (__bf16)f1;
+      }
+    }
   }
   for (int f = 2 * net.L.levels; f < 32; ++f) row[f] = (__bf16)0.f;
   row[32] = (__bf16)n.x; row[33] = (__bf16)n.y; row[34] = (__bf16)n.z;

@@ -150,10 +150,10 @@ TT_HD void box2(const BvhNode &n, const RayPre &p, float tmax, float &tl, float 
 }
 
 // ---- BVH4 traversal ----------------------------------------------------------------------
-// A ray's traversal state; one step processes one node, so the loop below and the persistent
-// kernels with dynamic ray fetch (Aila & Laine 2009) share the same code.  `cur` holds the node
-// to process next, already fetched: a step issues the fetch of its successor before it
-// intersects its own leaf primitives, so the two memory latencies overlap.
+// A ray's traversal state; one step is one node visit or one primitive test, so the loop below
+// and the persistent kernels with dynamic ray fetch (Aila & Laine 2009) share the same code.
+// `cur` holds the node to visit next and `rec` the primitive to test next, both already fetched:
+// each step issues the fetch its successor step needs, so memory latency overlaps the tests.
 template <class Node>
 struct TravT {
   RayPre p;
